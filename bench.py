@@ -1,0 +1,224 @@
+"""Benchmark: plain GPU PatchMatch Mpix/s on the single-view workload.
+
+Workload (one "step" = one full ComputePatchMatch of the reference view):
+1600x1200 uint8 reference view + 8 neighbours rendered from the seeded
+scene generator (20 boxes on a sinusoidal terrain of amplitude 1.5 and
+period 6), depth range [1, 40], patch 7x7, min patch SD 1, 5 iterations,
+FP32 samples and FP32 NCC.  Inputs are resident in HBM when the timed region
+starts (views uploaded by add_view); each step runs random init, 5 red/black
+iterations and the median post-process into device-resident output maps.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the red/black
+PatchMatch half-iteration), timed with HIP events on the stream it runs on;
+its algorithmic work is 28 flop per bilinear NCC sample plus 160 flop of
+setup per scoring, over the scorings the kernel actually executed (a device
+counter).  `cpu_baseline` times the oracle (the C restatement of the
+reference's CPU ComputePatchMatch) on bounded crops of the same scene, one
+crop per host thread, before the GPU is touched.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 vector (published)
+FLOP_PER_SAMPLE = 28
+FLOP_PER_SCORING = 160
+
+WORKLOAD = dict(width=1600, height=1200, neighbours=8, boxes=20,
+                terrain=(1.5, 6.0), min_depth=1.0, max_depth=40.0, patch=7,
+                min_patch_sd=1.0, iterations=5, seed=2000)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_scene():
+    from opensfm_amd import scenes
+    w = WORKLOAD
+    return scenes.scene_config2(seed=w["seed"], width=w["width"], height=w["height"])
+
+
+def crop_views(scene, x0, y0, cw, ch):
+    """Reference view cropped to (x0, y0, cw, ch) with K shifted; neighbours
+    unchanged, so the per-pixel work equals the full view's."""
+    views = []
+    for k, v in enumerate(scene.views):
+        if k == 0:
+            K = v.K.copy()
+            K[0, 2] -= x0
+            K[1, 2] -= y0
+            img = np.ascontiguousarray(v.image[y0:y0 + ch, x0:x0 + cw])
+            views.append((K, v.R, v.t, img, np.full_like(img, 255)))
+        else:
+            views.append((v.K, v.R, v.t, v.image, v.mask))
+    return views
+
+
+def cpu_baseline(scene, threads, target_s=12.0):
+    """Oracle ComputePatchMatch on crops of the reference view, one crop per
+    thread (ctypes releases the GIL)."""
+    import oracle
+    w = WORKLOAD
+
+    def run(x0, y0, cw, ch, seed):
+        est = oracle.OracleEstimator()
+        for K, R, t, img, mask in crop_views(scene, x0, y0, cw, ch):
+            est.add_view(K, R, t, img, mask)
+        est.set_depth_range(w["min_depth"], w["max_depth"], 50)
+        est.set_patch_size(w["patch"])
+        est.set_min_patch_sd(w["min_patch_sd"])
+        est.set_patchmatch_iterations(w["iterations"])
+        est.set_seed(seed)
+        t0 = time.perf_counter()
+        est.compute_patch_match()
+        return time.perf_counter() - t0
+
+    # calibrate on a small crop, then size one crop per thread for target_s
+    t_small = run(700, 500, 40, 40, 1)
+    per_px = t_small / ((40 - 6) * (40 - 6))
+    side = int(np.clip(np.sqrt(target_s / per_px), 40, 400))
+    rng = np.random.default_rng(0)
+    jobs = [(int(rng.integers(0, w["width"] - side)), int(rng.integers(0, w["height"] - side)))
+            for _ in range(threads)]
+    res = [None] * threads
+
+    def worker(k):
+        res[k] = run(jobs[k][0], jobs[k][1], side, side, 100 + k)
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker, args=(k,)) for k in range(threads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    wall = time.perf_counter() - t0
+    mpix = threads * side * side / wall / 1e6
+    return {
+        "value": mpix, "unit": "Mpix/s", "cores": threads, "kind": "port",
+        "sample": (f"{threads} concurrent {side}x{side} crops of the 1600x1200 reference "
+                   f"view (8 neighbours, 5 iterations, patch 7), one per host thread; "
+                   f"1-thread rate {side * side / np.mean(res) / 1e6:.5f} Mpix/s"),
+        "wall_s": wall,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the depthmap path shards by reference view, one process per GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    w = WORKLOAD
+    log("rendering the 1600x1200 scene (9 views)")
+    scene = make_scene()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or max(1, min(16, len(os.sched_getaffinity(0))))
+        log(f"CPU baseline: oracle PatchMatch on {threads} threads")
+        cpu = cpu_baseline(scene, threads)
+        log(f"CPU baseline {cpu['value']:.4f} Mpix/s ({cpu['wall_s']:.1f} s)")
+
+    from opensfm_amd import DepthmapEstimator, scenes
+    from opensfm_amd._capi import PM_MODE_PATCHMATCH
+
+    est = DepthmapEstimator()
+    for v in scene.views:
+        est.add_view(v.K, v.R, v.t, v.image, v.mask)
+    est.set_depth_range(w["min_depth"], w["max_depth"], 64)
+    est.set_patch_size(w["patch"])
+    est.set_min_patch_sd(w["min_patch_sd"])
+    est.set_patchmatch_iterations(w["iterations"])
+    est.set_seed(w["seed"])
+
+    for k in range(args.warmup):
+        log(f"warmup {k + 1}/{args.warmup}")
+        est.run(PM_MODE_PATCHMATCH)
+    dom_ms, dom_launches, dom_scorings = 0.0, 0, 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        est.run(PM_MODE_PATCHMATCH)  # synchronous: returns after the stream drains
+        st = est.stats()
+        dom_ms += st["dominant_ms"]
+        dom_launches += st["launches"]
+        dom_scorings += st["dominant_scorings"]
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {elapsed:.3f} s")
+    depth, plane, score, nghbr = est.download()
+
+    # PCIe-inclusive rate: compute + download of the four maps
+    t1 = time.perf_counter()
+    est.compute_patch_match()
+    pcie_s = time.perf_counter() - t1
+
+    npx = w["width"] * w["height"]
+    ms_per_step = elapsed / args.steps * 1e3
+    value = npx * args.steps / elapsed / 1e6
+    acc = scenes.depth_accuracy(depth, scene.ref.depth, w["min_depth"], w["max_depth"])
+    mean_score = float(score[depth > 0].mean()) if (depth > 0).any() else 0.0
+
+    p2 = w["patch"] ** 2
+    flop_per_launch = dom_scorings / max(1, dom_launches) * (FLOP_PER_SAMPLE * p2 + FLOP_PER_SCORING)
+    avg_launch_s = dom_ms / max(1, dom_launches) / 1e3
+    achieved = flop_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
+    out = {
+        "metric": "PatchMatch Mpix/s at equal accuracy (dZ<1%); plain GPU PatchMatch",
+        "value": round(value, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ray-cast scene, value-noise texture)",
+        "config": {"workload": "single-view plain PatchMatch 1600x1200 + 8 neighbours, "
+                               "patch 7, 5 iterations, depth [1, 40], FP32",
+                   "width": w["width"], "height": w["height"], "neighbours": w["neighbours"],
+                   "patch": w["patch"], "iterations": w["iterations"]},
+        "accuracy_within_1pct": round(acc, 4),
+        "mean_score": round(mean_score, 4),
+        "pcie_inclusive_mpix_s": round(npx / pcie_s / 1e6, 3),
+        "roofline": {
+            "kernel": "k_pm_half<7> (red/black PatchMatch half-iteration)",
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+            "scorings_per_launch": int(dom_scorings / max(1, dom_launches)),
+        },
+        "cpu_baseline": None if cpu is None else {k: (round(v, 6) if isinstance(v, float) else v)
+                                                   for k, v in cpu.items() if k != "wall_s"},
+    }
+    if cpu is not None:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/*
+ * opensfm_amd.h -- C ABI of the MI355X PatchMatch depthmap engine.
+ *
+ * Drop-in boundary for OpenSfM's legacy dense path.  Each entry point
+ * replaces one method of dense::DepthmapEstimator / dense::DepthmapCleaner
+ * (opensfm/src/lib/dense/depthmap.h:57-129, depthmap.cc:134-555), which the
+ * reference exposes to Python as pydense.DepthmapEstimator /
+ * pydense.DepthmapCleaner (opensfm/src/lib/dense/pydense.pyi:28-60).
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Matrices are row-major double[9]
+ *     (cv::Matx33d(const double*) reads row-major, depthmap.cc:150-152).
+ *   - Images are uint8 grayscale, `stride` bytes between rows.
+ *   - Inputs are copied on the call (the reference deep-copies in AddView,
+ *     depthmap.cc:156-157); output buffers are caller-owned host memory.
+ *   - Every function returns PM_OK or a negative status; pm_last_error()
+ *     gives the message of the calling thread's last failure.  No C++
+ *     exception crosses this boundary.
+ *   - A handle is not thread-safe; use one per thread (the reference's
+ *     callers do the same, opensfm/dense.py:191-238).  Each handle owns one
+ *     HIP stream on its device.
+ */
+#ifndef OPENSFM_AMD_H_
+#define OPENSFM_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the Python layer maps them to ValueError / RuntimeError /
+ * IndexError like pybind11 maps std::invalid_argument / runtime_error /
+ * out_of_range, opensfm/src/lib/dense/depthmap_bind.h:253-259) */
+#define PM_OK 0
+#define PM_ERR_INVALID (-1)
+#define PM_ERR_RUNTIME (-2)
+#define PM_ERR_RANGE (-3)
+
+/* compute modes */
+#define PM_MODE_PLANE_SWEEP 0      /* ComputeBruteForce, depthmap.cc:184-200 */
+#define PM_MODE_PATCHMATCH 1       /* ComputePatchMatch, depthmap.cc:202-213 */
+#define PM_MODE_PATCHMATCH_SAMPLE 2 /* ComputePatchMatchSample, :215-227 */
+
+typedef struct pm_estimator pm_estimator;
+typedef struct pm_cleaner pm_cleaner;
+
+/* Work and timing of the last compute on a handle. */
+typedef struct pm_stats {
+  int64_t scorings;      /* (hypothesis, view) NCC scorings executed */
+  int64_t dominant_scorings; /* of which in the dominant kernel's launches */
+  int32_t patch_size;
+  int32_t width, height; /* reference view */
+  int32_t launches;      /* launches of the dominant kernel (the plane
+                            sweep, or the red/black PatchMatch halves) */
+  float dominant_ms;     /* summed HIP-event time of those launches */
+  float total_ms;        /* HIP-event time of the whole device compute */
+} pm_stats;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* pm_last_error(void);
+
+/* Number of visible HIP devices. */
+int pm_device_count(int* count);
+
+/* ---- DepthmapEstimator ------------------------------------------------ */
+
+/* DepthmapEstimator::DepthmapEstimator  depthmap.cc:134-144
+ * (defaults: patch 7, 50 planes, 3 iterations, min patch SD 5). */
+int pm_estimator_create(int device, pm_estimator** out);
+int pm_estimator_destroy(pm_estimator* est);
+
+/* DepthmapEstimator::AddView  depthmap.cc:146-162.  View 0 is the
+ * reference view; K, R, t are world->camera (x_c = R X + t). */
+int pm_estimator_add_view(pm_estimator* est, const double* K, const double* R,
+                          const double* t, const uint8_t* image,
+                          const uint8_t* mask, int width, int height,
+                          int stride);
+
+/* DepthmapEstimator::SetDepthRange  depthmap.cc:164-169 */
+int pm_estimator_set_depth_range(pm_estimator* est, double min_depth,
+                                 double max_depth, int num_depth_planes);
+/* DepthmapEstimator::SetPatchMatchIterations  depthmap.cc:171-173 */
+int pm_estimator_set_patchmatch_iterations(pm_estimator* est, int n);
+/* DepthmapEstimator::SetPatchSize  depthmap.cc:175-178 (odd, 3..11) */
+int pm_estimator_set_patch_size(pm_estimator* est, int size);
+/* DepthmapEstimator::SetMinPatchSD  depthmap.cc:180-182 */
+int pm_estimator_set_min_patch_sd(pm_estimator* est, float sd);
+/* Seed of the counter-based generator that replaces rand() / mt19937
+ * (depthmap.cc:127-132, 141).  Same seed => bit-identical maps. */
+int pm_estimator_set_seed(pm_estimator* est, uint64_t seed);
+/* Far-propagation probes per direction of the red/black kernel (default 6;
+ * 0 = direct neighbours only). */
+int pm_estimator_set_far_probes(pm_estimator* est, int n);
+
+/* Run PostProcess (5x5 median, depthmap.cc:489-502) after PatchMatch
+ * (default 1, as the reference always does; 0 returns the raw depth). */
+int pm_estimator_set_post_process(pm_estimator* est, int enable);
+
+/* ComputeBruteForce / ComputePatchMatch / ComputePatchMatchSample
+ * (depthmap.cc:184-227) into caller buffers: depth H*W f32, plane H*W*3 f32
+ * (the plane vector v of v^T X + 1 = 0 in the reference camera frame),
+ * score H*W f32, nghbr H*W i32 (DepthmapEstimatorResult, depthmap.h:51-56).
+ * Equivalent to pm_estimator_run followed by pm_estimator_download. */
+int pm_estimator_compute(pm_estimator* est, int mode, float* depth,
+                         float* plane, float* score, int32_t* nghbr);
+
+/* Device-only compute: leaves the four maps in HBM (for timing). */
+int pm_estimator_run(pm_estimator* est, int mode);
+/* Copies the maps of the last run to host buffers. */
+int pm_estimator_download(pm_estimator* est, float* depth, float* plane,
+                          float* score, int32_t* nghbr);
+
+/* DepthmapEstimator::ComputePlaneImageScore  depthmap.cc:441-478 for n
+ * fixed hypotheses: pixel (ij[2k], ij[2k+1]) = (row, col), plane
+ * planes[3k..3k+2], neighbour view views[k] (1..N-1); homographies composed
+ * in FP64 as PlaneInducedHomographyBaked (depthmap.cc:96-102), NCC in the
+ * reference's FP32 order.  Pixels must be at least (patch-1)/2 from the
+ * border. */
+int pm_estimator_score_hypotheses(pm_estimator* est, int n, const int32_t* ij,
+                                  const float* planes, const int32_t* views,
+                                  float* out);
+
+int pm_estimator_stats(pm_estimator* est, pm_stats* out);
+
+/* ---- DepthmapCleaner -------------------------------------------------- */
+
+/* DepthmapCleaner::DepthmapCleaner  depthmap.cc:504-505
+ * (defaults: threshold 0.01, 2 consistent views). */
+int pm_cleaner_create(int device, pm_cleaner** out);
+int pm_cleaner_destroy(pm_cleaner* cl);
+/* DepthmapCleaner::SetSameDepthThreshold  depthmap.cc:507-509 */
+int pm_cleaner_set_same_depth_threshold(pm_cleaner* cl, float t);
+/* DepthmapCleaner::SetMinConsistentViews  depthmap.cc:511-513 */
+int pm_cleaner_set_min_consistent_views(pm_cleaner* cl, int n);
+/* DepthmapCleaner::AddView  depthmap.cc:515-522 (view 0 is cleaned) */
+int pm_cleaner_add_view(pm_cleaner* cl, const double* K, const double* R,
+                        const double* t, const float* depth, int width,
+                        int height);
+/* DepthmapCleaner::Clean  depthmap.cc:524-555 into clean_depth
+ * (view 0's H*W f32). */
+int pm_cleaner_clean(pm_cleaner* cl, float* clean_depth);
+/* HIP-event time of the last clean kernel, milliseconds. */
+int pm_cleaner_last_ms(pm_cleaner* cl, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENSFM_AMD_H_ */

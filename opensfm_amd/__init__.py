@@ -1,0 +1,10 @@
+"""MI355X-native PatchMatch multi-view-stereo depthmap engine.
+
+A drop-in for OpenSfM's dense depthmap path (pydense.DepthmapEstimator /
+pydense.DepthmapCleaner, opensfm/src/lib/dense/pydense.pyi:28-60), computed
+by hand-written HIP kernels for gfx950 behind the C ABI in
+include/opensfm_amd.h.
+"""
+from .pydense import DepthmapCleaner, DepthmapEstimator  # noqa: F401
+
+__all__ = ["DepthmapEstimator", "DepthmapCleaner"]

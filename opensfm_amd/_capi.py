@@ -1,0 +1,125 @@
+"""ctypes binding of the engine's C ABI (include/opensfm_amd.h).
+
+The shared library is built in-tree by ``opensfm_amd._build`` (hipcc,
+gfx950).  There is no fallback: if the library is missing or no HIP device is
+present, the calls fail loudly with the library's own error message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+from . import _build
+
+PM_OK = 0
+PM_ERR_INVALID = -1
+PM_ERR_RUNTIME = -2
+PM_ERR_RANGE = -3
+
+PM_MODE_PLANE_SWEEP = 0
+PM_MODE_PATCHMATCH = 1
+PM_MODE_PATCHMATCH_SAMPLE = 2
+
+
+class pm_stats(C.Structure):
+    _fields_ = [
+        ("scorings", C.c_int64),
+        ("dominant_scorings", C.c_int64),
+        ("patch_size", C.c_int32),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("launches", C.c_int32),
+        ("dominant_ms", C.c_float),
+        ("total_ms", C.c_float),
+    ]
+
+
+# Every symbol declared in include/opensfm_amd.h, with (restype, argtypes).
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+SIGNATURES = {
+    "pm_last_error": (C.c_char_p, []),
+    "pm_device_count": (C.c_int, [_ip]),
+    "pm_estimator_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "pm_estimator_destroy": (C.c_int, [_vp]),
+    "pm_estimator_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _u8p, _u8p, C.c_int,
+                                        C.c_int, C.c_int]),
+    "pm_estimator_set_depth_range": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int]),
+    "pm_estimator_set_patchmatch_iterations": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_patch_size": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_min_patch_sd": (C.c_int, [_vp, C.c_float]),
+    "pm_estimator_set_seed": (C.c_int, [_vp, C.c_uint64]),
+    "pm_estimator_set_far_probes": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_post_process": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_compute": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _ip]),
+    "pm_estimator_run": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_download": (C.c_int, [_vp, _fp, _fp, _fp, _ip]),
+    "pm_estimator_score_hypotheses": (C.c_int, [_vp, C.c_int, _ip, _fp, _ip, _fp]),
+    "pm_estimator_stats": (C.c_int, [_vp, C.POINTER(pm_stats)]),
+    "pm_cleaner_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "pm_cleaner_destroy": (C.c_int, [_vp]),
+    "pm_cleaner_set_same_depth_threshold": (C.c_int, [_vp, C.c_float]),
+    "pm_cleaner_set_min_consistent_views": (C.c_int, [_vp, C.c_int]),
+    "pm_cleaner_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _fp, C.c_int, C.c_int]),
+    "pm_cleaner_clean": (C.c_int, [_vp, _fp]),
+    "pm_cleaner_last_ms": (C.c_int, [_vp, _fp]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def library_path() -> str:
+    return _build.LIB
+
+
+def load():
+    """Load the engine library (building it first if it is absent)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        path = library_path()
+        if not os.path.exists(path):
+            _build.build()
+        lib = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def last_error() -> str:
+    msg = load().pm_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int) -> None:
+    """Map a C status to the exception pybind11 would raise for the
+    reference's C++ exception (depthmap_bind.h:253-259)."""
+    if rc == PM_OK:
+        return
+    msg = last_error() or f"status {rc}"
+    if rc == PM_ERR_INVALID:
+        raise ValueError(msg)
+    if rc == PM_ERR_RANGE:
+        raise IndexError(msg)
+    raise RuntimeError(msg)
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    check(load().pm_device_count(C.byref(n)))
+    return n.value
+
+
+def ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,836 @@
+// Host side of the C ABI (include/opensfm_amd.h): handle objects, FP64
+// camera algebra, device buffers, launches.
+//
+// Built with -ffp-contract=off like the kernels: the host composes the plane
+// sweep's homographies and tabulates the bilateral weights with the
+// reference's operation order (depthmap.cc:96-102, 120-125, 184-200,
+// 480-487).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/opensfm_amd.h"
+#include "pm_kernels.h"
+#include "pm_types.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+struct HipError {
+  hipError_t err;
+  const char* what;
+};
+
+#define HIP_CHECK(expr)                                   \
+  do {                                                    \
+    hipError_t e_ = (expr);                               \
+    if (e_ != hipSuccess) throw HipError{e_, #expr};      \
+  } while (0)
+
+// Runs `fn`, mapping HIP failures and C++ exceptions to status codes.
+template <typename F>
+int guarded(F&& fn) {
+  try {
+    g_last_error.clear();
+    return fn();
+  } catch (const HipError& e) {
+    return fail(PM_ERR_RUNTIME, std::string("HIP error '") +
+                                    hipGetErrorString(e.err) + "' in " + e.what);
+  } catch (const std::bad_alloc&) {
+    return fail(PM_ERR_RUNTIME, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(PM_ERR_RUNTIME, e.what());
+  }
+}
+
+// ---- FP64 3x3 helpers in cv::Matx order (accumulate from zero) ----------
+
+void mat3_mul(const double* A, const double* B, double* C) {
+  double T[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += A[i * 3 + k] * B[k * 3 + j];
+      T[i * 3 + j] = s;
+    }
+  memcpy(C, T, sizeof(T));
+}
+
+void mat3_vec(const double* A, const double* x, double* y) {
+  double t[3];
+  for (int i = 0; i < 3; ++i) {
+    double s = 0;
+    for (int k = 0; k < 3; ++k) s += A[i * 3 + k] * x[k];
+    t[i] = s;
+  }
+  memcpy(y, t, sizeof(t));
+}
+
+void mat3_transpose(const double* A, double* B) {
+  double T[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) T[j * 3 + i] = A[i * 3 + j];
+  memcpy(B, T, sizeof(T));
+}
+
+// cv::Matx33d::inv() (DECOMP_LU): determinant and cofactors for 3x3.
+bool inv3(const double* a, double* b) {
+#define A(r, c) a[(r) * 3 + (c)]
+  double d = A(0, 0) * (A(1, 1) * A(2, 2) - A(2, 1) * A(1, 2)) -
+             A(0, 1) * (A(1, 0) * A(2, 2) - A(2, 0) * A(1, 2)) +
+             A(0, 2) * (A(1, 0) * A(2, 1) - A(2, 0) * A(1, 1));
+  if (d == 0) return false;
+  d = 1 / d;
+  double t[9];
+  t[0] = (A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1)) * d;
+  t[1] = (A(0, 2) * A(2, 1) - A(0, 1) * A(2, 2)) * d;
+  t[2] = (A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1)) * d;
+  t[3] = (A(1, 2) * A(2, 0) - A(1, 0) * A(2, 2)) * d;
+  t[4] = (A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0)) * d;
+  t[5] = (A(0, 2) * A(1, 0) - A(0, 0) * A(1, 2)) * d;
+  t[6] = (A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0)) * d;
+  t[7] = (A(0, 1) * A(2, 0) - A(0, 0) * A(2, 1)) * d;
+  t[8] = (A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0)) * d;
+#undef A
+  memcpy(b, t, sizeof(t));
+  return true;
+}
+
+bool all_finite(const double* p, int n) {
+  for (int k = 0; k < n; ++k)
+    if (!std::isfinite(p[k])) return false;
+  return true;
+}
+
+// ---- device memory RAII ---------------------------------------------------
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t n) {
+    if (n <= bytes && p) return;
+    release();
+    HIP_CHECK(hipMalloc(&p, n ? n : 1));
+    bytes = n;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    HIP_CHECK(hipGetDevice(&prev));
+    if (prev != dev) HIP_CHECK(hipSetDevice(dev));
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0)
+      hipSetDevice(prev);
+  }
+};
+
+int check_device(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(PM_ERR_RUNTIME, "no HIP device available");
+  if (device < 0 || device >= n)
+    return fail(PM_ERR_RANGE, "device index " + std::to_string(device) +
+                                  " out of range [0, " + std::to_string(n) + ")");
+  return PM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// DepthmapEstimator
+
+struct HostView {
+  double K[9], R[9], t[3], Kinv[9], Q[9], a[3];
+  int width, height;
+  DevBuf img, quad;
+};
+
+struct pm_estimator {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // parameters (defaults of DepthmapEstimator::DepthmapEstimator,
+  // depthmap.cc:134-144)
+  int patch_size = 7;
+  double min_depth = 0, max_depth = 0;
+  int num_depth_planes = 50;
+  int iterations = 3;
+  float min_patch_variance = 5 * 5;
+  uint64_t seed = 0x5EEDull;
+  int n_far = 6;
+  int post_process = 1;
+  std::vector<std::unique_ptr<HostView>> views;
+  DevBuf mask0;
+  // per-compute device buffers
+  DevBuf viewtab, wlut, state, score, nghbr, counter;
+  DevBuf out_depth, out_plane, out_score, out_nghbr;
+  DevBuf sweep_H, sweep_planes;
+  bool have_result = false;
+  pm_stats stats{};
+  int wlut_patch = -1;
+
+  ~pm_estimator() {
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Bilateral weights of every (|dcolor|, dx^2 + dy^2) pair, with the
+// reference's FP32 argument and expf (BilateralWeight, depthmap.cc:480-487).
+std::vector<float> bilateral_table(int hpz) {
+  const float dcolor_sigma = 50.0f;
+  const float dx_sigma = 5.0f;
+  const float dcolor_factor = 1.0f / (2 * dcolor_sigma * dcolor_sigma);
+  const float dx_factor = 1.0f / (2 * dx_sigma * dx_sigma);
+  const int nr2 = 2 * hpz * hpz + 1;
+  std::vector<float> t(256 * (size_t)nr2);
+  for (int dc = 0; dc < 256; ++dc) {
+    for (int r2 = 0; r2 < nr2; ++r2) {
+      const float dcolor = (float)dc;
+      const float rr = (float)r2;  // dx*dx + dy*dy, exact in FP32
+      t[(size_t)dc * nr2 + r2] =
+          expf(-dcolor * dcolor * dcolor_factor - rr * dx_factor);
+    }
+  }
+  return t;
+}
+
+// PlaneFromDepthAndNormal (depthmap.cc:120-125) in the reference's order.
+void plane_from_depth_and_normal(float x, float y, const double* Kinv,
+                                 float depth, const float* normal,
+                                 float* plane) {
+  double S[9];
+  for (int k = 0; k < 9; ++k) S[k] = Kinv[k] * depth;
+  const double q[3] = {x, y, 1};
+  double pd[3];
+  mat3_vec(S, q, pd);
+  const float point[3] = {(float)pd[0], (float)pd[1], (float)pd[2]};
+  float s = 0;
+  for (int k = 0; k < 3; ++k) s += normal[k] * point[k];
+  const float denom = -s;
+  const float m = (1e-6f < denom) ? denom : 1e-6f;
+  const float inv = 1.f / m;
+  for (int k = 0; k < 3; ++k) plane[k] = normal[k] * inv;
+}
+
+// PlaneInducedHomographyBaked (depthmap.cc:96-102).
+void homography_baked(const double* K1inv, const double* Q2, const double* a2,
+                      const double* K2, const float* vf, float* H) {
+  const double v[3] = {vf[0], vf[1], vf[2]};
+  double S[9], T[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      s += a2[i] * v[j];
+      S[i * 3 + j] = Q2[i * 3 + j] + s;
+    }
+  mat3_mul(K2, S, T);
+  mat3_mul(T, K1inv, T);
+  for (int k = 0; k < 9; ++k) H[k] = (float)T[k];
+}
+
+int validate_for_compute(pm_estimator* e) {
+  if (e->views.size() < 2)
+    return fail(PM_ERR_INVALID, "need a reference view and at least one neighbour view (got " +
+                                    std::to_string(e->views.size()) + ")");
+  if ((int)e->views.size() > pm::kMaxViews)
+    return fail(PM_ERR_INVALID, "at most " + std::to_string(pm::kMaxViews) + " views");
+  if (!(e->min_depth > 0) || !(e->max_depth > e->min_depth) ||
+      !std::isfinite(e->max_depth))
+    return fail(PM_ERR_INVALID, "depth range must satisfy 0 < min_depth < max_depth (call set_depth_range)");
+  const HostView& r = *e->views[0];
+  const int hpz = (e->patch_size - 1) / 2;
+  if (r.width < 2 * hpz + 1 || r.height < 2 * hpz + 1)
+    return fail(PM_ERR_INVALID, "reference view smaller than one patch");
+  return PM_OK;
+}
+
+// Fills the device view table and the argument block for a compute.
+pm::PmArgs prepare(pm_estimator* e) {
+  const int n = (int)e->views.size();
+  const HostView& r = *e->views[0];
+  std::vector<pm::PmView> tab(n);
+  for (int v = 0; v < n; ++v) {
+    const HostView& V = *e->views[v];
+    pm::PmView& T = tab[v];
+    memset(&T, 0, sizeof(T));
+    T.quad = V.quad.as<uint32_t>();
+    T.width = V.width;
+    T.height = V.height;
+    double KQ[9], M[9], b[3];
+    mat3_mul(V.K, V.Q, KQ);
+    mat3_mul(KQ, r.Kinv, M);
+    mat3_vec(V.K, V.a, b);
+    for (int k = 0; k < 9; ++k) T.M[k] = (float)M[k];
+    for (int k = 0; k < 3; ++k) T.b[k] = (float)b[k];
+    memcpy(T.K, V.K, sizeof(T.K));
+    memcpy(T.Q, V.Q, sizeof(T.Q));
+    memcpy(T.a, V.a, sizeof(T.a));
+  }
+  e->viewtab.ensure(sizeof(pm::PmView) * n);
+  HIP_CHECK(hipMemcpyAsync(e->viewtab.p, tab.data(), sizeof(pm::PmView) * n,
+                           hipMemcpyHostToDevice, e->stream));
+  const int hpz = (e->patch_size - 1) / 2;
+  if (e->wlut_patch != e->patch_size) {
+    std::vector<float> t = bilateral_table(hpz);
+    e->wlut.ensure(t.size() * sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(e->wlut.p, t.data(), t.size() * sizeof(float),
+                             hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    e->wlut_patch = e->patch_size;
+  }
+  const size_t np = (size_t)r.width * r.height;
+  e->state.ensure(np * 4 * sizeof(float));
+  e->score.ensure(np * sizeof(float));
+  e->nghbr.ensure(np * sizeof(int));
+  e->counter.ensure(2 * sizeof(unsigned long long));
+  HIP_CHECK(hipMemsetAsync(e->state.p, 0, np * 4 * sizeof(float), e->stream));
+  HIP_CHECK(hipMemsetAsync(e->score.p, 0, np * sizeof(float), e->stream));
+  HIP_CHECK(hipMemsetAsync(e->nghbr.p, 0, np * sizeof(int), e->stream));
+  HIP_CHECK(hipMemsetAsync(e->counter.p, 0, 2 * sizeof(unsigned long long), e->stream));
+  HIP_CHECK(hipStreamSynchronize(e->stream));  // `tab` feeds an async copy
+
+  pm::PmArgs a;
+  memset(&a, 0, sizeof(a));
+  a.W = r.width;
+  a.H = r.height;
+  a.nviews = n;
+  a.hpz = hpz;
+  a.views = e->viewtab.as<pm::PmView>();
+  a.ref_img = r.img.as<uint8_t>();
+  a.ref_mask = e->mask0.as<uint8_t>();
+  a.wlut = e->wlut.as<float>();
+  a.state = e->state.as<float>();
+  a.score = e->score.as<float>();
+  a.nghbr = e->nghbr.as<int>();
+  for (int k = 0; k < 9; ++k) {
+    a.Kinv0f[k] = (float)r.Kinv[k];
+    a.Kinv0[k] = r.Kinv[k];
+  }
+  a.log_min = (float)log(e->min_depth);
+  a.log_max = (float)log(e->max_depth);
+  a.min_patch_var = e->min_patch_variance;
+  a.seed_lo = (uint32_t)(e->seed & 0xffffffffu);
+  a.seed_hi = (uint32_t)(e->seed >> 32);
+  a.n_far = e->n_far;
+  a.counter = e->counter.as<unsigned long long>();
+  return a;
+}
+
+int run(pm_estimator* e, int mode) {
+  int rc = validate_for_compute(e);
+  if (rc) return rc;
+  if (mode != PM_MODE_PLANE_SWEEP && mode != PM_MODE_PATCHMATCH &&
+      mode != PM_MODE_PATCHMATCH_SAMPLE)
+    return fail(PM_ERR_INVALID, "unknown compute mode " + std::to_string(mode));
+  if (mode == PM_MODE_PATCHMATCH_SAMPLE)
+    return fail(PM_ERR_INVALID, "compute_patch_match_sample is not implemented on the HIP engine yet");
+  DeviceGuard dg(e->device);
+  e->have_result = false;
+  pm::PmArgs a = prepare(e);
+  const HostView& r = *e->views[0];
+  const int n = (int)e->views.size();
+  float dom_ms = 0.0f;
+  int launches = 0;
+  HIP_CHECK(hipEventRecord(e->ev[0], e->stream));
+  if (mode == PM_MODE_PLANE_SWEEP) {
+    const int D = e->num_depth_planes;
+    if (D < 2) return fail(PM_ERR_INVALID, "plane sweep needs num_depth_planes >= 2");
+    // The sweep plane v_d = n / (d K^-1 q)_z is pixel independent when the
+    // last row of K^-1 is (0, 0, k); every pinhole K satisfies this.
+    if (r.Kinv[6] != 0.0 || r.Kinv[7] != 0.0)
+      return fail(PM_ERR_INVALID, "plane sweep needs a pinhole K (last row 0, 0, k)");
+    std::vector<float> planes(3 * (size_t)D), H(9 * (size_t)D * n, 0.0f);
+    for (int d = 0; d < D; ++d) {
+      // ComputeBruteForce (depthmap.cc:190-195)
+      const float depth = (float)(1 / (1 / e->min_depth +
+                                       d * (1 / e->max_depth - 1 / e->min_depth) /
+                                           (D - 1)));
+      const float normal[3] = {0, 0, -1};
+      plane_from_depth_and_normal(0.0f, 0.0f, r.Kinv, depth, normal, &planes[3 * d]);
+      for (int v = 1; v < n; ++v) {
+        const HostView& V = *e->views[v];
+        homography_baked(r.Kinv, V.Q, V.a, V.K, &planes[3 * d],
+                         &H[((size_t)d * n + v) * 9]);
+      }
+    }
+    e->sweep_H.ensure(H.size() * sizeof(float));
+    e->sweep_planes.ensure(planes.size() * sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(e->sweep_H.p, H.data(), H.size() * sizeof(float),
+                             hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(hipMemcpyAsync(e->sweep_planes.p, planes.data(),
+                             planes.size() * sizeof(float),
+                             hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(hipEventRecord(e->ev[2], e->stream));
+    HIP_CHECK(pm::launch_sweep(a, e->sweep_H.as<float>(),
+                               e->sweep_planes.as<float>(), D, e->stream));
+    HIP_CHECK(hipEventRecord(e->ev[3], e->stream));
+    HIP_CHECK(hipEventSynchronize(e->ev[3]));
+    HIP_CHECK(hipEventElapsedTime(&dom_ms, e->ev[2], e->ev[3]));
+    launches = 1;
+  } else {
+    HIP_CHECK(pm::launch_pm_init(a, e->stream));
+    pm::PmArgs ah = a;  // the red/black launches count into slot 1
+    ah.counter = a.counter + 1;
+    for (int it = 0; it < e->iterations; ++it) {
+      for (int colour = 0; colour < 2; ++colour) {
+        HIP_CHECK(hipEventRecord(e->ev[2], e->stream));
+        HIP_CHECK(pm::launch_pm_half(ah, colour, it, e->stream));
+        HIP_CHECK(hipEventRecord(e->ev[3], e->stream));
+        HIP_CHECK(hipEventSynchronize(e->ev[3]));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e->ev[2], e->ev[3]));
+        dom_ms += ms;
+        ++launches;
+      }
+    }
+  }
+  const size_t np = (size_t)r.width * r.height;
+  e->out_depth.ensure(np * sizeof(float));
+  e->out_plane.ensure(np * 3 * sizeof(float));
+  e->out_score.ensure(np * sizeof(float));
+  e->out_nghbr.ensure(np * sizeof(int));
+  HIP_CHECK(pm::launch_pack(a, mode == PM_MODE_PLANE_SWEEP ? 0 : e->post_process,
+                            e->out_depth.as<float>(), e->out_plane.as<float>(),
+                            e->out_score.as<float>(), e->out_nghbr.as<int>(),
+                            e->stream));
+  HIP_CHECK(hipEventRecord(e->ev[1], e->stream));
+  HIP_CHECK(hipEventSynchronize(e->ev[1]));
+  float total_ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&total_ms, e->ev[0], e->ev[1]));
+  unsigned long long cnt[2] = {0, 0};
+  HIP_CHECK(hipMemcpy(cnt, e->counter.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  e->stats.scorings = (int64_t)(cnt[0] + cnt[1]);
+  e->stats.dominant_scorings =
+      (int64_t)(mode == PM_MODE_PLANE_SWEEP ? cnt[0] : cnt[1]);
+  e->stats.patch_size = e->patch_size;
+  e->stats.width = r.width;
+  e->stats.height = r.height;
+  e->stats.launches = launches;
+  e->stats.dominant_ms = dom_ms;
+  e->stats.total_ms = total_ms;
+  e->have_result = true;
+  return PM_OK;
+}
+
+int download(pm_estimator* e, float* depth, float* plane, float* score,
+             int32_t* nghbr) {
+  if (!e->have_result) return fail(PM_ERR_INVALID, "no computed maps to download");
+  if (!depth || !plane || !score || !nghbr)
+    return fail(PM_ERR_INVALID, "null output buffer");
+  DeviceGuard dg(e->device);
+  const HostView& r = *e->views[0];
+  const size_t np = (size_t)r.width * r.height;
+  HIP_CHECK(hipMemcpyAsync(depth, e->out_depth.p, np * sizeof(float),
+                           hipMemcpyDeviceToHost, e->stream));
+  HIP_CHECK(hipMemcpyAsync(plane, e->out_plane.p, np * 3 * sizeof(float),
+                           hipMemcpyDeviceToHost, e->stream));
+  HIP_CHECK(hipMemcpyAsync(score, e->out_score.p, np * sizeof(float),
+                           hipMemcpyDeviceToHost, e->stream));
+  HIP_CHECK(hipMemcpyAsync(nghbr, e->out_nghbr.p, np * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, e->stream));
+  HIP_CHECK(hipStreamSynchronize(e->stream));
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pm_last_error(void) { return g_last_error.c_str(); }
+
+int pm_device_count(int* count) {
+  if (!count) return fail(PM_ERR_INVALID, "null count");
+  int n = 0;
+  hipError_t err = hipGetDeviceCount(&n);
+  if (err != hipSuccess) n = 0;
+  *count = n;
+  g_last_error.clear();
+  return PM_OK;
+}
+
+int pm_estimator_create(int device, pm_estimator** out) {
+  if (!out) return fail(PM_ERR_INVALID, "null handle pointer");
+  *out = nullptr;
+  int rc = check_device(device);
+  if (rc) return rc;
+  return guarded([&] {
+    std::unique_ptr<pm_estimator> e(new pm_estimator());
+    e->device = device;
+    DeviceGuard dg(device);
+    HIP_CHECK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    for (auto& ev : e->ev) HIP_CHECK(hipEventCreate(&ev));
+    *out = e.release();
+    return PM_OK;
+  });
+}
+
+int pm_estimator_destroy(pm_estimator* est) {
+  if (!est) return PM_OK;
+  return guarded([&] {
+    DeviceGuard dg(est->device);
+    hipStreamSynchronize(est->stream);
+    delete est;
+    return PM_OK;
+  });
+}
+
+int pm_estimator_add_view(pm_estimator* e, const double* K, const double* R,
+                          const double* t, const uint8_t* image,
+                          const uint8_t* mask, int width, int height,
+                          int stride) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (!K || !R || !t || !image || !mask)
+    return fail(PM_ERR_INVALID, "null K, R, t, image or mask");
+  if (width < 2 || height < 2)
+    return fail(PM_ERR_INVALID, "image must be at least 2x2");
+  if (stride < width) return fail(PM_ERR_INVALID, "stride smaller than width");
+  if (!all_finite(K, 9) || !all_finite(R, 9) || !all_finite(t, 3))
+    return fail(PM_ERR_INVALID, "K, R and t must be finite");
+  if ((int)e->views.size() >= pm::kMaxViews)
+    return fail(PM_ERR_INVALID, "at most " + std::to_string(pm::kMaxViews) + " views");
+  return guarded([&] {
+    std::unique_ptr<HostView> v(new HostView());
+    memcpy(v->K, K, sizeof(v->K));
+    memcpy(v->R, R, sizeof(v->R));
+    memcpy(v->t, t, sizeof(v->t));
+    if (!inv3(v->K, v->Kinv)) return fail(PM_ERR_INVALID, "K is singular");
+    // Q_i = R_i R_0^T, a_i = Q_i t_0 - t_i  (depthmap.cc:153-155)
+    const HostView* ref = e->views.empty() ? v.get() : e->views[0].get();
+    double R0t[9], Qt0[3];
+    mat3_transpose(ref->R, R0t);
+    mat3_mul(v->R, R0t, v->Q);
+    mat3_vec(v->Q, ref->t, Qt0);
+    for (int k = 0; k < 3; ++k) v->a[k] = Qt0[k] - v->t[k];
+    v->width = width;
+    v->height = height;
+    DeviceGuard dg(e->device);
+    const size_t np = (size_t)width * height;
+    v->img.ensure(np);
+    HIP_CHECK(hipMemcpy2DAsync(v->img.p, width, image, stride, width, height,
+                               hipMemcpyHostToDevice, e->stream));
+    v->quad.ensure((size_t)(width + 1) * (height + 1) * sizeof(uint32_t));
+    HIP_CHECK(pm::launch_build_quad(v->img.as<uint8_t>(), v->quad.as<uint32_t>(),
+                                    width, height, e->stream));
+    if (e->views.empty()) {
+      e->mask0.ensure(np);
+      HIP_CHECK(hipMemcpy2DAsync(e->mask0.p, width, mask, stride, width, height,
+                                 hipMemcpyHostToDevice, e->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    e->views.push_back(std::move(v));
+    e->have_result = false;
+    return PM_OK;
+  });
+}
+
+int pm_estimator_set_depth_range(pm_estimator* e, double min_depth,
+                                 double max_depth, int num_depth_planes) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  e->min_depth = min_depth;
+  e->max_depth = max_depth;
+  e->num_depth_planes = num_depth_planes;
+  g_last_error.clear();
+  return PM_OK;
+}
+
+int pm_estimator_set_patchmatch_iterations(pm_estimator* e, int n) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (n < 0) return fail(PM_ERR_INVALID, "iterations must be >= 0");
+  e->iterations = n;
+  return PM_OK;
+}
+
+int pm_estimator_set_patch_size(pm_estimator* e, int size) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (size < 3 || size > 11 || size % 2 == 0)
+    return fail(PM_ERR_INVALID, "patch size must be odd and in [3, 11], got " +
+                                    std::to_string(size));
+  e->patch_size = size;
+  return PM_OK;
+}
+
+int pm_estimator_set_min_patch_sd(pm_estimator* e, float sd) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  e->min_patch_variance = sd * sd;
+  return PM_OK;
+}
+
+int pm_estimator_set_seed(pm_estimator* e, uint64_t seed) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  e->seed = seed;
+  return PM_OK;
+}
+
+int pm_estimator_set_far_probes(pm_estimator* e, int n) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (n < 0 || n > 64) return fail(PM_ERR_INVALID, "far probes must be in [0, 64]");
+  e->n_far = n;
+  return PM_OK;
+}
+
+int pm_estimator_set_post_process(pm_estimator* e, int enable) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  e->post_process = enable ? 1 : 0;
+  return PM_OK;
+}
+
+int pm_estimator_run(pm_estimator* e, int mode) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  return guarded([&] { return run(e, mode); });
+}
+
+int pm_estimator_download(pm_estimator* e, float* depth, float* plane,
+                          float* score, int32_t* nghbr) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  return guarded([&] { return download(e, depth, plane, score, nghbr); });
+}
+
+int pm_estimator_compute(pm_estimator* e, int mode, float* depth, float* plane,
+                         float* score, int32_t* nghbr) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  return guarded([&] {
+    int rc = run(e, mode);
+    if (rc) return rc;
+    return download(e, depth, plane, score, nghbr);
+  });
+}
+
+int pm_estimator_score_hypotheses(pm_estimator* e, int n, const int32_t* ij,
+                                  const float* planes, const int32_t* views,
+                                  float* out) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (n < 0) return fail(PM_ERR_INVALID, "negative hypothesis count");
+  if (n == 0) return PM_OK;
+  if (!ij || !planes || !views || !out) return fail(PM_ERR_INVALID, "null buffer");
+  if (e->views.size() < 2)
+    return fail(PM_ERR_INVALID, "need a reference view and at least one neighbour view");
+  const HostView& r = *e->views[0];
+  const int hpz = (e->patch_size - 1) / 2;
+  for (int k = 0; k < n; ++k) {
+    const int i = ij[2 * k], j = ij[2 * k + 1];
+    if (i < hpz || i >= r.height - hpz || j < hpz || j >= r.width - hpz)
+      return fail(PM_ERR_RANGE, "hypothesis " + std::to_string(k) +
+                                    ": pixel closer than (patch-1)/2 to the border");
+    if (views[k] < 1 || views[k] >= (int)e->views.size())
+      return fail(PM_ERR_RANGE, "hypothesis " + std::to_string(k) + ": view index out of range");
+  }
+  return guarded([&] {
+    DeviceGuard dg(e->device);
+    pm::PmArgs a = prepare(e);
+    // group by view: one launch per view keeps the view index uniform
+    const int nv = (int)e->views.size();
+    std::vector<int> order(n), start(nv + 1, 0);
+    for (int k = 0; k < n; ++k) start[views[k] + 1]++;
+    for (int v = 0; v < nv; ++v) start[v + 1] += start[v];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int k = 0; k < n; ++k) order[fill[views[k]]++] = k;
+    std::vector<int32_t> s_ij(2 * (size_t)n);
+    std::vector<float> s_pl(3 * (size_t)n), s_out(n);
+    for (int m = 0; m < n; ++m) {
+      const int k = order[m];
+      s_ij[2 * m] = ij[2 * k];
+      s_ij[2 * m + 1] = ij[2 * k + 1];
+      for (int c = 0; c < 3; ++c) s_pl[3 * m + c] = planes[3 * k + c];
+    }
+    DevBuf d_ij, d_pl, d_out;
+    d_ij.ensure(sizeof(int32_t) * 2 * n);
+    d_pl.ensure(sizeof(float) * 3 * n);
+    d_out.ensure(sizeof(float) * n);
+    HIP_CHECK(hipMemcpyAsync(d_ij.p, s_ij.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(hipMemcpyAsync(d_pl.p, s_pl.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, e->stream));
+    for (int v = 1; v < nv; ++v) {
+      const int cnt = start[v + 1] - start[v];
+      if (cnt == 0) continue;
+      HIP_CHECK(pm::launch_score_hyp(a, v, cnt, d_ij.as<int>() + 2 * start[v],
+                                     d_pl.as<float>() + 3 * start[v],
+                                     d_out.as<float>() + start[v], e->stream));
+    }
+    HIP_CHECK(hipMemcpyAsync(s_out.data(), d_out.p, sizeof(float) * n, hipMemcpyDeviceToHost, e->stream));
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    for (int m = 0; m < n; ++m) out[order[m]] = s_out[m];
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    return PM_OK;
+  });
+}
+
+int pm_estimator_stats(pm_estimator* e, pm_stats* out) {
+  if (!e || !out) return fail(PM_ERR_INVALID, "null argument");
+  *out = e->stats;
+  return PM_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// DepthmapCleaner
+
+struct CleanerView {
+  double K[9], R[9], t[3];
+  int width, height;
+  DevBuf depth;
+};
+
+struct pm_cleaner {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  float same_depth_threshold = 0.01f;
+  int min_consistent_views = 2;
+  std::vector<std::unique_ptr<CleanerView>> views;
+  DevBuf table, ref, out;
+  float last_ms = 0;
+  ~pm_cleaner() {
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+int pm_cleaner_create(int device, pm_cleaner** out) {
+  if (!out) return fail(PM_ERR_INVALID, "null handle pointer");
+  *out = nullptr;
+  int rc = check_device(device);
+  if (rc) return rc;
+  return guarded([&] {
+    std::unique_ptr<pm_cleaner> c(new pm_cleaner());
+    c->device = device;
+    DeviceGuard dg(device);
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& ev : c->ev) HIP_CHECK(hipEventCreate(&ev));
+    *out = c.release();
+    return PM_OK;
+  });
+}
+
+int pm_cleaner_destroy(pm_cleaner* c) {
+  if (!c) return PM_OK;
+  return guarded([&] {
+    DeviceGuard dg(c->device);
+    hipStreamSynchronize(c->stream);
+    delete c;
+    return PM_OK;
+  });
+}
+
+int pm_cleaner_set_same_depth_threshold(pm_cleaner* c, float t) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  c->same_depth_threshold = t;
+  return PM_OK;
+}
+
+int pm_cleaner_set_min_consistent_views(pm_cleaner* c, int n) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  c->min_consistent_views = n;
+  return PM_OK;
+}
+
+int pm_cleaner_add_view(pm_cleaner* c, const double* K, const double* R,
+                        const double* t, const float* depth, int width,
+                        int height) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  if (!K || !R || !t || !depth) return fail(PM_ERR_INVALID, "null K, R, t or depth");
+  if (width < 1 || height < 1) return fail(PM_ERR_INVALID, "empty depthmap");
+  if (!all_finite(K, 9) || !all_finite(R, 9) || !all_finite(t, 3))
+    return fail(PM_ERR_INVALID, "K, R and t must be finite");
+  if ((int)c->views.size() >= 1024) return fail(PM_ERR_INVALID, "too many views");
+  return guarded([&] {
+    std::unique_ptr<CleanerView> v(new CleanerView());
+    memcpy(v->K, K, sizeof(v->K));
+    memcpy(v->R, R, sizeof(v->R));
+    memcpy(v->t, t, sizeof(v->t));
+    v->width = width;
+    v->height = height;
+    DeviceGuard dg(c->device);
+    const size_t bytes = (size_t)width * height * sizeof(float);
+    v->depth.ensure(bytes);
+    HIP_CHECK(hipMemcpyAsync(v->depth.p, depth, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->views.push_back(std::move(v));
+    return PM_OK;
+  });
+}
+
+int pm_cleaner_clean(pm_cleaner* c, float* clean_depth) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  if (!clean_depth) return fail(PM_ERR_INVALID, "null output");
+  if (c->views.empty()) return fail(PM_ERR_INVALID, "no views added");
+  return guarded([&] {
+    DeviceGuard dg(c->device);
+    const int n = (int)c->views.size();
+    std::vector<pm::ClView> tab(n);
+    for (int v = 0; v < n; ++v) {
+      const CleanerView& V = *c->views[v];
+      tab[v].depth = V.depth.as<float>();
+      tab[v].width = V.width;
+      tab[v].height = V.height;
+      memcpy(tab[v].K, V.K, sizeof(V.K));
+      memcpy(tab[v].R, V.R, sizeof(V.R));
+      memcpy(tab[v].t, V.t, sizeof(V.t));
+    }
+    pm::ClRef ref;
+    if (!inv3(c->views[0]->K, ref.Kinv)) return fail(PM_ERR_INVALID, "K of view 0 is singular");
+    mat3_transpose(c->views[0]->R, ref.Rt);
+    c->table.ensure(sizeof(pm::ClView) * n);
+    c->ref.ensure(sizeof(pm::ClRef));
+    const CleanerView& V0 = *c->views[0];
+    const size_t np = (size_t)V0.width * V0.height;
+    c->out.ensure(np * sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(c->table.p, tab.data(), sizeof(pm::ClView) * n,
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->ref.p, &ref, sizeof(ref), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
+    HIP_CHECK(pm::launch_clean(c->table.as<pm::ClView>(), c->ref.as<pm::ClRef>(), n,
+                               V0.width, V0.height, c->same_depth_threshold,
+                               c->min_consistent_views, c->out.as<float>(), c->stream));
+    HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
+    HIP_CHECK(hipMemcpyAsync(clean_depth, c->out.p, np * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms, c->ev[0], c->ev[1]));
+    return PM_OK;
+  });
+}
+
+int pm_cleaner_last_ms(pm_cleaner* c, float* ms) {
+  if (!c || !ms) return fail(PM_ERR_INVALID, "null argument");
+  *ms = c->last_ms;
+  return PM_OK;
+}
+
+}  // extern "C"

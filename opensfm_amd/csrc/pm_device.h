@@ -1,0 +1,384 @@
+// Device-side building blocks of the PatchMatch depthmap engine (gfx950).
+//
+// The whole library is compiled with -ffp-contract=off, so plain C++
+// arithmetic below follows the written operation order exactly (this is what
+// the "exact" scorer relies on for parity with the reference's FP32 code);
+// the "fast" scorer asks for FMAs explicitly with __builtin_fmaf.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pm_types.h"
+
+namespace pm {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC'11).  Each pixel owns a
+// key; the counter encodes (iteration, half-iteration, step).
+
+struct u4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u4 philox4x32_10(uint32_t k0, uint32_t k1, u4 c) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t m0 = 0xD2511F53u, m1 = 0xCD9E8D57u;
+    uint32_t hi0 = __umulhi(m0, c.x), lo0 = m0 * c.x;
+    uint32_t hi1 = __umulhi(m1, c.z), lo1 = m1 * c.z;
+    c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// [0, 1) with 24 random bits
+__device__ __forceinline__ float u01(uint32_t x) {
+  return (float)(x >> 8) * (1.0f / 16777216.0f);
+}
+// (0, 1]
+__device__ __forceinline__ float u01_open0(uint32_t x) {
+  return ((float)(x >> 8) + 1.0f) * (1.0f / 16777216.0f);
+}
+
+// Two standard normals from two uniforms (Box-Muller).
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float* n0,
+                                           float* n1) {
+  float r = sqrtf(-2.0f * __logf(u01_open0(a)));
+  float s, c;
+  __sincosf(6.28318530717958647692f * u01(b), &s, &c);
+  *n0 = r * c;
+  *n1 = r * s;
+}
+
+// ---------------------------------------------------------------------------
+// Packed bilinear texels.  View images are stored as "quads": texel (ix, iy)
+// packs the four u8 neighbours im[iy][ix], im[iy][ix+1], im[iy+1][ix],
+// im[iy+1][ix+1] into one u32, so one 4-byte gather feeds one bilinear
+// sample.  Texels exist for ix in [-1, W-1] and iy in [-1, H-1] (pitch W+1,
+// one zero column/row in front); only 0 <= ix <= W-2, 0 <= iy <= H-2 hold
+// image data, all others are zero, so a clamped out-of-image sample reads 0.
+
+__device__ __forceinline__ float ub0(uint32_t v) { return (float)(v & 0xffu); }
+__device__ __forceinline__ float ub1(uint32_t v) { return (float)((v >> 8) & 0xffu); }
+__device__ __forceinline__ float ub2(uint32_t v) { return (float)((v >> 16) & 0xffu); }
+__device__ __forceinline__ float ub3(uint32_t v) { return (float)(v >> 24); }
+
+// ---------------------------------------------------------------------------
+// Homography in FP64 exactly as PlaneInducedHomographyBaked
+// (depthmap.cc:96-102): H = K2 * (Q2 + a2 * v^T) * K1inv, products
+// accumulated from zero in k order (cv::Matx), then cast to FP32.
+
+__device__ __forceinline__ void homography_baked_f64(const double* K1inv,
+                                                     const double* Q2,
+                                                     const double* a2,
+                                                     const double* K2, float v0,
+                                                     float v1, float v2,
+                                                     float* H) {
+  const double v[3] = {(double)v0, (double)v1, (double)v2};
+  double S[9], T[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      s += a2[i] * v[j];
+      S[i * 3 + j] = Q2[i * 3 + j] + s;
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s += K2[i * 3 + k] * S[k * 3 + j];
+      T[i * 3 + j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s += T[i * 3 + k] * K1inv[k * 3 + j];
+      H[i * 3 + j] = (float)s;
+    }
+}
+
+// DepthOfPlaneBackprojection (depthmap.cc:114-118) in FP64 with the
+// reference's order: r = plane^T K^-1, s = r . (x, y, 1).
+__device__ __forceinline__ float depth_of_plane_f64(const double* Kinv,
+                                                    float p0, float p1,
+                                                    float p2, double x,
+                                                    double y) {
+  const double p[3] = {(double)p0, (double)p1, (double)p2};
+  double r[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s += p[k] * Kinv[k * 3 + j];
+    r[j] = s;
+  }
+  double s = 0;
+  s += r[0] * x;
+  s += r[1] * y;
+  s += r[2] * 1.0;
+  float denom = (float)(-s);
+  float m = (1e-6f < denom) ? denom : 1e-6f;
+  return __fdiv_rn(1.0f, m);
+}
+
+// ---------------------------------------------------------------------------
+// Reference-side patch data: hypothesis-invariant per pixel.  w[k] is the
+// bilateral weight of sample k (from the host-tabulated table, bit-identical
+// to expf of the reference's argument); the u8 intensities x[k] are kept
+// packed four to a register and w[k] * x[k] is re-formed when needed (the
+// same FP32 product the reference forms).  The reference accumulates
+// Sum w x, Sum w x x and Sum w in sample order; those sums do not depend on
+// the hypothesis and are formed once.
+
+template <int P>
+struct RefPatch {
+  static constexpr int N = P * P;
+  float w[N];
+  uint32_t xp[(N + 3) / 4];
+  float sumw, meanx, varx;  // exact-order FP32
+  float rsumw;              // 1 / sumw for the fast path
+  __device__ __forceinline__ float x(int k) const {
+    return (float)((xp[k >> 2] >> (8 * (k & 3))) & 0xffu);
+  }
+};
+
+// Packed intensities re-read per scoring.  The empty asm makes them opaque
+// to the optimiser, which would otherwise hoist the 49 unpacked floats (and
+// the 49 products w*x) out of the hypothesis loops into registers.
+template <int P>
+struct XUnpack {
+  uint32_t xp[(P * P + 3) / 4];
+  __device__ __forceinline__ explicit XUnpack(const RefPatch<P>& rp) {
+#pragma unroll
+    for (int m = 0; m < (P * P + 3) / 4; ++m) {
+      xp[m] = rp.xp[m];
+      asm volatile("" : "+v"(xp[m]));
+    }
+  }
+  __device__ __forceinline__ float operator()(int k) const {
+    return (float)((xp[k >> 2] >> (8 * (k & 3))) & 0xffu);
+  }
+};
+
+// Bilateral weight table: wlut[|dcolor| * nr2 + (dx*dx + dy*dy)], nr2 =
+// 2*hpz*hpz + 1.
+template <int P>
+__device__ __forceinline__ bool load_ref_patch(RefPatch<P>& rp,
+                                               const uint8_t* __restrict__ img,
+                                               int W, int i, int j,
+                                               const float* __restrict__ wlut) {
+  constexpr int hpz = (P - 1) / 2;
+  constexpr int nr2 = 2 * hpz * hpz + 1;
+  const int c = img[i * W + j];
+  float sumx = 0, sumxx = 0, sumw = 0;
+#pragma unroll
+  for (int k = 0; k < (P * P + 3) / 4; ++k) rp.xp[k] = 0;
+#pragma unroll
+  for (int dy = -hpz; dy <= hpz; ++dy) {
+#pragma unroll
+    for (int dx = -hpz; dx <= hpz; ++dx) {
+      const int k = (dy + hpz) * P + (dx + hpz);
+      const int x = img[(i + dy) * W + (j + dx)];
+      const int dc = x - c;
+      const float w = wlut[(dc < 0 ? -dc : dc) * nr2 + dx * dx + dy * dy];
+      const float xf = (float)x;
+      rp.w[k] = w;
+      rp.xp[k >> 2] |= (uint32_t)x << (8 * (k & 3));
+      const float wx = w * xf;
+      sumx += wx;
+      sumxx += wx * xf;
+      sumw += w;
+    }
+  }
+  rp.sumw = sumw;
+  rp.rsumw = __frcp_rn(sumw);
+  const float meanx = __fdiv_rn(sumx, sumw);
+  const float meanxx = __fdiv_rn(sumxx, sumw);
+  rp.meanx = meanx;
+  rp.varx = meanxx - meanx * meanx;
+  // NCCEstimator::Get returns -1 for every hypothesis when var_x < 0.1.
+  return !(rp.varx < 0.1);
+}
+
+// ---------------------------------------------------------------------------
+// Texel gathers through a buffer resource: 32-bit byte offsets (one VGPR per
+// address), hardware range check as a second line of defence.
+
+struct QuadTex {
+  __amdgpu_buffer_rsrc_t rsrc;  // base = texel (0, 0)
+  int pitch;                    // texels per row (W + 1)
+  float xmax, ymax;             // W - 1, H - 1: largest texel index (zero)
+};
+
+// Wave-uniform copies (scalar registers) of values the caller knows are
+// uniform, e.g. fields of a view selected by a uniform loop index.
+__device__ __forceinline__ int uniform(int x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+__device__ __forceinline__ float uniform(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+template <typename T>
+__device__ __forceinline__ T* uniform(T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T*)(((uint64_t)hi << 32) | lo);
+}
+
+// Texture of a view whose index is uniform across the wavefront.
+__device__ __forceinline__ QuadTex make_tex(const uint32_t* quad, int W, int H) {
+  QuadTex t;
+  quad = uniform(quad);
+  W = uniform(W);
+  H = uniform(H);
+  t.pitch = W + 1;
+  // records from texel (0, 0) to the end; texel (-1, -1) would be index
+  // -(pitch + 1), an out-of-range record that reads 0 like the border
+  const int records = t.pitch * (H + 1) - (t.pitch + 1);
+  t.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(quad + t.pitch + 1),
+                                             (short)0, records * 4, 0x00020000);
+  t.xmax = (float)(W - 1);
+  t.ymax = (float)(H - 1);
+  return t;
+}
+
+// Texel whose top-left corner is (floor x, floor y), clamped into the zero
+// border: a sample outside [0, W-1) x [0, H-1) reads four zeros.  Texel
+// (ix, iy) is record iy * pitch + ix; row -1 gives negative (out-of-range)
+// record indices, which the buffer unit returns as 0.
+__device__ __forceinline__ uint32_t fetch_quad(const QuadTex& t, float flx,
+                                               float fly) {
+  const int ix = (int)__builtin_amdgcn_fmed3f(flx, -1.0f, t.xmax);
+  const int iy = (int)__builtin_amdgcn_fmed3f(fly, -1.0f, t.ymax);
+  const int rec = __mul24(iy, t.pitch) + ix;
+  return __builtin_amdgcn_raw_buffer_load_b32(t.rsrc, rec << 2, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Exact scorer: ComputePlaneImageScore (depthmap.cc:441-478) with the
+// reference's FP32 operation order, given the FP32 homography H.
+// LinearInterpolation (depthmap.cc:14-30) returns 0 outside
+// [0, W-1) x [0, H-1): here the clamped fetch reads a zero texel there, the
+// bilinear blend of zeros is +0, and a non-finite coordinate (NaN blend) is
+// mapped to 0 as well.  Inside, floor(x) == (int)x and x - floor(x) is the
+// reference's dx bit for bit.
+
+template <int P>
+__device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
+                                             const float* H, int i, int j,
+                                             const QuadTex& tex) {
+  constexpr int hpz = (P - 1) / 2;
+  const float fj = (float)j, fi = (float)i;
+  const float u = H[0] * fj + H[1] * fi + H[2];
+  const float v = H[3] * fj + H[4] * fi + H[5];
+  const float w = H[6] * fj + H[7] * fi + H[8];
+  if (w == 0.0f) return -1.0f;
+  const float ww = w * w;
+  const float dfdx_x = __fdiv_rn(H[0] * w - H[6] * u, ww);
+  const float dfdx_y = __fdiv_rn(H[3] * w - H[6] * v, ww);
+  const float dfdy_x = __fdiv_rn(H[1] * w - H[7] * u, ww);
+  const float dfdy_y = __fdiv_rn(H[4] * w - H[7] * v, ww);
+  const float Hx0 = __fdiv_rn(u, w);
+  const float Hy0 = __fdiv_rn(v, w);
+
+  float cx[P], cy[P];
+#pragma unroll
+  for (int dx = -hpz; dx <= hpz; ++dx) {
+    cx[dx + hpz] = Hx0 + dfdx_x * (float)dx;
+    cy[dx + hpz] = Hy0 + dfdx_y * (float)dx;
+  }
+  const XUnpack<P> xk(rp);
+  float sumy = 0, sumyy = 0, sumxy = 0;
+#pragma unroll
+  for (int dy = -hpz; dy <= hpz; ++dy) {
+    const float rx = dfdy_x * (float)dy;
+    const float ry = dfdy_y * (float)dy;
+#pragma unroll
+    for (int dx = -hpz; dx <= hpz; ++dx) {
+      const int k = (dy + hpz) * P + (dx + hpz);
+      const float x2 = cx[dx + hpz] + rx;
+      const float y2 = cy[dx + hpz] + ry;
+      const float flx = floorf(x2), fly = floorf(y2);
+      const uint32_t t = fetch_quad(tex, flx, fly);
+      const float fx = x2 - flx;
+      const float fy = y2 - fly;
+      const float im0 = (1.0f - fx) * ub0(t) + fx * ub1(t);
+      const float im1 = (1.0f - fx) * ub2(t) + fx * ub3(t);
+      float y = (1.0f - fy) * im0 + fy * im1;
+      y = (y == y) ? y : 0.0f;
+      const float wy = rp.w[k] * y;
+      sumy += wy;
+      sumyy += wy * y;
+      sumxy += (rp.w[k] * xk(k)) * y;
+    }
+    // one patch row of gathers in flight at a time: bounds register use
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // NCCEstimator::Get (depthmap.cc:58-74)
+  const float meany = __fdiv_rn(sumy, rp.sumw);
+  const float meanyy = __fdiv_rn(sumyy, rp.sumw);
+  const float meanxy = __fdiv_rn(sumxy, rp.sumw);
+  const float vary = meanyy - meany * meany;
+  if (rp.varx < 0.1 || vary < 0.1) return -1.0f;
+  return __fdiv_rn(meanxy - rp.meanx * meany, __fsqrt_rn(rp.varx * vary));
+}
+
+// ---------------------------------------------------------------------------
+// Fast scorer for PatchMatch: the same local-affine bilateral NCC, with FMAs
+// and one-ulp reciprocals.  Takes the warp at the pixel: (Hx0, Hy0) and the
+// Jacobian columns.  A non-finite coordinate yields a NaN score, which never
+// wins a strict comparison.
+
+template <int P>
+__device__ __forceinline__ float score_fast_affine(
+    const RefPatch<P>& rp, float Hx0, float Hy0, float dfdx_x, float dfdx_y,
+    float dfdy_x, float dfdy_y, const QuadTex& tex) {
+  constexpr int hpz = (P - 1) / 2;
+  const XUnpack<P> xk(rp);
+  float sumy = 0, sumyy = 0, sumxy = 0;
+#pragma unroll
+  for (int dy = -hpz; dy <= hpz; ++dy) {
+    const float rx = __builtin_fmaf(dfdy_x, (float)dy, Hx0);
+    const float ry = __builtin_fmaf(dfdy_y, (float)dy, Hy0);
+#pragma unroll
+    for (int dx = -hpz; dx <= hpz; ++dx) {
+      const int k = (dy + hpz) * P + (dx + hpz);
+      const float x2 = __builtin_fmaf(dfdx_x, (float)dx, rx);
+      const float y2 = __builtin_fmaf(dfdx_y, (float)dx, ry);
+      const float flx = floorf(x2), fly = floorf(y2);
+      const uint32_t t = fetch_quad(tex, flx, fly);
+      const float fx = x2 - flx;
+      const float fy = y2 - fly;
+      const float a = ub0(t), b = ub1(t), c = ub2(t), d = ub3(t);
+      const float im0 = __builtin_fmaf(fx, b - a, a);
+      const float im1 = __builtin_fmaf(fx, d - c, c);
+      const float y = __builtin_fmaf(fy, im1 - im0, im0);
+      const float wy = rp.w[k] * y;
+      sumy += wy;
+      sumyy = __builtin_fmaf(wy, y, sumyy);
+      sumxy = __builtin_fmaf(xk(k), wy, sumxy);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const float meany = sumy * rp.rsumw;
+  const float meanyy = sumyy * rp.rsumw;
+  const float meanxy = sumxy * rp.rsumw;
+  const float vary = __builtin_fmaf(-meany, meany, meanyy);
+  if (vary < 0.1f) return -1.0f;
+  const float cov = __builtin_fmaf(-rp.meanx, meany, meanxy);
+  return cov * rsqrtf(rp.varx * vary);
+}
+
+}  // namespace pm

@@ -1,0 +1,24 @@
+// Launch wrappers of the engine's HIP kernels (defined in pm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pm_types.h"
+
+namespace pm {
+
+hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
+                             hipStream_t s);
+hipError_t launch_sweep(const PmArgs& a, const float* Htab, const float* planes,
+                        int nplanes, hipStream_t s);
+hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
+                            const float* pl, float* out, hipStream_t s);
+hipError_t launch_pm_init(const PmArgs& a, hipStream_t s);
+hipError_t launch_pm_half(const PmArgs& a, int colour, int iter, hipStream_t s);
+hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
+                       float* score, int* nghbr, hipStream_t s);
+hipError_t launch_clean(const ClView* views, const ClRef* ref, int n, int W,
+                        int H, float thr, int min_views, float* out,
+                        hipStream_t s);
+
+}  // namespace pm

@@ -1,0 +1,590 @@
+// HIP kernels of the PatchMatch depthmap engine for gfx950 (MI355X).
+//
+// Mapping: one lane per reference pixel, 64-lane wavefronts over 2D pixel
+// blocks (16 x 4 pixels per wave, 256-thread workgroups) so that the 49
+// bilinear gathers of a wave land in a compact region of the neighbour view
+// and hit the same L1/L2 lines.  The hypothesis-invariant reference-side
+// patch data (bilateral weights, w*x, the x moments) lives in registers for
+// the whole kernel.
+#include <hip/hip_runtime.h>
+
+#include "pm_device.h"
+#include "pm_kernels.h"
+
+namespace pm {
+
+// Adds each lane's count to *c with one atomic per wavefront.  Every lane of
+// the wave must reach this call.
+__device__ __forceinline__ void wave_count(unsigned long long* c, unsigned v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(c, (unsigned long long)v);
+}
+
+// ---------------------------------------------------------------------------
+// Zero-bordered quad texture from a u8 image (see pm_device.h).
+
+__global__ void k_build_quad(const uint8_t* __restrict__ img,
+                             uint32_t* __restrict__ quad, int W, int H) {
+  const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ty = blockIdx.y * blockDim.y + threadIdx.y;
+  if (tx > W || ty > H) return;
+  const int ix = tx - 1, iy = ty - 1;
+  uint32_t v = 0;
+  if (ix >= 0 && iy >= 0 && ix <= W - 2 && iy <= H - 2) {
+    const uint8_t* r0 = img + (size_t)iy * W + ix;
+    const uint8_t* r1 = r0 + W;
+    v = (uint32_t)r0[0] | ((uint32_t)r0[1] << 8) | ((uint32_t)r1[0] << 16) |
+        ((uint32_t)r1[1] << 24);
+  }
+  quad[(size_t)ty * (W + 1) + tx] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Plane sweep = ComputeBruteForce (depthmap.cc:184-200), exact FP32 order.
+// Htab[(d * nviews + v) * 9] is the FP32 homography of sweep plane d into
+// view v, composed on the host in FP64 exactly as the reference does;
+// planes[3 d] is the plane vector.  Scores start at 0 (AssignMatrices), so a
+// pixel whose best NCC is <= 0 keeps depth 0.
+
+template <int P>
+__device__ __forceinline__ unsigned sweep_pixel(const PmArgs& a,
+                                                const float* __restrict__ Htab,
+                                                const float* __restrict__ planes,
+                                                int nplanes) {
+  constexpr int hpz = (P - 1) / 2;
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i < hpz || i >= a.H - hpz || j < hpz || j >= a.W - hpz) return 0;
+  RefPatch<P> rp;
+  if (!load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) return 0;
+
+  float best_s = 0.0f;
+  int best_n = 0;
+  int best_d = -1;
+  for (int d = 0; d < nplanes; ++d) {
+    float s = -1.0f;
+    int n = 0;
+    for (int v = 1; v < a.nviews; ++v) {
+      const PmView& V = a.views[v];
+      const float* H = Htab + ((size_t)d * a.nviews + v) * 9;
+      float Hr[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Hr[k] = H[k];
+      const QuadTex tex = make_tex(V.quad, V.width, V.height);
+      const float sc = score_exact<P>(rp, Hr, i, j, tex);
+      if (sc > s) {
+        s = sc;
+        n = v;
+      }
+    }
+    if (s > best_s) {
+      best_s = s;
+      best_n = n;
+      best_d = d;
+    }
+  }
+  const size_t idx = (size_t)i * a.W + j;
+  if (best_d >= 0) {
+    const float v0 = planes[3 * best_d], v1 = planes[3 * best_d + 1],
+                v2 = planes[3 * best_d + 2];
+    const float depth = depth_of_plane_f64(a.Kinv0, v0, v1, v2, (double)j, (double)i);
+    reinterpret_cast<float4*>(a.state)[idx] = make_float4(v0, v1, v2, depth);
+    a.score[idx] = best_s;
+    a.nghbr[idx] = best_n;
+  }
+  return (unsigned)(nplanes * (a.nviews - 1));
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_sweep(PmArgs a,
+                                               const float* __restrict__ Htab,
+                                               const float* __restrict__ planes,
+                                               int nplanes) {
+  const unsigned n = sweep_pixel<P>(a, Htab, planes, nplanes);
+  if (a.counter) wave_count(a.counter, n);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-hypothesis scoring: ComputePlaneImageScore for a list of
+// (pixel, plane) pairs against one neighbour view, with the homography
+// composed in FP64 on the device in the reference's order.  out[k] is the
+// FP32 NCC score.  The host groups hypotheses by view (one launch each).
+
+template <int P>
+__global__ __launch_bounds__(256) void k_score_hyp(PmArgs a, int view, int n,
+                                                   const int* __restrict__ ij,
+                                                   const float* __restrict__ pl,
+                                                   float* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = ij[2 * k], j = ij[2 * k + 1];
+  RefPatch<P> rp;
+  load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut);
+  const PmView& V = a.views[view];
+  float H[9];
+  homography_baked_f64(a.Kinv0, V.Q, V.a, V.K, pl[3 * k], pl[3 * k + 1],
+                       pl[3 * k + 2], H);
+  const QuadTex tex = make_tex(V.quad, V.width, V.height);
+  out[k] = score_exact<P>(rp, H, i, j, tex);
+}
+
+// ---------------------------------------------------------------------------
+// PatchMatch (fast FP32 path).
+
+struct PixelRay {
+  float x, y, z;  // K_0^-1 (j, i, 1)
+};
+
+__device__ __forceinline__ PixelRay pixel_ray(const PmArgs& a, int i, int j) {
+  const float fj = (float)j, fi = (float)i;
+  PixelRay r;
+  r.x = __builtin_fmaf(a.Kinv0f[0], fj, __builtin_fmaf(a.Kinv0f[1], fi, a.Kinv0f[2]));
+  r.y = __builtin_fmaf(a.Kinv0f[3], fj, __builtin_fmaf(a.Kinv0f[4], fi, a.Kinv0f[5]));
+  r.z = __builtin_fmaf(a.Kinv0f[6], fj, __builtin_fmaf(a.Kinv0f[7], fi, a.Kinv0f[8]));
+  return r;
+}
+
+// PlaneFromDepthAndNormal (depthmap.cc:120-125) in FP32.
+__device__ __forceinline__ void plane_from_depth_normal(const PixelRay& r,
+                                                        float depth, float n0,
+                                                        float n1, float n2,
+                                                        float* v) {
+  const float px = depth * r.x, py = depth * r.y, pz = depth * r.z;
+  const float denom =
+      -__builtin_fmaf(n0, px, __builtin_fmaf(n1, py, n2 * pz));
+  const float inv = __frcp_rn(fmaxf(1e-6f, denom));
+  v[0] = n0 * inv;
+  v[1] = n1 * inv;
+  v[2] = n2 * inv;
+}
+
+// Score a plane against every neighbour view: max NCC with the lowest view
+// index winning ties (ComputePlaneScore, depthmap.cc:408-419).  Returns the
+// score; *nb gets the view and *depth the plane's depth at the pixel
+// (DepthOfPlaneBackprojection, depthmap.cc:114-118).
+template <int P>
+__device__ __forceinline__ float score_plane_all(const PmArgs& a,
+                                                 const RefPatch<P>& rp,
+                                                 const PixelRay& r, int i,
+                                                 int j, float v0, float v1,
+                                                 float v2, int* nb,
+                                                 float* depth) {
+  const float* Ki = a.Kinv0f;
+  // g = K_0^-T v, gq = g . (j, i, 1) = v . ray
+  const float g0 = __builtin_fmaf(v0, Ki[0], __builtin_fmaf(v1, Ki[3], v2 * Ki[6]));
+  const float g1 = __builtin_fmaf(v0, Ki[1], __builtin_fmaf(v1, Ki[4], v2 * Ki[7]));
+  const float gq = __builtin_fmaf(v0, r.x, __builtin_fmaf(v1, r.y, v2 * r.z));
+  *depth = __frcp_rn(fmaxf(1e-6f, -gq));
+  const float fj = (float)j, fi = (float)i;
+  float best = -1.0f;
+  int bn = 0;
+  for (int v = 1; v < a.nviews; ++v) {
+    const PmView& V = a.views[v];
+    const float* M = V.M;
+    const float* b = V.b;
+    const float u = __builtin_fmaf(b[0], gq, __builtin_fmaf(M[0], fj, __builtin_fmaf(M[1], fi, M[2])));
+    const float vv = __builtin_fmaf(b[1], gq, __builtin_fmaf(M[3], fj, __builtin_fmaf(M[4], fi, M[5])));
+    const float w = __builtin_fmaf(b[2], gq, __builtin_fmaf(M[6], fj, __builtin_fmaf(M[7], fi, M[8])));
+    float s = -1.0f;
+    if (w != 0.0f) {
+      const float H00 = __builtin_fmaf(b[0], g0, M[0]);
+      const float H01 = __builtin_fmaf(b[0], g1, M[1]);
+      const float H10 = __builtin_fmaf(b[1], g0, M[3]);
+      const float H11 = __builtin_fmaf(b[1], g1, M[4]);
+      const float H20 = __builtin_fmaf(b[2], g0, M[6]);
+      const float H21 = __builtin_fmaf(b[2], g1, M[7]);
+      const float rw = __frcp_rn(w);
+      const float Hx0 = u * rw, Hy0 = vv * rw;
+      // (H00 w - H20 u) / w^2 = (H00 - H20 u/w) / w
+      const float dfdx_x = __builtin_fmaf(-H20, Hx0, H00) * rw;
+      const float dfdx_y = __builtin_fmaf(-H20, Hy0, H10) * rw;
+      const float dfdy_x = __builtin_fmaf(-H21, Hx0, H01) * rw;
+      const float dfdy_y = __builtin_fmaf(-H21, Hy0, H11) * rw;
+      const QuadTex tex = make_tex(V.quad, V.width, V.height);
+      s = score_fast_affine<P>(rp, Hx0, Hy0, dfdx_x, dfdx_y, dfdy_x, dfdy_y,
+                               tex);
+    }
+    if (s > best) {
+      best = s;
+      bn = v;
+    }
+  }
+  *nb = bn;
+  return best;
+}
+
+__device__ __forceinline__ bool is_interior(const PmArgs& a, int i, int j) {
+  return i >= a.hpz && i < a.H - a.hpz && j >= a.hpz && j < a.W - a.hpz;
+}
+
+// Random initialisation + ignore mask (RandomInitialization and
+// ComputeIgnoreMask, depthmap.cc:237-281).  Masked and low-variance pixels
+// keep the zero state (the reference scores them and then zeroes them).
+template <int P>
+__device__ __forceinline__ unsigned init_pixel(const PmArgs& a) {
+  constexpr int hpz = (P - 1) / 2;
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (!is_interior(a, i, j)) return 0;
+  const size_t idx = (size_t)i * a.W + j;
+  if (a.ref_mask[idx] == 0) return 0;
+  // PatchVariance / Variance (depthmap.cc:32-44, 271-281): two-pass, FP32
+  float sum = 0;
+#pragma unroll
+  for (int u = -hpz; u <= hpz; ++u)
+#pragma unroll
+    for (int v = -hpz; v <= hpz; ++v) sum += (float)a.ref_img[(i + u) * a.W + j + v];
+  const float mean = __fdiv_rn(sum, (float)(P * P));
+  float sum2 = 0;
+#pragma unroll
+  for (int u = -hpz; u <= hpz; ++u)
+#pragma unroll
+    for (int v = -hpz; v <= hpz; ++v) {
+      const float dd = (float)a.ref_img[(i + u) * a.W + j + v] - mean;
+      sum2 += dd * dd;
+    }
+  if (__fdiv_rn(sum2, (float)(P * P)) < a.min_patch_var) return 0;
+
+  const u4 r = philox4x32_10(a.seed_lo ^ (uint32_t)idx, a.seed_hi,
+                             u4{0xFFFFFFFFu, 0u, 0u, 0u});
+  const float depth = expf(a.log_min + (a.log_max - a.log_min) * u01(r.x));
+  const float n0 = -1.0f + 2.0f * u01(r.y);
+  const float n1 = -1.0f + 2.0f * u01(r.z);
+  const PixelRay ray = pixel_ray(a, i, j);
+  float v[3];
+  plane_from_depth_normal(ray, depth, n0, n1, -1.0f, v);
+  RefPatch<P> rp;
+  float s = -1.0f;
+  int nb = 0;
+  unsigned count = 0;
+  if (load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) {
+    float dtmp;
+    s = score_plane_all<P>(a, rp, ray, i, j, v[0], v[1], v[2], &nb, &dtmp);
+    count = a.nviews - 1;
+  }
+  reinterpret_cast<float4*>(a.state)[idx] = make_float4(v[0], v[1], v[2], depth);
+  a.score[idx] = s;
+  a.nghbr[idx] = nb;
+  return count;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_pm_init(PmArgs a) {
+  const unsigned n = init_pixel<P>(a);
+  if (a.counter) wave_count(a.counter, n);
+}
+
+// One half of a red/black PatchMatch iteration: every interior pixel with
+// (i + j) % 2 == colour tests the planes of its four direct neighbours, the
+// best-scoring pixel among n_far odd-distance probes in each direction, and
+// then six random refinements of its current plane with the reference's
+// schedule (depth * exp(0.02 * 0.3^k N), slopes + 0.5 * 0.8^k N,
+// depthmap.cc:333-360).  Only opposite-colour pixels are read, so the
+// result does not depend on the order in which lanes run.
+template <int P>
+__device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
+                                               int iter) {
+  const int cx = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int j = 2 * cx + ((i + colour) & 1);
+  if (!is_interior(a, i, j)) return 0;
+  const size_t idx = (size_t)i * a.W + j;
+  const float4* st = reinterpret_cast<const float4*>(a.state);
+  float4 cur = st[idx];
+  if (cur.w == 0.0f) return 0;
+  RefPatch<P> rp;
+  if (!load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) return 0;
+  const PixelRay ray = pixel_ray(a, i, j);
+  float best_s = a.score[idx];
+  int best_n = a.nghbr[idx];
+
+  // Propagation candidates: the 4 direct neighbours and, per direction, the
+  // best-scoring of n_far probes at odd distances 3, 5, 7, ...
+  float4 cand[8];
+  bool have[8];
+  const int di[4] = {-1, 1, 0, 0};
+  const int dj[4] = {0, 0, -1, 1};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    cand[k] = st[(size_t)(i + di[k]) * a.W + (j + dj[k])];
+    have[k] = cand[k].w != 0.0f;
+    float bs = -2.0f;
+    size_t bidx = idx;
+    for (int m = 0; m < a.n_far; ++m) {
+      const int dist = 3 + 2 * m;
+      const int ii = i + di[k] * dist, jj = j + dj[k] * dist;
+      if (!is_interior(a, ii, jj)) break;
+      const size_t q = (size_t)ii * a.W + jj;
+      const float s = a.score[q];
+      if (s > bs) {
+        bs = s;
+        bidx = q;
+      }
+    }
+    cand[4 + k] = st[bidx];
+    have[4 + k] = bs > -2.0f && cand[4 + k].w != 0.0f;
+  }
+
+  // One loop over 8 propagation + 6 refinement candidates, so the scorer is
+  // inlined once.  Refinement k perturbs the current best plane with the
+  // reference's schedule: depth * exp(0.02 * 0.3^k N), slopes + 0.5 * 0.8^k N
+  // (depthmap.cc:333-360).
+  unsigned tries = 0;
+  float depth_range = 0.02f, normal_range = 0.5f;
+  for (int c = 0; c < 14; ++c) {
+    float v0, v1, v2;
+    bool go;
+    if (c < 8) {
+      v0 = cand[c].x;
+      v1 = cand[c].y;
+      v2 = cand[c].z;
+      go = have[c] && !(v0 == cur.x && v1 == cur.y && v2 == cur.z);
+    } else {
+      const int k = c - 8;
+      const u4 r = philox4x32_10(a.seed_lo ^ (uint32_t)idx, a.seed_hi,
+                                 u4{(uint32_t)iter, (uint32_t)k, 1u, 0u});
+      float g0, g1, g2, g3;
+      box_muller(r.x, r.y, &g0, &g1);
+      box_muller(r.z, r.w, &g2, &g3);
+      const float depth = cur.w * expf(depth_range * g0);
+      go = cur.z != 0.0f;
+      float v[3] = {0.0f, 0.0f, 0.0f};
+      if (go) {
+        const float nx = -cur.x / cur.z + normal_range * g1;
+        const float ny = -cur.y / cur.z + normal_range * g2;
+        plane_from_depth_normal(ray, depth, nx, ny, -1.0f, v);
+        depth_range *= 0.3f;
+        normal_range *= 0.8f;
+      }
+      v0 = v[0];
+      v1 = v[1];
+      v2 = v[2];
+    }
+    if (go) {
+      int nb;
+      float dep;
+      const float s = score_plane_all<P>(a, rp, ray, i, j, v0, v1, v2, &nb, &dep);
+      ++tries;
+      if (s > best_s) {
+        best_s = s;
+        best_n = nb;
+        cur = make_float4(v0, v1, v2, dep);
+      }
+    }
+  }
+  reinterpret_cast<float4*>(a.state)[idx] = cur;
+  a.score[idx] = best_s;
+  a.nghbr[idx] = best_n;
+  return tries * (unsigned)(a.nviews - 1);
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour,
+                                                 int iter) {
+  const unsigned n = half_pixel<P>(a, colour, iter);
+  if (a.counter) wave_count(a.counter, n);
+}
+
+// ---------------------------------------------------------------------------
+// Output packing with the optional PostProcess (depthmap.cc:489-502):
+// 5x5 median with replicate border, then depth = 0 where d == 0 or
+// |d - m| / d > 0.05.  Plane/score/nghbr are copied unchanged.
+
+__device__ __forceinline__ void cswap(float& x, float& y) {
+  const float lo = fminf(x, y), hi = fmaxf(x, y);
+  x = lo;
+  y = hi;
+}
+
+__global__ __launch_bounds__(256) void k_pack(PmArgs a, int median,
+                                              float* __restrict__ out_depth,
+                                              float* __restrict__ out_plane,
+                                              float* __restrict__ out_score,
+                                              int* __restrict__ out_nghbr) {
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i >= a.H || j >= a.W) return;
+  const size_t idx = (size_t)i * a.W + j;
+  const float4* st = reinterpret_cast<const float4*>(a.state);
+  const float4 s = st[idx];
+  float d = s.w;
+  if (median) {
+    float win[25];
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      const int ii = min(max(i + u - 2, 0), a.H - 1);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const int jj = min(max(j + v - 2, 0), a.W - 1);
+        win[u * 5 + v] = st[(size_t)ii * a.W + jj].w;
+      }
+    }
+    // odd-even transposition sort (25 rounds) -> exact median win[12]
+#pragma unroll
+    for (int r = 0; r < 25; ++r) {
+#pragma unroll
+      for (int k = (r & 1); k + 1 < 25; k += 2) cswap(win[k], win[k + 1]);
+    }
+    const float m = win[12];
+    if (d == 0.0f || __fdiv_rn(fabsf(d - m), d) > 0.05) d = 0.0f;
+  }
+  out_depth[idx] = d;
+  out_plane[3 * idx + 0] = s.x;
+  out_plane[3 * idx + 1] = s.y;
+  out_plane[3 * idx + 2] = s.z;
+  out_score[idx] = a.score[idx];
+  out_nghbr[idx] = a.nghbr[idx];
+}
+
+// ---------------------------------------------------------------------------
+// DepthmapCleaner::Clean (depthmap.cc:524-555): FP64 back-projection and
+// projection, FP32 bilinear of the other depthmaps, vote counting from 1.
+
+__device__ __forceinline__ float lerp_depth(const float* __restrict__ img,
+                                            int W, int H, float y, float x) {
+  if (!(x >= 0.0f) || x >= W - 1 || !(y >= 0.0f) || y >= H - 1) return 0.0f;
+  const int ix = (int)x, iy = (int)y;
+  const float dx = x - ix, dy = y - iy;
+  const float* r0 = img + (size_t)iy * W + ix;
+  const float* r1 = r0 + W;
+  const float im0 = (1 - dx) * r0[0] + dx * r0[1];
+  const float im1 = (1 - dx) * r1[0] + dx * r1[1];
+  return (1 - dy) * im0 + dy * im1;
+}
+
+__global__ __launch_bounds__(256) void k_clean(const ClView* __restrict__ views,
+                                               const ClRef* __restrict__ ref,
+                                               int n, float thr, int min_views,
+                                               float* __restrict__ out) {
+  const ClView& V0 = views[0];
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i >= V0.height || j >= V0.width) return;
+  const size_t idx = (size_t)i * V0.width + j;
+  const float depth = V0.depth[idx];
+  // Backproject (depthmap.cc:109-112): R^T (depth K^-1 (x, y, 1) - t)
+  double S[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) S[k] = ref->Kinv[k] * (double)depth;
+  const double q[3] = {(double)j, (double)i, 1.0};
+  double X[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s += S[r * 3 + k] * q[k];
+    X[r] = s - V0.t[r];
+  }
+  double Y[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s += ref->Rt[r * 3 + k] * X[k];
+    Y[r] = s;
+  }
+  const float p0 = (float)Y[0], p1 = (float)Y[1], p2 = (float)Y[2];
+  int consistent = 1;
+  for (int o = 1; o < n; ++o) {
+    const ClView& Vo = views[o];
+    const double pd[3] = {(double)p0, (double)p1, (double)p2};
+    double Z[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s += Vo.R[r * 3 + k] * pd[k];
+      Z[r] = s + Vo.t[r];
+    }
+    double U[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s += Vo.K[r * 3 + k] * Z[k];
+      U[r] = s;
+    }
+    const float r0 = (float)U[0], r1 = (float)U[1], r2 = (float)U[2];
+    if (r2 < 1e-8 || isnan(r2)) continue;
+    const float u = __fdiv_rn(r0, r2);
+    const float v = __fdiv_rn(r1, r2);
+    const float dr = lerp_depth(Vo.depth, Vo.width, Vo.height, v, u);
+    if (fabsf(dr - r2) < r2 * thr) consistent++;
+  }
+  out[idx] = consistent >= min_views ? depth : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers
+
+static inline dim3 grid16(int w, int h) {
+  return dim3((w + 15) / 16, (h + 15) / 16);
+}
+
+hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
+                             hipStream_t s) {
+  dim3 blk(16, 16);
+  dim3 grd((W + 1 + 15) / 16, (H + 1 + 15) / 16);
+  hipLaunchKernelGGL(k_build_quad, grd, blk, 0, s, img, quad, W, H);
+  return hipGetLastError();
+}
+
+#define PM_DISPATCH_P(P_, CALL)         \
+  switch (P_) {                         \
+    case 3: { constexpr int P = 3; CALL; } break; \
+    case 5: { constexpr int P = 5; CALL; } break; \
+    case 7: { constexpr int P = 7; CALL; } break; \
+    case 9: { constexpr int P = 9; CALL; } break; \
+    case 11: { constexpr int P = 11; CALL; } break; \
+    default: return hipErrorInvalidValue; \
+  }
+
+hipError_t launch_sweep(const PmArgs& a, const float* Htab, const float* planes,
+                        int nplanes, hipStream_t s) {
+  const int P = 2 * a.hpz + 1;
+  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_sweep<P>, grid16(a.W, a.H), dim3(256),
+                                      0, s, a, Htab, planes, nplanes));
+  return hipGetLastError();
+}
+
+hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
+                            const float* pl, float* out, hipStream_t s) {
+  const int P = 2 * a.hpz + 1;
+  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_score_hyp<P>, dim3((n + 255) / 256),
+                                      dim3(256), 0, s, a, view, n, ij, pl, out));
+  return hipGetLastError();
+}
+
+hipError_t launch_pm_init(const PmArgs& a, hipStream_t s) {
+  const int P = 2 * a.hpz + 1;
+  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_init<P>, grid16(a.W, a.H), dim3(256),
+                                      0, s, a));
+  return hipGetLastError();
+}
+
+hipError_t launch_pm_half(const PmArgs& a, int colour, int iter,
+                          hipStream_t s) {
+  const int P = 2 * a.hpz + 1;
+  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_half<P>,
+                                      grid16((a.W + 1) / 2, a.H), dim3(256), 0,
+                                      s, a, colour, iter));
+  return hipGetLastError();
+}
+
+hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
+                       float* score, int* nghbr, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack, grid16(a.W, a.H), dim3(256), 0, s, a, median,
+                     depth, plane, score, nghbr);
+  return hipGetLastError();
+}
+
+hipError_t launch_clean(const ClView* views, const ClRef* ref, int n, int W,
+                        int H, float thr, int min_views, float* out,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_clean, grid16(W, H), dim3(256), 0, s, views, ref, n, thr,
+                     min_views, out);
+  return hipGetLastError();
+}
+
+}  // namespace pm

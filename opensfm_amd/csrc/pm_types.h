@@ -1,0 +1,56 @@
+// Plain-data structures shared by the host code and the kernels.
+#pragma once
+
+#include <stdint.h>
+
+namespace pm {
+
+constexpr int kMaxViews = 64;  // reference view + up to 63 neighbours
+
+// One view as the kernels see it.  For neighbour view v (v >= 1) the plane-
+// induced homography of plane v_pi (in the reference camera frame) is
+//   H_v = K_v (Q_v + a_v v_pi^T) K_0^-1 = M_v + b_v (K_0^-T v_pi)^T
+// with Q_v = R_v R_0^T, a_v = Q_v t_0 - t_v (depthmap.cc:153-155),
+// M_v = K_v Q_v K_0^-1 and b_v = K_v a_v composed in FP64 on the host.
+struct PmView {
+  const uint32_t* quad;  // zero-bordered 2x2 texel quads, pitch width + 1
+  int width, height;
+  float M[9];
+  float b[3];
+  double K[9], Q[9], a[3];  // FP64 inputs of the exact homography
+};
+
+// Arguments common to the estimator kernels (passed by value).
+struct PmArgs {
+  int W, H;            // reference view size
+  int nviews;          // views including the reference
+  int hpz;             // (patch_size - 1) / 2
+  const PmView* views;  // device table [nviews]
+  const uint8_t* ref_img;
+  const uint8_t* ref_mask;
+  const float* wlut;   // bilateral weight table, 256 x (2 hpz^2 + 1)
+  float* state;        // 4 floats per pixel: plane v (3) and depth
+  float* score;
+  int* nghbr;
+  float Kinv0f[9];     // K_0^-1 in FP32 (fast path)
+  double Kinv0[9];     // K_0^-1 in FP64 (exact path)
+  float log_min, log_max;
+  float min_patch_var;
+  uint32_t seed_lo, seed_hi;
+  int n_far;           // far propagation probes per direction
+  unsigned long long* counter;  // scorings executed (nullptr: not counted)
+};
+
+// Cleaner view (DepthmapCleaner::AddView, depthmap.cc:515-522).
+struct ClView {
+  const float* depth;
+  int width, height;
+  double K[9], R[9], t[3];
+};
+
+// Inverse pose of the view being cleaned: K_0^-1 and R_0^T.
+struct ClRef {
+  double Kinv[9], Rt[9];
+};
+
+}  // namespace pm

@@ -1,0 +1,194 @@
+"""``pydense``-compatible surface of the MI355X depthmap engine.
+
+Mirrors ``pydense.DepthmapEstimator`` and ``pydense.DepthmapCleaner``
+(opensfm/src/lib/dense/pydense.pyi:28-60; C++ classes in
+opensfm/src/lib/dense/depthmap.h:57-129): same method names, argument order,
+defaults and return layout, so code written against the reference reads the
+same.  Every compute runs on the GPU through the C ABI in
+include/opensfm_amd.h; the calls release the GIL (ctypes does) like the
+reference's bindings do (depthmap_bind.h:57-60).
+
+Additive API: ``compute_plane_sweep`` (alias of ``compute_brute_force``),
+``set_seed``, ``set_far_probes``, ``score_hypotheses``, ``stats`` and the
+``device`` constructor argument.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+
+
+def _mat(a, shape, name):
+    a = np.asarray(a, dtype=np.float64)
+    if a.size != int(np.prod(shape)):
+        raise ValueError(f"{name} must have {int(np.prod(shape))} elements, got shape {a.shape}")
+    return np.ascontiguousarray(a.reshape(shape))
+
+
+class DepthmapEstimator:
+    """PatchMatch / plane-sweep depthmap estimator (depthmap.h:57-111)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _capi.load()
+        h = C.c_void_p()
+        _capi.check(self._lib.pm_estimator_create(int(device), C.byref(h)))
+        self._h = h
+        self._shape = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.pm_estimator_destroy(h)
+            self._h = None
+
+    # -- DepthmapEstimator::AddView (depthmap.cc:146-162)
+    def add_view(self, K, R, t, image, mask) -> None:
+        K = _mat(K, (3, 3), "K")
+        R = _mat(R, (3, 3), "R")
+        t = _mat(t, (3,), "t")
+        image = np.asarray(image)
+        mask = np.asarray(mask)
+        if image.ndim != 2 or image.dtype != np.uint8:
+            raise ValueError("image must be a 2-D uint8 array")
+        if mask.shape != image.shape or mask.dtype != np.uint8:
+            raise ValueError("mask must be a uint8 array of the image's shape")
+        image = np.ascontiguousarray(image)
+        mask = np.ascontiguousarray(mask)
+        h, w = image.shape
+        _capi.check(self._lib.pm_estimator_add_view(
+            self._h, _capi.ptr(K, C.c_double), _capi.ptr(R, C.c_double),
+            _capi.ptr(t, C.c_double), _capi.ptr(image, C.c_uint8),
+            _capi.ptr(mask, C.c_uint8), w, h, w))
+        if self._shape is None:
+            self._shape = (h, w)
+
+    def set_depth_range(self, min_depth: float, max_depth: float,
+                        num_depth_planes: int) -> None:
+        _capi.check(self._lib.pm_estimator_set_depth_range(
+            self._h, float(min_depth), float(max_depth), int(num_depth_planes)))
+
+    def set_patchmatch_iterations(self, n: int) -> None:
+        _capi.check(self._lib.pm_estimator_set_patchmatch_iterations(self._h, int(n)))
+
+    def set_patch_size(self, size: int) -> None:
+        _capi.check(self._lib.pm_estimator_set_patch_size(self._h, int(size)))
+
+    def set_min_patch_sd(self, sd: float) -> None:
+        _capi.check(self._lib.pm_estimator_set_min_patch_sd(self._h, float(sd)))
+
+    def set_seed(self, seed: int) -> None:
+        _capi.check(self._lib.pm_estimator_set_seed(self._h, int(seed) & (2 ** 64 - 1)))
+
+    def set_far_probes(self, n: int) -> None:
+        _capi.check(self._lib.pm_estimator_set_far_probes(self._h, int(n)))
+
+    def set_post_process(self, enable: bool) -> None:
+        _capi.check(self._lib.pm_estimator_set_post_process(self._h, int(bool(enable))))
+
+    def _alloc(self):
+        if self._shape is None:
+            raise ValueError("need a reference view and at least one neighbour view (got 0)")
+        h, w = self._shape
+        return (np.zeros((h, w), np.float32), np.zeros((h, w, 3), np.float32),
+                np.zeros((h, w), np.float32), np.zeros((h, w), np.int32))
+
+    def _compute(self, mode):
+        depth, plane, score, nghbr = self._alloc()
+        _capi.check(self._lib.pm_estimator_compute(
+            self._h, mode, _capi.ptr(depth, C.c_float), _capi.ptr(plane, C.c_float),
+            _capi.ptr(score, C.c_float), _capi.ptr(nghbr, C.c_int32)))
+        return [depth, plane, score, nghbr]
+
+    # -- ComputeBruteForce / ComputePatchMatch / ComputePatchMatchSample
+    def compute_brute_force(self) -> list:
+        return self._compute(_capi.PM_MODE_PLANE_SWEEP)
+
+    compute_plane_sweep = compute_brute_force
+
+    def compute_patch_match(self) -> list:
+        return self._compute(_capi.PM_MODE_PATCHMATCH)
+
+    def compute_patch_match_sample(self) -> list:
+        return self._compute(_capi.PM_MODE_PATCHMATCH_SAMPLE)
+
+    # -- device-resident variants (timing)
+    def run(self, mode: int) -> None:
+        _capi.check(self._lib.pm_estimator_run(self._h, int(mode)))
+
+    def download(self) -> list:
+        depth, plane, score, nghbr = self._alloc()
+        _capi.check(self._lib.pm_estimator_download(
+            self._h, _capi.ptr(depth, C.c_float), _capi.ptr(plane, C.c_float),
+            _capi.ptr(score, C.c_float), _capi.ptr(nghbr, C.c_int32)))
+        return [depth, plane, score, nghbr]
+
+    def score_hypotheses(self, ij, planes, views) -> np.ndarray:
+        """ComputePlaneImageScore (depthmap.cc:441-478) of fixed hypotheses:
+        ij (n, 2) = (row, col), planes (n, 3), views (n,) in 1..N-1."""
+        ij = np.ascontiguousarray(np.asarray(ij, np.int32).reshape(-1, 2))
+        n = ij.shape[0]
+        planes = np.ascontiguousarray(np.asarray(planes, np.float32).reshape(n, 3))
+        views = np.ascontiguousarray(np.asarray(views, np.int32).reshape(n))
+        out = np.zeros(n, np.float32)
+        _capi.check(self._lib.pm_estimator_score_hypotheses(
+            self._h, n, _capi.ptr(ij, C.c_int32), _capi.ptr(planes, C.c_float),
+            _capi.ptr(views, C.c_int32), _capi.ptr(out, C.c_float)))
+        return out
+
+    def stats(self) -> dict:
+        s = _capi.pm_stats()
+        _capi.check(self._lib.pm_estimator_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+
+class DepthmapCleaner:
+    """Cross-view depth consistency check (depthmap.h:113-129)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _capi.load()
+        h = C.c_void_p()
+        _capi.check(self._lib.pm_cleaner_create(int(device), C.byref(h)))
+        self._h = h
+        self._shape = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.pm_cleaner_destroy(h)
+            self._h = None
+
+    def set_same_depth_threshold(self, t: float) -> None:
+        _capi.check(self._lib.pm_cleaner_set_same_depth_threshold(self._h, float(t)))
+
+    def set_min_consistent_views(self, n: int) -> None:
+        _capi.check(self._lib.pm_cleaner_set_min_consistent_views(self._h, int(n)))
+
+    def add_view(self, K, R, t, depth) -> None:
+        K = _mat(K, (3, 3), "K")
+        R = _mat(R, (3, 3), "R")
+        t = _mat(t, (3,), "t")
+        depth = np.asarray(depth)
+        if depth.ndim != 2:
+            raise ValueError("depth must be a 2-D array")
+        depth = np.ascontiguousarray(depth, dtype=np.float32)
+        h, w = depth.shape
+        _capi.check(self._lib.pm_cleaner_add_view(
+            self._h, _capi.ptr(K, C.c_double), _capi.ptr(R, C.c_double),
+            _capi.ptr(t, C.c_double), _capi.ptr(depth, C.c_float), w, h))
+        if self._shape is None:
+            self._shape = (h, w)
+
+    def clean(self) -> np.ndarray:
+        if self._shape is None:
+            raise ValueError("no views added")
+        out = np.zeros(self._shape, np.float32)
+        _capi.check(self._lib.pm_cleaner_clean(self._h, _capi.ptr(out, C.c_float)))
+        return out
+
+    def last_ms(self) -> float:
+        ms = C.c_float(0)
+        _capi.check(self._lib.pm_cleaner_last_ms(self._h, C.byref(ms)))
+        return ms.value

@@ -1,0 +1,258 @@
+"""CPU oracle for the dense depthmap path -- TEST INFRASTRUCTURE ONLY.
+
+A ctypes wrapper over ``oracle/pm_oracle.c``, a plain-C restatement of
+OpenSfM's legacy ``dense::DepthmapEstimator`` / ``dense::DepthmapCleaner``
+(opensfm/src/lib/dense/src/depthmap.cc:8-555).  Only ``tests/``,
+``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` import
+this package; the engine in ``opensfm_amd`` never does.
+
+Parity pins (each checked in tests/test_oracle.py):
+  inv3, homography_baked, plane_from_depth_and_normal,
+  depth_of_plane_backprojection  -- closed forms of doc/dense_matching.md
+  linear_interp_u8/f32          -- textbook bilinear + zero-outside rule
+  variance                      -- numpy.var
+  ncc / bilateral_weight        -- independent FP64 weighted Pearson correlation
+  score / score_f64             -- identity-camera and affine-intensity cases,
+                                   FP64 twin, brute-force numpy re-statement
+  compute_brute_force           -- textured fronto-parallel plane at a sweep depth
+  median5                       -- scipy.ndimage.median_filter(mode="nearest")
+  clean                         -- FP64 numpy vote count, identical/perturbed views
+  compute_patch_match           -- synthetic ground truth and invariants
+Parity unpinned at the last-ulp level (no reference binary can be built here:
+depthmap.cc needs OpenCV, which the image lacks): the exact rounding of
+cv::Matx33d::inv and the expf/exp overload choice in BilateralWeight.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libpm_oracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []),
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    L.orc_create.restype = C.c_void_p
+    L.orc_destroy.argtypes = [C.c_void_p]
+    L.orc_add_view.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _u8p, _u8p,
+                               C.c_int, C.c_int]
+    L.orc_set_depth_range.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+    L.orc_set_patchmatch_iterations.argtypes = [C.c_void_p, C.c_int]
+    L.orc_set_patch_size.argtypes = [C.c_void_p, C.c_int]
+    L.orc_set_min_patch_sd.argtypes = [C.c_void_p, C.c_float]
+    L.orc_set_seed.argtypes = [C.c_void_p, C.c_uint64]
+    L.orc_last_scorings.argtypes = [C.c_void_p]
+    L.orc_last_scorings.restype = C.c_int64
+    L.orc_compute.argtypes = [C.c_void_p, C.c_int, _f32p, _f32p, _f32p, _i32p,
+                              C.c_int]
+    L.orc_score.argtypes = [C.c_void_p, C.c_int, C.c_int, _f32p, C.c_int]
+    L.orc_score.restype = C.c_float
+    L.orc_score_f64.argtypes = [C.c_void_p, C.c_int, C.c_int, _f64p, C.c_int]
+    L.orc_score_f64.restype = C.c_double
+    L.orc_inv3.argtypes = [_f64p, _f64p]
+    L.orc_inv3.restype = C.c_int
+    L.orc_linear_interp_u8.argtypes = [_u8p, C.c_int, C.c_int, C.c_float, C.c_float]
+    L.orc_linear_interp_u8.restype = C.c_float
+    L.orc_linear_interp_f32.argtypes = [_f32p, C.c_int, C.c_int, C.c_float, C.c_float]
+    L.orc_linear_interp_f32.restype = C.c_float
+    L.orc_variance.argtypes = [_f32p, C.c_int]
+    L.orc_variance.restype = C.c_float
+    L.orc_ncc.argtypes = [_f32p, _f32p, _f32p, C.c_int]
+    L.orc_ncc.restype = C.c_float
+    L.orc_bilateral_weight.argtypes = [C.c_float, C.c_float, C.c_float]
+    L.orc_bilateral_weight.restype = C.c_float
+    L.orc_homography_baked.argtypes = [_f64p, _f64p, _f64p, _f64p, _f32p, _f32p]
+    L.orc_plane_from_depth_and_normal.argtypes = [C.c_float, C.c_float, _f64p,
+                                                  C.c_float, _f32p, _f32p]
+    L.orc_depth_of_plane_backprojection.argtypes = [C.c_double, C.c_double,
+                                                    _f64p, _f32p]
+    L.orc_depth_of_plane_backprojection.restype = C.c_float
+    L.orc_median5.argtypes = [_f32p, _f32p, C.c_int, C.c_int]
+    L.orc_clean.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.POINTER(C.c_void_p),
+                            _i32p, _i32p, C.c_float, C.c_int, _f32p]
+    L.orc_clean.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _f64(a, n):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+    assert a.size == n, a.shape
+    return a
+
+
+def inv3(K):
+    out = np.zeros(9)
+    ok = lib().orc_inv3(_f64(K, 9), out)
+    if not ok:
+        raise ValueError("singular matrix")
+    return out.reshape(3, 3)
+
+
+def linear_interp_u8(img, y, x):
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    return lib().orc_linear_interp_u8(img, img.shape[1], img.shape[0], y, x)
+
+
+def linear_interp_f32(img, y, x):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    return lib().orc_linear_interp_f32(img, img.shape[1], img.shape[0], y, x)
+
+
+def variance(x):
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    return lib().orc_variance(x, x.size)
+
+
+def ncc(x, y, w):
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+    w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+    return lib().orc_ncc(x, y, w, x.size)
+
+
+def bilateral_weight(dcolor, dx, dy):
+    return lib().orc_bilateral_weight(dcolor, dx, dy)
+
+
+def homography_baked(K1inv, Q2, a2, K2, v):
+    H = np.zeros(9, np.float32)
+    lib().orc_homography_baked(_f64(K1inv, 9), _f64(Q2, 9), _f64(a2, 3),
+                               _f64(K2, 9),
+                               np.ascontiguousarray(v, dtype=np.float32), H)
+    return H.reshape(3, 3)
+
+
+def plane_from_depth_and_normal(x, y, K, depth, normal):
+    out = np.zeros(3, np.float32)
+    lib().orc_plane_from_depth_and_normal(
+        x, y, _f64(K, 9), depth, np.ascontiguousarray(normal, dtype=np.float32),
+        out)
+    return out
+
+
+def depth_of_plane_backprojection(x, y, K, plane):
+    return lib().orc_depth_of_plane_backprojection(
+        x, y, _f64(K, 9), np.ascontiguousarray(plane, dtype=np.float32))
+
+
+def median5(img):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.zeros_like(img)
+    lib().orc_median5(img, out, img.shape[1], img.shape[0])
+    return out
+
+
+def clean(views, same_depth_threshold=0.01, min_consistent_views=2):
+    """views: list of (K, R, t, depth[H,W] float32); view 0 is cleaned."""
+    n = len(views)
+    Ks = np.ascontiguousarray(np.stack([np.asarray(v[0], np.float64).reshape(9) for v in views]))
+    Rs = np.ascontiguousarray(np.stack([np.asarray(v[1], np.float64).reshape(9) for v in views]))
+    ts = np.ascontiguousarray(np.stack([np.asarray(v[2], np.float64).reshape(3) for v in views]))
+    depths = [np.ascontiguousarray(v[3], dtype=np.float32) for v in views]
+    ptrs = (C.c_void_p * n)(*[d.ctypes.data for d in depths])
+    ws = np.array([d.shape[1] for d in depths], np.int32)
+    hs = np.array([d.shape[0] for d in depths], np.int32)
+    out = np.zeros_like(depths[0])
+    rc = lib().orc_clean(n, Ks.reshape(-1), Rs.reshape(-1), ts.reshape(-1),
+                         ptrs, ws, hs, same_depth_threshold,
+                         min_consistent_views, out)
+    if rc != 0:
+        raise ValueError("clean needs at least one view")
+    return out
+
+
+class OracleEstimator:
+    """Mirror of pydense.DepthmapEstimator (pydense.pyi:43-60) over the oracle."""
+
+    def __init__(self):
+        self._L = lib()
+        self._h = self._L.orc_create()
+        self._shape = None
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.orc_destroy(self._h)
+            self._h = None
+
+    def add_view(self, K, R, t, image, mask):
+        image = np.ascontiguousarray(image, dtype=np.uint8)
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        if self._shape is None:
+            self._shape = image.shape
+        self._L.orc_add_view(self._h, _f64(K, 9), _f64(R, 9), _f64(t, 3), image,
+                             mask, image.shape[1], image.shape[0])
+
+    def set_depth_range(self, mn, mx, n):
+        self._L.orc_set_depth_range(self._h, mn, mx, n)
+
+    def set_patchmatch_iterations(self, n):
+        self._L.orc_set_patchmatch_iterations(self._h, n)
+
+    def set_patch_size(self, s):
+        self._L.orc_set_patch_size(self._h, s)
+
+    def set_min_patch_sd(self, sd):
+        self._L.orc_set_min_patch_sd(self._h, sd)
+
+    def set_seed(self, seed):
+        self._L.orc_set_seed(self._h, seed)
+
+    @property
+    def last_scorings(self):
+        return self._L.orc_last_scorings(self._h)
+
+    def _compute(self, mode, stop_after=-1):
+        h, w = self._shape
+        depth = np.zeros((h, w), np.float32)
+        plane = np.zeros((h, w, 3), np.float32)
+        score = np.zeros((h, w), np.float32)
+        nghbr = np.zeros((h, w), np.int32)
+        self._L.orc_compute(self._h, mode, depth, plane.reshape(-1), score, nghbr,
+                            stop_after)
+        return [depth, plane, score, nghbr]
+
+    def compute_brute_force(self):
+        return self._compute(0)
+
+    compute_plane_sweep = compute_brute_force
+
+    def compute_patch_match(self, stop_after_iterations=-1):
+        return self._compute(1, stop_after_iterations)
+
+    def compute_patch_match_sample(self):
+        return self._compute(2)
+
+    def score(self, i, j, plane, other):
+        return self._L.orc_score(self._h, i, j,
+                                 np.ascontiguousarray(plane, dtype=np.float32),
+                                 other)
+
+    def score_f64(self, i, j, plane, other):
+        return self._L.orc_score_f64(self._h, i, j,
+                                     np.ascontiguousarray(plane, dtype=np.float64),
+                                     other)

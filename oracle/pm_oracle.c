@@ -1,0 +1,801 @@
+/*
+ * pm_oracle.c -- CPU restatement of OpenSfM's legacy dense depthmap path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP engine in
+ * opensfm_amd/: only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it.  The product never links or calls it.
+ *
+ * It restates, function by function, opensfm/src/lib/dense/src/depthmap.cc
+ * (and the declarations in opensfm/src/lib/dense/depthmap.h) in plain C with
+ * the reference's arithmetic: FP64 where the reference uses cv::Matx33d /
+ * cv::Vec3d, FP32 where it uses float / cv::Vec3f / cv::Matx33f, the same
+ * operation order, and no FMA contraction (the Makefile builds with
+ * -ffp-contract=off, as the reference's x86-64 build has no -march flag and
+ * therefore no FMA).
+ *
+ * Third-party arithmetic restated here (OpenCV 4.13.0, conda-linux-64.lock,
+ * not vendored in the reference):
+ *   - cv::Matx products: accumulate from 0 in k order        (mat3_mul)
+ *   - cv::Matx33d::inv(): 3x3 cofactor / determinant inverse (inv3)
+ *   - Vec3f / float: multiply by the float reciprocal        (plane_from_depth_and_normal)
+ *   - cv::medianBlur(CV_32F, 5): exact 5x5 median, replicate border (median5)
+ * exp/sqrt/fabs on float arguments resolve to the float overloads (expf,
+ * sqrtf, fabsf); with the double overloads the results differ by < 1 ulp.
+ *
+ * Deviations from the reference, all on inputs where the reference is
+ * undefined or non-reproducible:
+ *   - the random source is a seeded splitmix64 stream instead of rand() and
+ *     std::mt19937(std::random_device) (depthmap.cc:127-132, 141);
+ *   - LinearInterpolation treats a NaN coordinate as outside the image (the
+ *     reference casts NaN to int, which is undefined behaviour).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_MAX_PATCH 15
+
+/* ------------------------------------------------------------------ */
+/* 3x3 FP64 helpers in OpenCV's accumulation order                     */
+/* ------------------------------------------------------------------ */
+
+/* C = A * B, row-major, s = 0; s += a(i,k) * b(k,j)  (cv::Matx_MatMulOp) */
+static void mat3_mul(const double* A, const double* B, double* C) {
+  double T[9];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += A[i * 3 + k] * B[k * 3 + j];
+      T[i * 3 + j] = s;
+    }
+  }
+  memcpy(C, T, sizeof(T));
+}
+
+/* y = A * x  (cv::Matx33d * cv::Vec3d) */
+static void mat3_vec(const double* A, const double* x, double* y) {
+  double t[3];
+  for (int i = 0; i < 3; ++i) {
+    double s = 0;
+    for (int k = 0; k < 3; ++k) s += A[i * 3 + k] * x[k];
+    t[i] = s;
+  }
+  memcpy(y, t, sizeof(t));
+}
+
+static void mat3_transpose(const double* A, double* B) {
+  double T[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) T[j * 3 + i] = A[i * 3 + j];
+  memcpy(B, T, sizeof(T));
+}
+
+/* cv::Matx33d::inv() with DECOMP_LU: for 3x3 OpenCV uses the determinant and
+ * cofactors (cv::internal::Matx_FastInvOp<_Tp, 3>).  Returns 0 if singular. */
+int orc_inv3(const double* a, double* b) {
+#define A(r, c) a[(r) * 3 + (c)]
+  double d = A(0, 0) * (A(1, 1) * A(2, 2) - A(2, 1) * A(1, 2)) -
+             A(0, 1) * (A(1, 0) * A(2, 2) - A(2, 0) * A(1, 2)) +
+             A(0, 2) * (A(1, 0) * A(2, 1) - A(2, 0) * A(1, 1));
+  if (d == 0) return 0;
+  d = 1 / d;
+  double t[9];
+  t[0] = (A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1)) * d;
+  t[1] = (A(0, 2) * A(2, 1) - A(0, 1) * A(2, 2)) * d;
+  t[2] = (A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1)) * d;
+  t[3] = (A(1, 2) * A(2, 0) - A(1, 0) * A(2, 2)) * d;
+  t[4] = (A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0)) * d;
+  t[5] = (A(0, 2) * A(1, 0) - A(0, 0) * A(1, 2)) * d;
+  t[6] = (A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0)) * d;
+  t[7] = (A(0, 1) * A(2, 0) - A(0, 0) * A(2, 1)) * d;
+  t[8] = (A(0, 0) * A(1, 1) - A(0, 1) * A(1, 0)) * d;
+#undef A
+  memcpy(b, t, sizeof(t));
+  return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Pixel-level primitives                                               */
+/* ------------------------------------------------------------------ */
+
+/* LinearInterpolation<unsigned char>  depthmap.cc:14-30 */
+float orc_linear_interp_u8(const uint8_t* img, int width, int height, float y,
+                           float x) {
+  if (!(x >= 0.0f) || x >= width - 1 || !(y >= 0.0f) || y >= height - 1) {
+    return 0.0f;
+  }
+  int ix = (int)x;
+  int iy = (int)y;
+  float dx = x - ix;
+  float dy = y - iy;
+  float im00 = img[iy * width + ix];
+  float im01 = img[iy * width + ix + 1];
+  float im10 = img[(iy + 1) * width + ix];
+  float im11 = img[(iy + 1) * width + ix + 1];
+  float im0 = (1 - dx) * im00 + dx * im01;
+  float im1 = (1 - dx) * im10 + dx * im11;
+  return (1 - dy) * im0 + dy * im1;
+}
+
+/* LinearInterpolation<float>  depthmap.cc:14-30 (cleaner) */
+float orc_linear_interp_f32(const float* img, int width, int height, float y,
+                            float x) {
+  if (!(x >= 0.0f) || x >= width - 1 || !(y >= 0.0f) || y >= height - 1) {
+    return 0.0f;
+  }
+  int ix = (int)x;
+  int iy = (int)y;
+  float dx = x - ix;
+  float dy = y - iy;
+  float im00 = img[iy * width + ix];
+  float im01 = img[iy * width + ix + 1];
+  float im10 = img[(iy + 1) * width + ix];
+  float im11 = img[(iy + 1) * width + ix + 1];
+  float im0 = (1 - dx) * im00 + dx * im01;
+  float im1 = (1 - dx) * im10 + dx * im11;
+  return (1 - dy) * im0 + dy * im1;
+}
+
+/* Variance  depthmap.cc:32-44 (two-pass, population) */
+float orc_variance(const float* x, int n) {
+  float sum = 0;
+  for (int i = 0; i < n; ++i) sum += x[i];
+  float mean = sum / n;
+  float sum2 = 0;
+  for (int i = 0; i < n; ++i) sum2 += (x[i] - mean) * (x[i] - mean);
+  return sum2 / n;
+}
+
+/* NCCEstimator  depthmap.cc:46-74 */
+typedef struct {
+  float sumx, sumy, sumxx, sumyy, sumxy, sumw;
+} ncc_t;
+
+static void ncc_push(ncc_t* e, float x, float y, float w) {
+  e->sumx += w * x;
+  e->sumy += w * y;
+  e->sumxx += w * x * x;
+  e->sumyy += w * y * y;
+  e->sumxy += w * x * y;
+  e->sumw += w;
+}
+
+static float ncc_get(const ncc_t* e) {
+  if (e->sumw == 0.0) return -1;
+  float meanx = e->sumx / e->sumw;
+  float meany = e->sumy / e->sumw;
+  float meanxx = e->sumxx / e->sumw;
+  float meanyy = e->sumyy / e->sumw;
+  float meanxy = e->sumxy / e->sumw;
+  float varx = meanxx - meanx * meanx;
+  float vary = meanyy - meany * meany;
+  if (varx < 0.1 || vary < 0.1) return -1;
+  return (meanxy - meanx * meany) / sqrtf(varx * vary);
+}
+
+/* NCC of explicit samples, for tests: x, y, w arrays of length n. */
+float orc_ncc(const float* x, const float* y, const float* w, int n) {
+  ncc_t e = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < n; ++k) ncc_push(&e, x[k], y[k], w[k]);
+  return ncc_get(&e);
+}
+
+/* DepthmapEstimator::BilateralWeight  depthmap.cc:480-487 */
+float orc_bilateral_weight(float dcolor, float dx, float dy) {
+  const float dcolor_sigma = 50.0f;
+  const float dx_sigma = 5.0f;
+  const float dcolor_factor = 1.0f / (2 * dcolor_sigma * dcolor_sigma);
+  const float dx_factor = 1.0f / (2 * dx_sigma * dx_sigma);
+  return expf(-dcolor * dcolor * dcolor_factor -
+              (dx * dx + dy * dy) * dx_factor);
+}
+
+/* PlaneInducedHomographyBaked  depthmap.cc:96-102:
+ * H = K2 * (Q2 + a2 * v^T) * K1inv in FP64, cast to FP32. */
+void orc_homography_baked(const double* K1inv, const double* Q2,
+                          const double* a2, const double* K2, const float* vf,
+                          float* H) {
+  double v[3] = {vf[0], vf[1], vf[2]};
+  double S[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      s += a2[i] * v[j];
+      S[i * 3 + j] = Q2[i * 3 + j] + s;
+    }
+  double T[9];
+  mat3_mul(K2, S, T);
+  mat3_mul(T, K1inv, T);
+  for (int k = 0; k < 9; ++k) H[k] = (float)T[k];
+}
+
+/* PlaneFromDepthAndNormal  depthmap.cc:120-125 */
+void orc_plane_from_depth_and_normal(float x, float y, const double* K,
+                                     float depth, const float* normal,
+                                     float* plane) {
+  double Kinv[9];
+  orc_inv3(K, Kinv);
+  double S[9];
+  for (int k = 0; k < 9; ++k) S[k] = Kinv[k] * depth;
+  double q[3] = {x, y, 1};
+  double pd[3];
+  mat3_vec(S, q, pd);
+  float point[3] = {(float)pd[0], (float)pd[1], (float)pd[2]};
+  float s = 0;
+  for (int k = 0; k < 3; ++k) s += normal[k] * point[k];
+  float denom = -s;
+  float m = (1e-6f < denom) ? denom : 1e-6f; /* std::max(1e-6f, denom) */
+  float inv = 1.f / m;
+  for (int k = 0; k < 3; ++k) plane[k] = normal[k] * inv;
+}
+
+/* DepthOfPlaneBackprojection  depthmap.cc:114-118 */
+float orc_depth_of_plane_backprojection(double x, double y, const double* K,
+                                        const float* planef) {
+  double Kinv[9];
+  orc_inv3(K, Kinv);
+  double p[3] = {planef[0], planef[1], planef[2]};
+  double r[3];
+  for (int j = 0; j < 3; ++j) {
+    double s = 0;
+    for (int k = 0; k < 3; ++k) s += p[k] * Kinv[k * 3 + j];
+    r[j] = s;
+  }
+  double q[3] = {x, y, 1};
+  double s = 0;
+  for (int k = 0; k < 3; ++k) s += r[k] * q[k];
+  float denom = (float)(-s);
+  float m = (1e-6f < denom) ? denom : 1e-6f;
+  return 1.0f / m;
+}
+
+/* 5x5 median with replicate border (cv::medianBlur(src, dst, 5), CV_32F) */
+static int cmp_float(const void* a, const void* b) {
+  float x = *(const float*)a, y = *(const float*)b;
+  return (x > y) - (x < y);
+}
+
+void orc_median5(const float* src, float* dst, int width, int height) {
+  float win[25];
+  for (int i = 0; i < height; ++i) {
+    for (int j = 0; j < width; ++j) {
+      int c = 0;
+      for (int u = -2; u <= 2; ++u) {
+        int ii = i + u;
+        ii = ii < 0 ? 0 : (ii >= height ? height - 1 : ii);
+        for (int v = -2; v <= 2; ++v) {
+          int jj = j + v;
+          jj = jj < 0 ? 0 : (jj >= width ? width - 1 : jj);
+          win[c++] = src[ii * width + jj];
+        }
+      }
+      qsort(win, 25, sizeof(float), cmp_float);
+      dst[i * width + j] = win[12];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Seeded random source (replaces rand() / std::mt19937)               */
+/* ------------------------------------------------------------------ */
+
+static uint64_t splitmix64(uint64_t* s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+/* UniformRand  depthmap.cc:127-132, with a 31-bit integer like glibc rand() */
+static float uniform_rand(uint64_t* s, float a, float b) {
+  int r = (int)(splitmix64(s) >> 33);
+  return a + (b - a) * (float)r / (float)2147483647;
+}
+
+/* std::normal_distribution<float>(0, 1): Box-Muller on two uniforms. */
+static float unit_normal(uint64_t* s) {
+  double u1 = ((splitmix64(s) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+  double u2 = (splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);
+  return (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+/* std::uniform_int_distribution<int>(a, b) */
+static int uniform_int(uint64_t* s, int a, int b) {
+  uint64_t span = (uint64_t)(b - a) + 1;
+  return a + (int)(splitmix64(s) % span);
+}
+
+/* ------------------------------------------------------------------ */
+/* DepthmapEstimator  depthmap.h:57-111, depthmap.cc:134-502            */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int n, cap;
+  double *K, *R, *t, *Kinv, *Q, *a; /* 9, 9, 3, 9, 9, 3 per view */
+  uint8_t** img;
+  uint8_t** mask;
+  int *w, *h;
+  int patch_size;
+  double min_depth, max_depth;
+  int num_depth_planes;
+  int iterations;
+  float min_patch_variance;
+  uint64_t rng;
+  int64_t scorings; /* ComputePlaneImageScore calls of the last compute */
+  /* result maps of the running compute */
+  float *depth, *plane, *score;
+  int32_t* nghbr;
+} orc_est;
+
+void* orc_create(void) {
+  orc_est* e = (orc_est*)calloc(1, sizeof(orc_est));
+  e->patch_size = 7;
+  e->num_depth_planes = 50;
+  e->iterations = 3;
+  e->min_patch_variance = 5 * 5;
+  e->rng = 0x5EEDull;
+  return e;
+}
+
+void orc_destroy(void* p) {
+  orc_est* e = (orc_est*)p;
+  if (!e) return;
+  for (int v = 0; v < e->n; ++v) {
+    free(e->img[v]);
+    free(e->mask[v]);
+  }
+  free(e->K); free(e->R); free(e->t); free(e->Kinv); free(e->Q); free(e->a);
+  free(e->img); free(e->mask); free(e->w); free(e->h);
+  free(e);
+}
+
+/* DepthmapEstimator::AddView  depthmap.cc:146-162 */
+int orc_add_view(void* p, const double* K, const double* R, const double* t,
+                 const uint8_t* image, const uint8_t* mask, int width,
+                 int height) {
+  orc_est* e = (orc_est*)p;
+  if (e->n == e->cap) {
+    int c = e->cap ? 2 * e->cap : 8;
+    e->K = realloc(e->K, sizeof(double) * 9 * c);
+    e->R = realloc(e->R, sizeof(double) * 9 * c);
+    e->t = realloc(e->t, sizeof(double) * 3 * c);
+    e->Kinv = realloc(e->Kinv, sizeof(double) * 9 * c);
+    e->Q = realloc(e->Q, sizeof(double) * 9 * c);
+    e->a = realloc(e->a, sizeof(double) * 3 * c);
+    e->img = realloc(e->img, sizeof(uint8_t*) * c);
+    e->mask = realloc(e->mask, sizeof(uint8_t*) * c);
+    e->w = realloc(e->w, sizeof(int) * c);
+    e->h = realloc(e->h, sizeof(int) * c);
+    e->cap = c;
+  }
+  int v = e->n;
+  memcpy(e->K + 9 * v, K, 9 * sizeof(double));
+  memcpy(e->R + 9 * v, R, 9 * sizeof(double));
+  memcpy(e->t + 3 * v, t, 3 * sizeof(double));
+  orc_inv3(e->K + 9 * v, e->Kinv + 9 * v);
+  double R0t[9];
+  mat3_transpose(e->R, R0t); /* view 0 is the reference */
+  mat3_mul(e->R + 9 * v, R0t, e->Q + 9 * v);
+  double qt[3];
+  mat3_vec(e->Q + 9 * v, e->t, qt);
+  for (int k = 0; k < 3; ++k) e->a[3 * v + k] = qt[k] - e->t[3 * v + k];
+  size_t np = (size_t)width * height;
+  e->img[v] = (uint8_t*)malloc(np);
+  e->mask[v] = (uint8_t*)malloc(np);
+  memcpy(e->img[v], image, np);
+  memcpy(e->mask[v], mask, np);
+  e->w[v] = width;
+  e->h[v] = height;
+  e->n++;
+  return v;
+}
+
+void orc_set_depth_range(void* p, double mn, double mx, int planes) {
+  orc_est* e = (orc_est*)p;
+  e->min_depth = mn;
+  e->max_depth = mx;
+  e->num_depth_planes = planes;
+}
+void orc_set_patchmatch_iterations(void* p, int n) { ((orc_est*)p)->iterations = n; }
+void orc_set_patch_size(void* p, int s) { ((orc_est*)p)->patch_size = s; }
+void orc_set_min_patch_sd(void* p, float sd) { ((orc_est*)p)->min_patch_variance = sd * sd; }
+void orc_set_seed(void* p, uint64_t seed) { ((orc_est*)p)->rng = seed; }
+int64_t orc_last_scorings(void* p) { return ((orc_est*)p)->scorings; }
+
+/* ComputePlaneImageScore  depthmap.cc:441-478 (local-affine warp) */
+static float plane_image_score(orc_est* e, int i, int j, const float* plane,
+                               int other) {
+  e->scorings++;
+  float H[9];
+  orc_homography_baked(e->Kinv, e->Q + 9 * other, e->a + 3 * other,
+                       e->K + 9 * other, plane, H);
+  int hpz = (e->patch_size - 1) / 2;
+  float fj = (float)j, fi = (float)i;
+  float u = H[0] * fj + H[1] * fi + H[2];
+  float v = H[3] * fj + H[4] * fi + H[5];
+  float w = H[6] * fj + H[7] * fi + H[8];
+  if (w == 0.0) return -1.0f;
+
+  float dfdx_x = (H[0] * w - H[6] * u) / (w * w);
+  float dfdx_y = (H[3] * w - H[6] * v) / (w * w);
+  float dfdy_x = (H[1] * w - H[7] * u) / (w * w);
+  float dfdy_y = (H[4] * w - H[7] * v) / (w * w);
+
+  float Hx0 = u / w;
+  float Hy0 = v / w;
+
+  const uint8_t* im0 = e->img[0];
+  int W0 = e->w[0];
+  float im1_center = im0[i * W0 + j];
+  ncc_t ncc = {0, 0, 0, 0, 0, 0};
+  for (int dy = -hpz; dy <= hpz; ++dy) {
+    for (int dx = -hpz; dx <= hpz; ++dx) {
+      float im1 = im0[(i + dy) * W0 + j + dx];
+      float x2 = Hx0 + dfdx_x * dx + dfdy_x * dy;
+      float y2 = Hy0 + dfdx_y * dx + dfdy_y * dy;
+      float im2 = orc_linear_interp_u8(e->img[other], e->w[other],
+                                       e->h[other], y2, x2);
+      float weight = orc_bilateral_weight(im1 - im1_center, dx, dy);
+      ncc_push(&ncc, im1, im2, weight);
+    }
+  }
+  return ncc_get(&ncc);
+}
+
+float orc_score(void* p, int i, int j, const float* plane, int other) {
+  return plane_image_score((orc_est*)p, i, j, plane, other);
+}
+
+/* FP64 twin of ComputePlaneImageScore: the same local-affine algorithm with
+ * every quantity (homography, Jacobian, bilinear sample, bilateral weight,
+ * moments) in double.  Used to state the error of the FP32/FP16/BF16 paths. */
+double orc_score_f64(void* p, int i, int j, const double* v, int other) {
+  orc_est* e = (orc_est*)p;
+  const double* K1inv = e->Kinv;
+  const double* Q2 = e->Q + 9 * other;
+  const double* a2 = e->a + 3 * other;
+  const double* K2 = e->K + 9 * other;
+  double S[9], T[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) S[r * 3 + c] = Q2[r * 3 + c] + a2[r] * v[c];
+  mat3_mul(K2, S, T);
+  mat3_mul(T, K1inv, T);
+  const double* H = T;
+  int hpz = (e->patch_size - 1) / 2;
+  double u = H[0] * j + H[1] * i + H[2];
+  double vv = H[3] * j + H[4] * i + H[5];
+  double w = H[6] * j + H[7] * i + H[8];
+  if (w == 0.0) return -1.0;
+  double dfdx_x = (H[0] * w - H[6] * u) / (w * w);
+  double dfdx_y = (H[3] * w - H[6] * vv) / (w * w);
+  double dfdy_x = (H[1] * w - H[7] * u) / (w * w);
+  double dfdy_y = (H[4] * w - H[7] * vv) / (w * w);
+  double Hx0 = u / w, Hy0 = vv / w;
+  const uint8_t* im0 = e->img[0];
+  const uint8_t* im2p = e->img[other];
+  int W0 = e->w[0], W2 = e->w[other], H2 = e->h[other];
+  double c = im0[i * W0 + j];
+  double sx = 0, sy = 0, sxx = 0, syy = 0, sxy = 0, sw = 0;
+  for (int dy = -hpz; dy <= hpz; ++dy) {
+    for (int dx = -hpz; dx <= hpz; ++dx) {
+      double x1 = im0[(i + dy) * W0 + j + dx];
+      double x2 = Hx0 + dfdx_x * dx + dfdy_x * dy;
+      double y2 = Hy0 + dfdx_y * dx + dfdy_y * dy;
+      double y = 0;
+      if (x2 >= 0 && x2 < W2 - 1 && y2 >= 0 && y2 < H2 - 1) {
+        int ix = (int)x2, iy = (int)y2;
+        double fx = x2 - ix, fy = y2 - iy;
+        double a00 = im2p[iy * W2 + ix], a01 = im2p[iy * W2 + ix + 1];
+        double a10 = im2p[(iy + 1) * W2 + ix], a11 = im2p[(iy + 1) * W2 + ix + 1];
+        y = (1 - fy) * ((1 - fx) * a00 + fx * a01) + fy * ((1 - fx) * a10 + fx * a11);
+      }
+      double dc = x1 - c;
+      double wgt = exp(-dc * dc / 5000.0 - (double)(dx * dx + dy * dy) / 50.0);
+      sx += wgt * x1; sy += wgt * y; sxx += wgt * x1 * x1; syy += wgt * y * y;
+      sxy += wgt * x1 * y; sw += wgt;
+    }
+  }
+  if (sw == 0.0) return -1.0;
+  double mx = sx / sw, my = sy / sw;
+  double vx = sxx / sw - mx * mx, vy = syy / sw - my * my;
+  if (vx < 0.1 || vy < 0.1) return -1.0;
+  return (sxy / sw - mx * my) / sqrt(vx * vy);
+}
+
+/* ComputePlaneScore  depthmap.cc:408-419 */
+static void plane_score(orc_est* e, int i, int j, const float* plane,
+                        float* score, int* nghbr) {
+  *score = -1.0f;
+  *nghbr = 0;
+  for (int other = 1; other < e->n; ++other) {
+    float s = plane_image_score(e, i, j, plane, other);
+    if (s > *score) {
+      *score = s;
+      *nghbr = other;
+    }
+  }
+}
+
+/* AssignPixel  depthmap.cc:398-406 */
+static void assign_pixel(orc_est* e, int i, int j, float depth,
+                         const float* plane, float score, int nghbr) {
+  int W = e->w[0];
+  size_t k = (size_t)i * W + j;
+  e->depth[k] = depth;
+  e->plane[3 * k + 0] = plane[0];
+  e->plane[3 * k + 1] = plane[1];
+  e->plane[3 * k + 2] = plane[2];
+  e->score[k] = score;
+  e->nghbr[k] = nghbr;
+}
+
+/* CheckPlaneCandidate  depthmap.cc:376-386 */
+static void check_plane_candidate(orc_est* e, int i, int j,
+                                  const float* plane) {
+  float score;
+  int nghbr;
+  plane_score(e, i, j, plane, &score, &nghbr);
+  if (score > e->score[(size_t)i * e->w[0] + j]) {
+    float depth = orc_depth_of_plane_backprojection(j, i, e->K, plane);
+    assign_pixel(e, i, j, depth, plane, score, nghbr);
+  }
+}
+
+/* CheckPlaneImageCandidate  depthmap.cc:388-396 */
+static void check_plane_image_candidate(orc_est* e, int i, int j,
+                                        const float* plane, int nghbr) {
+  float score = plane_image_score(e, i, j, plane, nghbr);
+  if (score > e->score[(size_t)i * e->w[0] + j]) {
+    float depth = orc_depth_of_plane_backprojection(j, i, e->K, plane);
+    assign_pixel(e, i, j, depth, plane, score, nghbr);
+  }
+}
+
+/* ComputeBruteForce  depthmap.cc:184-200 */
+static void brute_force(orc_est* e) {
+  int hpz = (e->patch_size - 1) / 2;
+  int H = e->h[0], W = e->w[0];
+  for (int i = hpz; i < H - hpz; ++i) {
+    for (int j = hpz; j < W - hpz; ++j) {
+      for (int d = 0; d < e->num_depth_planes; ++d) {
+        float depth = (float)(1 / (1 / e->min_depth +
+                                   d * (1 / e->max_depth - 1 / e->min_depth) /
+                                       (e->num_depth_planes - 1)));
+        float normal[3] = {0, 0, -1};
+        float plane[3];
+        orc_plane_from_depth_and_normal((float)j, (float)i, e->K, depth,
+                                        normal, plane);
+        check_plane_candidate(e, i, j, plane);
+      }
+    }
+  }
+}
+
+/* RandomInitialization  depthmap.cc:237-256 */
+static void random_initialization(orc_est* e, int sample) {
+  int hpz = (e->patch_size - 1) / 2;
+  int H = e->h[0], W = e->w[0];
+  float lmin = (float)log(e->min_depth), lmax = (float)log(e->max_depth);
+  for (int i = hpz; i < H - hpz; ++i) {
+    for (int j = hpz; j < W - hpz; ++j) {
+      float depth = expf(uniform_rand(&e->rng, lmin, lmax));
+      float n0 = uniform_rand(&e->rng, -1, 1);
+      float n1 = uniform_rand(&e->rng, -1, 1);
+      float normal[3] = {n0, n1, -1};
+      float plane[3];
+      orc_plane_from_depth_and_normal((float)j, (float)i, e->K, depth, normal,
+                                      plane);
+      int nghbr;
+      float score;
+      if (sample) {
+        nghbr = uniform_int(&e->rng, e->n > 1 ? 1 : 0, e->n > 1 ? e->n - 1 : 0);
+        score = plane_image_score(e, i, j, plane, nghbr);
+      } else {
+        plane_score(e, i, j, plane, &score, &nghbr);
+      }
+      assign_pixel(e, i, j, depth, plane, score, nghbr);
+    }
+  }
+}
+
+/* PatchVariance  depthmap.cc:271-281 */
+static float patch_variance(orc_est* e, int i, int j) {
+  float patch[ORC_MAX_PATCH * ORC_MAX_PATCH];
+  int hpz = (e->patch_size - 1) / 2;
+  int c = 0;
+  int W = e->w[0];
+  for (int u = -hpz; u <= hpz; ++u)
+    for (int v = -hpz; v <= hpz; ++v) patch[c++] = e->img[0][(i + u) * W + j + v];
+  return orc_variance(patch, e->patch_size * e->patch_size);
+}
+
+/* ComputeIgnoreMask  depthmap.cc:258-269 */
+static void compute_ignore_mask(orc_est* e) {
+  int hpz = (e->patch_size - 1) / 2;
+  int H = e->h[0], W = e->w[0];
+  const float zero[3] = {0, 0, 0};
+  for (int i = hpz; i < H - hpz; ++i) {
+    for (int j = hpz; j < W - hpz; ++j) {
+      int masked = e->mask[0][i * W + j] == 0;
+      int low_variance = patch_variance(e, i, j) < e->min_patch_variance;
+      if (masked || low_variance) assign_pixel(e, i, j, 0.0f, zero, 0.0f, 0);
+    }
+  }
+}
+
+/* PatchMatchUpdatePixel  depthmap.cc:305-374 */
+static void update_pixel(orc_est* e, int i, int j, const int adj[2][2],
+                         int sample) {
+  int W = e->w[0];
+  size_t k0 = (size_t)i * W + j;
+  if (e->depth[k0] == 0.0f) return;
+
+  for (int k = 0; k < 2; ++k) {
+    int ia = i + adj[k][0], ja = j + adj[k][1];
+    size_t ka = (size_t)ia * W + ja;
+    if (e->depth[ka] == 0.0f) continue;
+    float plane[3] = {e->plane[3 * ka], e->plane[3 * ka + 1], e->plane[3 * ka + 2]};
+    if (sample) {
+      check_plane_image_candidate(e, i, j, plane, e->nghbr[ka]);
+    } else {
+      check_plane_candidate(e, i, j, plane);
+    }
+  }
+
+  float depth_range = 0.02f;
+  float normal_range = 0.5f;
+  int current_nghbr = e->nghbr[k0];
+  for (int k = 0; k < 6; ++k) {
+    float current_depth = e->depth[k0];
+    float depth = current_depth * expf(depth_range * unit_normal(&e->rng));
+    float cp[3] = {e->plane[3 * k0], e->plane[3 * k0 + 1], e->plane[3 * k0 + 2]};
+    if (cp[2] == 0.0) continue;
+    float g0 = unit_normal(&e->rng);
+    float g1 = unit_normal(&e->rng);
+    float normal[3] = {-cp[0] / cp[2] + normal_range * g0,
+                       -cp[1] / cp[2] + normal_range * g1, -1.0f};
+    float plane[3];
+    orc_plane_from_depth_and_normal((float)j, (float)i, e->K, depth, normal,
+                                    plane);
+    if (sample) {
+      check_plane_image_candidate(e, i, j, plane, current_nghbr);
+    } else {
+      check_plane_candidate(e, i, j, plane);
+    }
+    depth_range *= 0.3f;
+    normal_range *= 0.8f;
+  }
+
+  if (!sample || e->n <= 2) return;
+
+  int other = uniform_int(&e->rng, 1, e->n - 1);
+  while (other == current_nghbr) other = uniform_int(&e->rng, 1, e->n - 1);
+  float plane[3] = {e->plane[3 * k0], e->plane[3 * k0 + 1], e->plane[3 * k0 + 2]};
+  check_plane_image_candidate(e, i, j, plane, other);
+}
+
+/* PatchMatchForwardPass / PatchMatchBackwardPass  depthmap.cc:283-303 */
+static void forward_pass(orc_est* e, int sample) {
+  static const int adj[2][2] = {{-1, 0}, {0, -1}};
+  int hpz = (e->patch_size - 1) / 2;
+  for (int i = hpz; i < e->h[0] - hpz; ++i)
+    for (int j = hpz; j < e->w[0] - hpz; ++j) update_pixel(e, i, j, adj, sample);
+}
+
+static void backward_pass(orc_est* e, int sample) {
+  static const int adj[2][2] = {{0, 1}, {1, 0}};
+  int hpz = (e->patch_size - 1) / 2;
+  for (int i = e->h[0] - hpz - 1; i >= hpz; --i)
+    for (int j = e->w[0] - hpz - 1; j >= hpz; --j) update_pixel(e, i, j, adj, sample);
+}
+
+/* PostProcess  depthmap.cc:489-502 */
+static void post_process(orc_est* e) {
+  int H = e->h[0], W = e->w[0];
+  float* filtered = (float*)malloc(sizeof(float) * (size_t)H * W);
+  orc_median5(e->depth, filtered, W, H);
+  for (size_t k = 0; k < (size_t)H * W; ++k) {
+    float d = e->depth[k];
+    float m = filtered[k];
+    if (d == 0.0 || fabsf(d - m) / d > 0.05) e->depth[k] = 0;
+  }
+  free(filtered);
+}
+
+/* mode 0: ComputeBruteForce (depthmap.cc:184-200)
+ * mode 1: ComputePatchMatch (depthmap.cc:202-213)
+ * mode 2: ComputePatchMatchSample (depthmap.cc:215-227)
+ * Output buffers (AssignMatrices, depthmap.cc:229-235): depth H*W, plane
+ * H*W*3, score H*W, nghbr H*W, all zero-initialised here.
+ * stop_after_iterations < 0 runs everything; otherwise PatchMatch stops
+ * after that many iterations and skips PostProcess (used by tests that
+ * check the per-iteration score invariant). */
+int orc_compute(void* p, int mode, float* depth, float* plane, float* score,
+                int32_t* nghbr, int stop_after_iterations) {
+  orc_est* e = (orc_est*)p;
+  if (e->n < 1) return -1;
+  size_t np = (size_t)e->w[0] * e->h[0];
+  memset(depth, 0, np * sizeof(float));
+  memset(plane, 0, 3 * np * sizeof(float));
+  memset(score, 0, np * sizeof(float));
+  memset(nghbr, 0, np * sizeof(int32_t));
+  e->depth = depth;
+  e->plane = plane;
+  e->score = score;
+  e->nghbr = nghbr;
+  e->scorings = 0;
+  if (mode == 0) {
+    brute_force(e);
+  } else {
+    int sample = mode == 2;
+    random_initialization(e, sample);
+    compute_ignore_mask(e);
+    int iters = e->iterations;
+    if (stop_after_iterations >= 0 && stop_after_iterations < iters)
+      iters = stop_after_iterations;
+    for (int it = 0; it < iters; ++it) {
+      forward_pass(e, sample);
+      backward_pass(e, sample);
+    }
+    if (stop_after_iterations < 0) post_process(e);
+  }
+  e->depth = e->plane = e->score = NULL;
+  e->nghbr = NULL;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* DepthmapCleaner  depthmap.h:113-129, depthmap.cc:504-555              */
+/* ------------------------------------------------------------------ */
+
+/* Views are given as arrays: Ks (n*9), Rs (n*9), ts (n*3), depths[n],
+ * widths[n], heights[n].  View 0 is the one being cleaned. */
+int orc_clean(int n, const double* Ks, const double* Rs, const double* ts,
+              const float* const* depths, const int* widths,
+              const int* heights, float same_depth_threshold,
+              int min_consistent_views, float* clean_depth) {
+  if (n < 1) return -1;
+  int W = widths[0], H = heights[0];
+  double Kinv0[9], R0t[9];
+  orc_inv3(Ks, Kinv0);
+  mat3_transpose(Rs, R0t);
+  for (int i = 0; i < H; ++i) {
+    for (int j = 0; j < W; ++j) {
+      float depth = depths[0][(size_t)i * W + j];
+      /* Backproject  depthmap.cc:109-112: R^T (depth K^-1 q - t) */
+      double S[9];
+      for (int k = 0; k < 9; ++k) S[k] = Kinv0[k] * (double)depth;
+      double q[3] = {(double)j, (double)i, 1};
+      double X[3];
+      mat3_vec(S, q, X);
+      for (int k = 0; k < 3; ++k) X[k] = X[k] - ts[k];
+      mat3_vec(R0t, X, X);
+      float point[3] = {(float)X[0], (float)X[1], (float)X[2]};
+      int consistent_views = 1;
+      for (int other = 1; other < n; ++other) {
+        /* Project  depthmap.cc:104-107: K (R x + t) */
+        double pd[3] = {point[0], point[1], point[2]};
+        double y[3];
+        mat3_vec(Rs + 9 * other, pd, y);
+        for (int k = 0; k < 3; ++k) y[k] = y[k] + ts[3 * other + k];
+        mat3_vec(Ks + 9 * other, y, y);
+        float rp[3] = {(float)y[0], (float)y[1], (float)y[2]};
+        if (rp[2] < 1e-8 || isnan(rp[2])) continue;
+        float u = rp[0] / rp[2];
+        float v = rp[1] / rp[2];
+        float depth_of_point = rp[2];
+        float depth_at_reprojection = orc_linear_interp_f32(
+            depths[other], widths[other], heights[other], v, u);
+        if (fabsf(depth_at_reprojection - depth_of_point) <
+            depth_of_point * same_depth_threshold) {
+          consistent_views++;
+        }
+      }
+      clean_depth[(size_t)i * W + j] =
+          consistent_views >= min_consistent_views ? depths[0][(size_t)i * W + j]
+                                                   : 0;
+    }
+  }
+  return 0;
+}

@@ -1,0 +1,167 @@
+"""GPU parity tests: the HIP engine against the CPU oracle (oracle/).
+
+Tolerances:
+  * plane sweep (ComputeBruteForce) and fixed-hypothesis NCC scores: within
+    1e-4 absolute of the oracle's FP32 restatement (the engine follows the
+    reference's FP32 operation order, so in practice the maps are equal);
+  * cleaner and the 5x5 median post-process: bit-exact;
+  * PatchMatch (stochastic): mean score and fraction of pixels within 1 % of
+    the ground-truth depth at least the oracle's, per seed-averaged run.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from opensfm_amd import DepthmapCleaner, DepthmapEstimator, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(est, scene, planes=16, patch=7, sd=1.0, iters=3):
+    for v in scene.views:
+        est.add_view(v.K, v.R, v.t, v.image, v.mask)
+    est.set_depth_range(scene.min_depth, scene.max_depth, planes)
+    est.set_patch_size(patch)
+    est.set_min_patch_sd(sd)
+    est.set_patchmatch_iterations(iters)
+    return est
+
+
+@pytest.fixture(scope="module")
+def small_scene():
+    return scenes.make_scene(128, 96, 3, 4, seed=11, min_depth=2.0, max_depth=20.0)
+
+
+@pytest.mark.parametrize("patch", [3, 5, 7])
+def test_plane_sweep_matches_oracle(small_scene, patch):
+    print(f"plane sweep 128x96 patch {patch}", flush=True)
+    g = _setup(DepthmapEstimator(), small_scene, planes=24, patch=patch).compute_plane_sweep()
+    o = _setup(oracle.OracleEstimator(), small_scene, planes=24, patch=patch).compute_brute_force()
+    assert np.max(np.abs(g[2] - o[2])) <= 1e-4
+    assert np.array_equal(g[3], o[3])
+    assert np.array_equal(g[0], o[0])
+    assert np.array_equal(g[1], o[1])
+    assert (o[0] > 0).mean() > 0.5
+
+
+def test_plane_sweep_config1_64_planes(cfg1_scene):
+    print("plane sweep config 1, 64 planes", flush=True)
+    g = _setup(DepthmapEstimator(), cfg1_scene, planes=64).compute_plane_sweep()
+    o = _setup(oracle.OracleEstimator(), cfg1_scene, planes=64).compute_brute_force()
+    err = np.abs(g[2] - o[2])
+    assert err.max() <= 1e-4, err.max()
+    assert (g[3] == o[3]).mean() > 0.9999
+    assert (g[0] == o[0]).mean() > 0.9999
+
+
+def test_fixed_hypothesis_scores(cfg1_scene):
+    print("fixed hypotheses config 1", flush=True)
+    rng = np.random.default_rng(5)
+    n = 20000
+    hpz = 3
+    v0 = cfg1_scene.ref
+    H, W = v0.image.shape
+    ii = rng.integers(hpz, H - hpz, n)
+    jj = rng.integers(hpz, W - hpz, n)
+    depth = np.exp(rng.uniform(np.log(2), np.log(20), n))
+    normals = np.stack([rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), -np.ones(n)], 1)
+    planes = np.stack([oracle.plane_from_depth_and_normal(float(jj[k]), float(ii[k]), v0.K,
+                                                          float(depth[k]), normals[k])
+                       for k in range(n)]).astype(np.float32)
+    views = rng.integers(1, len(cfg1_scene.views), n).astype(np.int32)
+    g = _setup(DepthmapEstimator(), cfg1_scene).score_hypotheses(np.stack([ii, jj], 1), planes, views)
+    ref = _setup(oracle.OracleEstimator(), cfg1_scene)
+    o = np.array([ref.score(int(ii[k]), int(jj[k]), planes[k], int(views[k])) for k in range(n)],
+                 np.float32)
+    assert np.max(np.abs(g - o)) <= 1e-4
+    assert np.mean(g == o) > 0.999
+    assert np.mean(o > -1) > 0.3  # the check is not vacuous
+
+
+def test_cleaner_bit_exact(cfg1_scene):
+    print("cleaner config 1", flush=True)
+    rng = np.random.default_rng(3)
+    views = []
+    for k, v in enumerate(cfg1_scene.views):
+        d = v.depth.copy()
+        if k == 2:  # one view disagrees in a band
+            d[60:120] *= 1.05
+        d[rng.random(d.shape) < 0.05] = 0
+        views.append((v.K, v.R, v.t, d))
+    for thr, mv in [(0.01, 2), (0.02, 3)]:
+        c = DepthmapCleaner()
+        c.set_same_depth_threshold(thr)
+        c.set_min_consistent_views(mv)
+        for K, R, t, d in views:
+            c.add_view(K, R, t, d)
+        g = c.clean()
+        o = oracle.clean(views, thr, mv)
+        assert np.array_equal(g, o)
+        assert 0.2 < (g > 0).mean() < (views[0][3] > 0).mean()
+
+
+def test_median_post_process_bit_exact(cfg1_scene):
+    print("median post-process config 1", flush=True)
+    est = _setup(DepthmapEstimator(), cfg1_scene, iters=1)
+    est.set_seed(99)
+    est.set_post_process(False)
+    raw = est.compute_patch_match()[0]
+    est.set_post_process(True)
+    post = est.compute_patch_match()[0]
+    m = oracle.median5(raw)
+    expect = raw.copy()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        # FP32 quotient compared in FP64, as `fabs(d - m) / d > 0.05` does
+        reject = (raw == 0) | ((np.abs(raw - m) / raw).astype(np.float64) > 0.05)
+    expect[reject] = 0
+    assert np.array_equal(post, expect)
+
+
+def test_patchmatch_deterministic(cfg1_scene):
+    print("PatchMatch determinism config 1", flush=True)
+    a = _setup(DepthmapEstimator(), cfg1_scene)
+    a.set_seed(1234)
+    r1 = a.compute_patch_match()
+    r2 = a.compute_patch_match()
+    for x, y in zip(r1, r2):
+        assert np.array_equal(x, y)
+    a.set_seed(1235)
+    r3 = a.compute_patch_match()
+    assert not np.array_equal(r1[0], r3[0])
+
+
+def test_patchmatch_invariants(cfg1_scene):
+    print("PatchMatch invariants config 1", flush=True)
+    est = _setup(DepthmapEstimator(), cfg1_scene)
+    depth, plane, score, nghbr = est.compute_patch_match()
+    H, W = depth.shape
+    hpz = 3
+    border = np.ones_like(depth, bool)
+    border[hpz:H - hpz, hpz:W - hpz] = False
+    assert np.all(depth[border] == 0) and np.all(score[border] == 0)
+    assert np.all(np.isfinite(depth)) and np.all(depth >= 0)
+    assert np.all(score <= 1.0 + 1e-5) and np.all(score >= -1.0)
+    live = (~border) & (plane[..., 2] != 0)
+    assert np.all((nghbr[live] >= 1) & (nghbr[live] < len(cfg1_scene.views)))
+
+
+def test_patchmatch_accuracy_vs_oracle(cfg1_scene):
+    """Config 1: 320x240, 4 neighbours, 3 iterations, patch 7, min SD 1."""
+    print("PatchMatch accuracy vs oracle config 1", flush=True)
+    gt = cfg1_scene.ref.depth
+    o_acc, o_score, g_acc, g_score = [], [], [], []
+    for seed in (1, 2):
+        o = _setup(oracle.OracleEstimator(), cfg1_scene)
+        o.set_seed(seed)
+        od, _, osc, _ = o.compute_patch_match()
+        o_acc.append(scenes.depth_accuracy(od, gt, 2.0, 20.0))
+        o_score.append(float(osc[od > 0].mean()))
+        g = _setup(DepthmapEstimator(), cfg1_scene)
+        g.set_seed(seed)
+        gd, _, gsc, _ = g.compute_patch_match()
+        g_acc.append(scenes.depth_accuracy(gd, gt, 2.0, 20.0))
+        g_score.append(float(gsc[gd > 0].mean()))
+    print(f"oracle acc {np.mean(o_acc):.4f} score {np.mean(o_score):.4f} | "
+          f"gpu acc {np.mean(g_acc):.4f} score {np.mean(g_score):.4f}", flush=True)
+    assert np.mean(g_acc) >= np.mean(o_acc) - 0.005
+    assert np.mean(g_score) >= np.mean(o_score) - 0.005
