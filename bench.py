@@ -121,6 +121,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (torch) so that "
+                         "profiler byte counters can be scaled against a known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -141,6 +144,16 @@ def main():
 
     from opensfm_amd import DepthmapEstimator, scenes
     from opensfm_amd._capi import PM_MODE_PATCHMATCH
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration} MiB read + written")
+        del src, dst
 
     est = DepthmapEstimator()
     for v in scene.views:

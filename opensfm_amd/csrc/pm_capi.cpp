@@ -659,30 +659,32 @@ int pm_estimator_score_hypotheses(pm_estimator* e, int n, const int32_t* ij,
     std::vector<int> fill(start.begin(), start.end() - 1);
     for (int k = 0; k < n; ++k) order[fill[views[k]]++] = k;
     std::vector<int32_t> s_ij(2 * (size_t)n);
-    std::vector<float> s_pl(3 * (size_t)n), s_out(n);
+    // FP64 composition on the host, FP32 homography per hypothesis
+    // (PlaneInducedHomographyBaked, depthmap.cc:96-102)
+    std::vector<float> s_H(9 * (size_t)n), s_out(n);
     for (int m = 0; m < n; ++m) {
       const int k = order[m];
       s_ij[2 * m] = ij[2 * k];
       s_ij[2 * m + 1] = ij[2 * k + 1];
-      for (int c = 0; c < 3; ++c) s_pl[3 * m + c] = planes[3 * k + c];
+      const HostView& V = *e->views[views[k]];
+      homography_baked(r.Kinv, V.Q, V.a, V.K, planes + 3 * (size_t)k, &s_H[9 * (size_t)m]);
     }
-    DevBuf d_ij, d_pl, d_out;
+    DevBuf d_ij, d_H, d_out;
     d_ij.ensure(sizeof(int32_t) * 2 * n);
-    d_pl.ensure(sizeof(float) * 3 * n);
+    d_H.ensure(sizeof(float) * 9 * n);
     d_out.ensure(sizeof(float) * n);
     HIP_CHECK(hipMemcpyAsync(d_ij.p, s_ij.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, e->stream));
-    HIP_CHECK(hipMemcpyAsync(d_pl.p, s_pl.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(hipMemcpyAsync(d_H.p, s_H.data(), sizeof(float) * 9 * n, hipMemcpyHostToDevice, e->stream));
     for (int v = 1; v < nv; ++v) {
       const int cnt = start[v + 1] - start[v];
       if (cnt == 0) continue;
       HIP_CHECK(pm::launch_score_hyp(a, v, cnt, d_ij.as<int>() + 2 * start[v],
-                                     d_pl.as<float>() + 3 * start[v],
+                                     d_H.as<float>() + 9 * (size_t)start[v],
                                      d_out.as<float>() + start[v], e->stream));
     }
     HIP_CHECK(hipMemcpyAsync(s_out.data(), d_out.p, sizeof(float) * n, hipMemcpyDeviceToHost, e->stream));
     HIP_CHECK(hipStreamSynchronize(e->stream));
     for (int m = 0; m < n; ++m) out[order[m]] = s_out[m];
-    HIP_CHECK(hipStreamSynchronize(e->stream));
     return PM_OK;
   });
 }

@@ -68,46 +68,6 @@ __device__ __forceinline__ float ub2(uint32_t v) { return (float)((v >> 16) & 0x
 __device__ __forceinline__ float ub3(uint32_t v) { return (float)(v >> 24); }
 
 // ---------------------------------------------------------------------------
-// Homography in FP64 exactly as PlaneInducedHomographyBaked
-// (depthmap.cc:96-102): H = K2 * (Q2 + a2 * v^T) * K1inv, products
-// accumulated from zero in k order (cv::Matx), then cast to FP32.
-
-__device__ __forceinline__ void homography_baked_f64(const double* K1inv,
-                                                     const double* Q2,
-                                                     const double* a2,
-                                                     const double* K2, float v0,
-                                                     float v1, float v2,
-                                                     float* H) {
-  const double v[3] = {(double)v0, (double)v1, (double)v2};
-  double S[9], T[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      double s = 0;
-      s += a2[i] * v[j];
-      S[i * 3 + j] = Q2[i * 3 + j] + s;
-    }
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      double s = 0;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) s += K2[i * 3 + k] * S[k * 3 + j];
-      T[i * 3 + j] = s;
-    }
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      double s = 0;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) s += T[i * 3 + k] * K1inv[k * 3 + j];
-      H[i * 3 + j] = (float)s;
-    }
-}
-
 // DepthOfPlaneBackprojection (depthmap.cc:114-118) in FP64 with the
 // reference's order: r = plane^T K^-1, s = r . (x, y, 1).
 __device__ __forceinline__ float depth_of_plane_f64(const double* Kinv,

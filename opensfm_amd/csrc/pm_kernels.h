@@ -12,7 +12,7 @@ hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
 hipError_t launch_sweep(const PmArgs& a, const float* Htab, const float* planes,
                         int nplanes, hipStream_t s);
 hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
-                            const float* pl, float* out, hipStream_t s);
+                            const float* Hs, float* out, hipStream_t s);
 hipError_t launch_pm_init(const PmArgs& a, hipStream_t s);
 hipError_t launch_pm_half(const PmArgs& a, int colour, int iter, hipStream_t s);
 hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,

@@ -107,14 +107,14 @@ __global__ __launch_bounds__(256) void k_sweep(PmArgs a,
 
 // ---------------------------------------------------------------------------
 // Fixed-hypothesis scoring: ComputePlaneImageScore for a list of
-// (pixel, plane) pairs against one neighbour view, with the homography
-// composed in FP64 on the device in the reference's order.  out[k] is the
+// (pixel, homography) pairs against one neighbour view; the host composes
+// each FP32 homography in FP64 with the reference's order.  out[k] is the
 // FP32 NCC score.  The host groups hypotheses by view (one launch each).
 
 template <int P>
 __global__ __launch_bounds__(256) void k_score_hyp(PmArgs a, int view, int n,
                                                    const int* __restrict__ ij,
-                                                   const float* __restrict__ pl,
+                                                   const float* __restrict__ Hs,
                                                    float* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -123,8 +123,8 @@ __global__ __launch_bounds__(256) void k_score_hyp(PmArgs a, int view, int n,
   load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut);
   const PmView& V = a.views[view];
   float H[9];
-  homography_baked_f64(a.Kinv0, V.Q, V.a, V.K, pl[3 * k], pl[3 * k + 1],
-                       pl[3 * k + 2], H);
+#pragma unroll
+  for (int m = 0; m < 9; ++m) H[m] = Hs[9 * (size_t)k + m];
   const QuadTex tex = make_tex(V.quad, V.width, V.height);
   out[k] = score_exact<P>(rp, H, i, j, tex);
 }
@@ -549,10 +549,10 @@ hipError_t launch_sweep(const PmArgs& a, const float* Htab, const float* planes,
 }
 
 hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
-                            const float* pl, float* out, hipStream_t s) {
+                            const float* Hs, float* out, hipStream_t s) {
   const int P = 2 * a.hpz + 1;
   PM_DISPATCH_P(P, hipLaunchKernelGGL(k_score_hyp<P>, dim3((n + 255) / 256),
-                                      dim3(256), 0, s, a, view, n, ij, pl, out));
+                                      dim3(256), 0, s, a, view, n, ij, Hs, out));
   return hipGetLastError();
 }
 

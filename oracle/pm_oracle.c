@@ -321,6 +321,7 @@ typedef struct {
   int num_depth_planes;
   int iterations;
   float min_patch_variance;
+  uint64_t seed;     /* each compute restarts the random stream from here */
   uint64_t rng;
   int64_t scorings; /* ComputePlaneImageScore calls of the last compute */
   /* result maps of the running compute */
@@ -334,7 +335,7 @@ void* orc_create(void) {
   e->num_depth_planes = 50;
   e->iterations = 3;
   e->min_patch_variance = 5 * 5;
-  e->rng = 0x5EEDull;
+  e->seed = 0x5EEDull;
   return e;
 }
 
@@ -400,7 +401,7 @@ void orc_set_depth_range(void* p, double mn, double mx, int planes) {
 void orc_set_patchmatch_iterations(void* p, int n) { ((orc_est*)p)->iterations = n; }
 void orc_set_patch_size(void* p, int s) { ((orc_est*)p)->patch_size = s; }
 void orc_set_min_patch_sd(void* p, float sd) { ((orc_est*)p)->min_patch_variance = sd * sd; }
-void orc_set_seed(void* p, uint64_t seed) { ((orc_est*)p)->rng = seed; }
+void orc_set_seed(void* p, uint64_t seed) { ((orc_est*)p)->seed = seed; }
 int64_t orc_last_scorings(void* p) { return ((orc_est*)p)->scorings; }
 
 /* ComputePlaneImageScore  depthmap.cc:441-478 (local-affine warp) */
@@ -725,6 +726,7 @@ int orc_compute(void* p, int mode, float* depth, float* plane, float* score,
   e->score = score;
   e->nghbr = nghbr;
   e->scorings = 0;
+  e->rng = e->seed;
   if (mode == 0) {
     brute_force(e);
   } else {
