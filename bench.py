@@ -150,7 +150,9 @@ def main():
         n = args.copy_calibration * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        # one elementwise kernel that reads and writes exactly MIB MiB each
+        # (a same-dtype copy_ would go to the DMA engine, not a kernel)
+        torch.neg(src, out=dst)
         torch.cuda.synchronize()
         log(f"copy calibration: {args.copy_calibration} MiB read + written")
         del src, dst

@@ -172,6 +172,22 @@ __device__ __forceinline__ bool load_ref_patch(RefPatch<P>& rp,
 }
 
 // ---------------------------------------------------------------------------
+// Correctly rounded FP32 square root for normal positive x: the hardware
+// estimate (within 1 ulp) corrected to the neighbouring float by the sign of
+// the FMA residuals x - s_down * s and x - s_up * s.
+
+__device__ __forceinline__ float sqrt_rn(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sdn = __int_as_float(__float_as_int(s) - 1);
+  const float sup = __int_as_float(__float_as_int(s) + 1);
+  const float vp = __builtin_fmaf(-sdn, s, x);
+  const float vs = __builtin_fmaf(-sup, s, x);
+  float r = (vp <= 0.0f) ? sdn : s;
+  r = (vs > 0.0f) ? sup : r;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
 // Texel gathers through a buffer resource: 32-bit byte offsets (one VGPR per
 // address), hardware range check as a second line of defence.
 
@@ -292,7 +308,8 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
   const float meanxy = __fdiv_rn(sumxy, rp.sumw);
   const float vary = meanyy - meany * meany;
   if (rp.varx < 0.1 || vary < 0.1) return -1.0f;
-  return __fdiv_rn(meanxy - rp.meanx * meany, __fsqrt_rn(rp.varx * vary));
+  // var_x, var_y >= 0.1 here, so the product is a normal float
+  return __fdiv_rn(meanxy - rp.meanx * meany, sqrt_rn(rp.varx * vary));
 }
 
 // ---------------------------------------------------------------------------
