@@ -76,7 +76,8 @@ _lib = None
 
 
 def library_path() -> str:
-    return _build.LIB
+    """The in-tree build, or the library named by OPENSFM_AMD_LIB."""
+    return os.environ.get("OPENSFM_AMD_LIB") or _build.LIB
 
 
 def load():

@@ -105,7 +105,7 @@ template <int P>
 struct RefPatch {
   static constexpr int N = P * P;
   float w[N];
-  uint32_t xp[(N + 3) / 4];
+  mutable uint32_t xp[(N + 3) / 4];  // made opaque in place per scoring
   float sumw, meanx, varx;  // exact-order FP32
   float rsumw;              // 1 / sumw for the fast path
   __device__ __forceinline__ float x(int k) const {
@@ -118,13 +118,10 @@ struct RefPatch {
 // the 49 products w*x) out of the hypothesis loops into registers.
 template <int P>
 struct XUnpack {
-  uint32_t xp[(P * P + 3) / 4];
-  __device__ __forceinline__ explicit XUnpack(const RefPatch<P>& rp) {
+  const uint32_t* xp;
+  __device__ __forceinline__ explicit XUnpack(const RefPatch<P>& rp) : xp(rp.xp) {
 #pragma unroll
-    for (int m = 0; m < (P * P + 3) / 4; ++m) {
-      xp[m] = rp.xp[m];
-      asm volatile("" : "+v"(xp[m]));
-    }
+    for (int m = 0; m < (P * P + 3) / 4; ++m) asm volatile("" : "+v"(rp.xp[m]));
   }
   __device__ __forceinline__ float operator()(int k) const {
     return (float)((xp[k >> 2] >> (8 * (k & 3))) & 0xffu);
@@ -299,7 +296,11 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
       sumyy += wy * y;
       sumxy += (rp.w[k] * xk(k)) * y;
     }
-    // one patch row of gathers in flight at a time: bounds register use
+    // One patch row of gathers in flight at a time, and the row's sums
+    // materialised here (otherwise the optimiser sinks the Sum w x y chain
+    // past the last row and keeps every row's samples live): bounds
+    // register use.
+    asm volatile("" : "+v"(sumy), "+v"(sumyy), "+v"(sumxy));
     __builtin_amdgcn_sched_barrier(0);
   }
   // NCCEstimator::Get (depthmap.cc:58-74)
@@ -347,6 +348,7 @@ __device__ __forceinline__ float score_fast_affine(
       sumyy = __builtin_fmaf(wy, y, sumyy);
       sumxy = __builtin_fmaf(xk(k), wy, sumxy);
     }
+    asm volatile("" : "+v"(sumy), "+v"(sumyy), "+v"(sumxy));
     __builtin_amdgcn_sched_barrier(0);
   }
   const float meany = sumy * rp.rsumw;

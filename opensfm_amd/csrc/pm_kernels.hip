@@ -21,6 +21,33 @@ __device__ __forceinline__ void wave_count(unsigned long long* c, unsigned v) {
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(c, (unsigned long long)v);
 }
 
+// XCD-aware tile order.  The MI355X deals workgroups of a launch round-robin
+// to its 8 XCDs, each with its own L2.  Renumbering so that XCD x walks the
+// contiguous row-major tile range [x * per, (x + 1) * per) keeps the
+// neighbour-view footprint of the tiles in flight on one XCD inside that
+// XCD's L2.  Launch with a 1-D grid of 8 * per workgroups (xcd_grid).
+struct Tile {
+  int x, y;
+  bool ok;
+};
+
+__device__ __forceinline__ Tile xcd_tile(int tiles_x, int tiles_y) {
+  const int total = tiles_x * tiles_y;
+  const int per = (total + 7) >> 3;
+  const int L = blockIdx.x;
+  const int t = (L & 7) * per + (L >> 3);
+  Tile r;
+  r.ok = t < total;
+  r.x = t % tiles_x;
+  r.y = t / tiles_x;
+  return r;
+}
+
+static inline dim3 xcd_grid(int tiles_x, int tiles_y) {
+  const int total = tiles_x * tiles_y;
+  return dim3(8 * ((total + 7) / 8));
+}
+
 // ---------------------------------------------------------------------------
 // Zero-bordered quad texture from a u8 image (see pm_device.h).
 
@@ -53,8 +80,10 @@ __device__ __forceinline__ unsigned sweep_pixel(const PmArgs& a,
                                                 const float* __restrict__ planes,
                                                 int nplanes) {
   constexpr int hpz = (P - 1) / 2;
-  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const Tile t = xcd_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
+  if (!t.ok) return 0;
+  const int j = t.x * 16 + (threadIdx.x & 15);
+  const int i = t.y * 16 + (threadIdx.x >> 4);
   if (i < hpz || i >= a.H - hpz || j < hpz || j >= a.W - hpz) return 0;
   RefPatch<P> rp;
   if (!load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) return 0;
@@ -224,8 +253,10 @@ __device__ __forceinline__ bool is_interior(const PmArgs& a, int i, int j) {
 template <int P>
 __device__ __forceinline__ unsigned init_pixel(const PmArgs& a) {
   constexpr int hpz = (P - 1) / 2;
-  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const Tile t = xcd_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
+  if (!t.ok) return 0;
+  const int j = t.x * 16 + (threadIdx.x & 15);
+  const int i = t.y * 16 + (threadIdx.x >> 4);
   if (!is_interior(a, i, j)) return 0;
   const size_t idx = (size_t)i * a.W + j;
   if (a.ref_mask[idx] == 0) return 0;
@@ -285,8 +316,10 @@ __global__ __launch_bounds__(256) void k_pm_init(PmArgs a) {
 template <int P>
 __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
                                                int iter) {
-  const int cx = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const Tile t = xcd_tile(((a.W + 1) / 2 + 15) >> 4, (a.H + 15) >> 4);
+  if (!t.ok) return 0;
+  const int cx = t.x * 16 + (threadIdx.x & 15);
+  const int i = t.y * 16 + (threadIdx.x >> 4);
   const int j = 2 * cx + ((i + colour) & 1);
   if (!is_interior(a, i, j)) return 0;
   const size_t idx = (size_t)i * a.W + j;
@@ -299,36 +332,11 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   float best_s = a.score[idx];
   int best_n = a.nghbr[idx];
 
-  // Propagation candidates: the 4 direct neighbours and, per direction, the
-  // best-scoring of n_far probes at odd distances 3, 5, 7, ...
-  float4 cand[8];
-  bool have[8];
-  const int di[4] = {-1, 1, 0, 0};
-  const int dj[4] = {0, 0, -1, 1};
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    cand[k] = st[(size_t)(i + di[k]) * a.W + (j + dj[k])];
-    have[k] = cand[k].w != 0.0f;
-    float bs = -2.0f;
-    size_t bidx = idx;
-    for (int m = 0; m < a.n_far; ++m) {
-      const int dist = 3 + 2 * m;
-      const int ii = i + di[k] * dist, jj = j + dj[k] * dist;
-      if (!is_interior(a, ii, jj)) break;
-      const size_t q = (size_t)ii * a.W + jj;
-      const float s = a.score[q];
-      if (s > bs) {
-        bs = s;
-        bidx = q;
-      }
-    }
-    cand[4 + k] = st[bidx];
-    have[4 + k] = bs > -2.0f && cand[4 + k].w != 0.0f;
-  }
-
   // One loop over 8 propagation + 6 refinement candidates, so the scorer is
-  // inlined once.  Refinement k perturbs the current best plane with the
-  // reference's schedule: depth * exp(0.02 * 0.3^k N), slopes + 0.5 * 0.8^k N
+  // inlined once.  Propagation: the 4 direct neighbours, then per direction
+  // the best-scoring of n_far probes at odd distances 3, 5, 7, ...
+  // Refinement k perturbs the current best plane with the reference's
+  // schedule: depth * exp(0.02 * 0.3^k N), slopes + 0.5 * 0.8^k N
   // (depthmap.cc:333-360).
   unsigned tries = 0;
   float depth_range = 0.02f, normal_range = 0.5f;
@@ -336,10 +344,32 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
     float v0, v1, v2;
     bool go;
     if (c < 8) {
-      v0 = cand[c].x;
-      v1 = cand[c].y;
-      v2 = cand[c].z;
-      go = have[c] && !(v0 == cur.x && v1 == cur.y && v2 == cur.z);
+      const int k = c & 3;
+      const int di = k == 0 ? -1 : (k == 1 ? 1 : 0);
+      const int dj = k == 2 ? -1 : (k == 3 ? 1 : 0);
+      size_t q = (size_t)(i + di) * a.W + (j + dj);
+      bool ok = true;
+      if (c >= 4) {
+        float bs = -2.0f;
+        q = idx;
+        for (int m = 0; m < a.n_far; ++m) {
+          const int dist = 3 + 2 * m;
+          const int ii = i + di * dist, jj = j + dj * dist;
+          if (!is_interior(a, ii, jj)) break;
+          const size_t p = (size_t)ii * a.W + jj;
+          const float s = a.score[p];
+          if (s > bs) {
+            bs = s;
+            q = p;
+          }
+        }
+        ok = bs > -2.0f;
+      }
+      const float4 nb = st[q];
+      v0 = nb.x;
+      v1 = nb.y;
+      v2 = nb.z;
+      go = ok && nb.w != 0.0f && !(v0 == cur.x && v1 == cur.y && v2 == cur.z);
     } else {
       const int k = c - 8;
       const u4 r = philox4x32_10(a.seed_lo ^ (uint32_t)idx, a.seed_hi,
@@ -543,7 +573,7 @@ hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
 hipError_t launch_sweep(const PmArgs& a, const float* Htab, const float* planes,
                         int nplanes, hipStream_t s) {
   const int P = 2 * a.hpz + 1;
-  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_sweep<P>, grid16(a.W, a.H), dim3(256),
+  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_sweep<P>, xcd_grid((a.W + 15) / 16, (a.H + 15) / 16), dim3(256),
                                       0, s, a, Htab, planes, nplanes));
   return hipGetLastError();
 }
@@ -558,7 +588,7 @@ hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
 
 hipError_t launch_pm_init(const PmArgs& a, hipStream_t s) {
   const int P = 2 * a.hpz + 1;
-  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_init<P>, grid16(a.W, a.H), dim3(256),
+  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_init<P>, xcd_grid((a.W + 15) / 16, (a.H + 15) / 16), dim3(256),
                                       0, s, a));
   return hipGetLastError();
 }
@@ -567,7 +597,7 @@ hipError_t launch_pm_half(const PmArgs& a, int colour, int iter,
                           hipStream_t s) {
   const int P = 2 * a.hpz + 1;
   PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_half<P>,
-                                      grid16((a.W + 1) / 2, a.H), dim3(256), 0,
+                                      xcd_grid(((a.W + 1) / 2 + 15) / 16, (a.H + 15) / 16), dim3(256), 0,
                                       s, a, colour, iter));
   return hipGetLastError();
 }
