@@ -143,7 +143,7 @@ def main():
         log(f"CPU baseline {cpu['value']:.4f} Mpix/s ({cpu['wall_s']:.1f} s)")
 
     from opensfm_amd import DepthmapEstimator, scenes
-    from opensfm_amd._capi import PM_MODE_PATCHMATCH
+    from opensfm_amd._capi import PM_MODE_PATCHMATCH, PM_MODE_PLANE_SWEEP
 
     if args.copy_calibration:
         import torch
@@ -186,6 +186,18 @@ def main():
     est.compute_patch_match()
     pcie_s = time.perf_counter() - t1
 
+    # ComputePlaneSweep over 64 fronto-parallel planes (exact FP32 path)
+    est.run(PM_MODE_PLANE_SWEEP)
+    t2 = time.perf_counter()
+    est.run(PM_MODE_PLANE_SWEEP)
+    sweep_s = time.perf_counter() - t2
+    sst = est.stats()
+    sweep_flop = sst["dominant_scorings"] * (FLOP_PER_SAMPLE * w["patch"] ** 2 + FLOP_PER_SCORING)
+    sweep = {"planes": 64, "mpix_s": round(w["width"] * w["height"] / sweep_s / 1e6, 3),
+             "kernel_ms": round(sst["dominant_ms"], 3),
+             "achieved_tflops": round(sweep_flop / (sst["dominant_ms"] / 1e3) / 1e12, 3)}
+    log(f"plane sweep 64 planes: {sweep}")
+
     npx = w["width"] * w["height"]
     ms_per_step = elapsed / args.steps * 1e3
     value = npx * args.steps / elapsed / 1e6
@@ -216,6 +228,7 @@ def main():
         "accuracy_within_1pct": round(acc, 4),
         "mean_score": round(mean_score, 4),
         "pcie_inclusive_mpix_s": round(npx / pcie_s / 1e6, 3),
+        "plane_sweep": sweep,
         "roofline": {
             "kernel": "k_pm_half<7> (red/black PatchMatch half-iteration)",
             "bound": "valu",

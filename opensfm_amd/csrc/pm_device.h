@@ -95,37 +95,18 @@ __device__ __forceinline__ float depth_of_plane_f64(const double* Kinv,
 // ---------------------------------------------------------------------------
 // Reference-side patch data: hypothesis-invariant per pixel.  w[k] is the
 // bilateral weight of sample k (from the host-tabulated table, bit-identical
-// to expf of the reference's argument); the u8 intensities x[k] are kept
-// packed four to a register and w[k] * x[k] is re-formed when needed (the
-// same FP32 product the reference forms).  The reference accumulates
-// Sum w x, Sum w x x and Sum w in sample order; those sums do not depend on
-// the hypothesis and are formed once.
+// to expf of the reference's argument) and wx[k] = w[k] * x[k], the same
+// FP32 product the reference forms in NCCEstimator::Push.  The reference
+// accumulates Sum w x, Sum w x x and Sum w in sample order; those sums do
+// not depend on the hypothesis and are formed once.
 
 template <int P>
 struct RefPatch {
   static constexpr int N = P * P;
   float w[N];
-  mutable uint32_t xp[(N + 3) / 4];  // made opaque in place per scoring
+  float wx[N];              // w[k] * x[k], the FP32 product the reference forms
   float sumw, meanx, varx;  // exact-order FP32
   float rsumw;              // 1 / sumw for the fast path
-  __device__ __forceinline__ float x(int k) const {
-    return (float)((xp[k >> 2] >> (8 * (k & 3))) & 0xffu);
-  }
-};
-
-// Packed intensities re-read per scoring.  The empty asm makes them opaque
-// to the optimiser, which would otherwise hoist the 49 unpacked floats (and
-// the 49 products w*x) out of the hypothesis loops into registers.
-template <int P>
-struct XUnpack {
-  const uint32_t* xp;
-  __device__ __forceinline__ explicit XUnpack(const RefPatch<P>& rp) : xp(rp.xp) {
-#pragma unroll
-    for (int m = 0; m < (P * P + 3) / 4; ++m) asm volatile("" : "+v"(rp.xp[m]));
-  }
-  __device__ __forceinline__ float operator()(int k) const {
-    return (float)((xp[k >> 2] >> (8 * (k & 3))) & 0xffu);
-  }
 };
 
 // Bilateral weight table: wlut[|dcolor| * nr2 + (dx*dx + dy*dy)], nr2 =
@@ -140,8 +121,6 @@ __device__ __forceinline__ bool load_ref_patch(RefPatch<P>& rp,
   const int c = img[i * W + j];
   float sumx = 0, sumxx = 0, sumw = 0;
 #pragma unroll
-  for (int k = 0; k < (P * P + 3) / 4; ++k) rp.xp[k] = 0;
-#pragma unroll
   for (int dy = -hpz; dy <= hpz; ++dy) {
 #pragma unroll
     for (int dx = -hpz; dx <= hpz; ++dx) {
@@ -151,8 +130,8 @@ __device__ __forceinline__ bool load_ref_patch(RefPatch<P>& rp,
       const float w = wlut[(dc < 0 ? -dc : dc) * nr2 + dx * dx + dy * dy];
       const float xf = (float)x;
       rp.w[k] = w;
-      rp.xp[k >> 2] |= (uint32_t)x << (8 * (k & 3));
       const float wx = w * xf;
+      rp.wx[k] = wx;
       sumx += wx;
       sumxx += wx * xf;
       sumw += w;
@@ -239,6 +218,27 @@ __device__ __forceinline__ uint32_t fetch_quad(const QuadTex& t, float flx,
   return __builtin_amdgcn_raw_buffer_load_b32(t.rsrc, rec << 2, 0, 0);
 }
 
+// Global-memory texel source for the scorers.
+struct GlobalFetch {
+  QuadTex tex;
+  __device__ __forceinline__ uint32_t operator()(float flx, float fly) const {
+    return fetch_quad(tex, flx, fly);
+  }
+};
+
+// Local-affine warp of a reference pixel into one view: the pixel's image
+// (Hx0, Hy0) and the Jacobian columns (dfdx_*, dfdy_*).
+struct Warp {
+  float Hx0, Hy0, dfdx_x, dfdx_y, dfdy_x, dfdy_y;
+};
+
+// One patch row of gathered texels and their bilinear fractions.
+template <int P>
+struct RowFetch {
+  uint32_t t[P];
+  float fx[P], fy[P];
+};
+
 // ---------------------------------------------------------------------------
 // Exact scorer: ComputePlaneImageScore (depthmap.cc:441-478) with the
 // reference's FP32 operation order, given the FP32 homography H.
@@ -272,21 +272,35 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
     cx[dx + hpz] = Hx0 + dfdx_x * (float)dx;
     cy[dx + hpz] = Hy0 + dfdx_y * (float)dx;
   }
-  const XUnpack<P> xk(rp);
-  float sumy = 0, sumyy = 0, sumxy = 0;
-#pragma unroll
-  for (int dy = -hpz; dy <= hpz; ++dy) {
+  // Two-stage row pipeline: the gathers of patch row r+1 are issued before
+  // row r is blended and accumulated (two rows of texels and fractions
+  // live), so one row's loads are in flight while the previous computes.
+  RowFetch<P> buf[2];
+  auto fetch = [&](RowFetch<P>& f, int dy) {
     const float rx = dfdy_x * (float)dy;
     const float ry = dfdy_y * (float)dy;
 #pragma unroll
     for (int dx = -hpz; dx <= hpz; ++dx) {
-      const int k = (dy + hpz) * P + (dx + hpz);
       const float x2 = cx[dx + hpz] + rx;
       const float y2 = cy[dx + hpz] + ry;
       const float flx = floorf(x2), fly = floorf(y2);
-      const uint32_t t = fetch_quad(tex, flx, fly);
-      const float fx = x2 - flx;
-      const float fy = y2 - fly;
+      f.t[dx + hpz] = fetch_quad(tex, flx, fly);
+      f.fx[dx + hpz] = x2 - flx;
+      f.fy[dx + hpz] = y2 - fly;
+    }
+  };
+  float sumy = 0, sumyy = 0, sumxy = 0;
+  fetch(buf[0], -hpz);
+#pragma unroll
+  for (int r = 0; r < P; ++r) {
+    if (r + 1 < P) fetch(buf[(r + 1) & 1], r + 1 - hpz);
+    __builtin_amdgcn_sched_barrier(0);
+    const RowFetch<P>& f = buf[r & 1];
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      const int k = r * P + c;
+      const uint32_t t = f.t[c];
+      const float fx = f.fx[c], fy = f.fy[c];
       const float im0 = (1.0f - fx) * ub0(t) + fx * ub1(t);
       const float im1 = (1.0f - fx) * ub2(t) + fx * ub3(t);
       float y = (1.0f - fy) * im0 + fy * im1;
@@ -294,12 +308,10 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
       const float wy = rp.w[k] * y;
       sumy += wy;
       sumyy += wy * y;
-      sumxy += (rp.w[k] * xk(k)) * y;
+      sumxy += rp.wx[k] * y;
     }
-    // One patch row of gathers in flight at a time, and the row's sums
-    // materialised here (otherwise the optimiser sinks the Sum w x y chain
-    // past the last row and keeps every row's samples live): bounds
-    // register use.
+    // the row's sums materialised here (otherwise the optimiser sinks the
+    // accumulation chains past the last row and keeps every sample live)
     asm volatile("" : "+v"(sumy), "+v"(sumyy), "+v"(sumxy));
     __builtin_amdgcn_sched_barrier(0);
   }
@@ -315,16 +327,21 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
 
 // ---------------------------------------------------------------------------
 // Fast scorer for PatchMatch: the same local-affine bilateral NCC, with FMAs
-// and one-ulp reciprocals.  Takes the warp at the pixel: (Hx0, Hy0) and the
-// Jacobian columns.  A non-finite coordinate yields a NaN score, which never
-// wins a strict comparison.
+// and one-ulp reciprocals.  Takes the pixel's warp and a texel source
+// (GlobalFetch, or any functor returning the texel of floor coordinates).
+// A non-finite coordinate yields a NaN score,
+// which never wins a strict comparison.
 
-template <int P>
-__device__ __forceinline__ float score_fast_affine(
-    const RefPatch<P>& rp, float Hx0, float Hy0, float dfdx_x, float dfdx_y,
-    float dfdy_x, float dfdy_y, const QuadTex& tex) {
+template <int P, class Fetch>
+__device__ __forceinline__ float score_fast_affine(const RefPatch<P>& rp,
+                                                   const Warp& wp,
+                                                   const Fetch& fetch_texel) {
   constexpr int hpz = (P - 1) / 2;
-  const XUnpack<P> xk(rp);
+  const float Hx0 = wp.Hx0, Hy0 = wp.Hy0;
+  const float dfdx_x = wp.dfdx_x, dfdx_y = wp.dfdx_y;
+  const float dfdy_x = wp.dfdy_x, dfdy_y = wp.dfdy_y;
+  // One patch row of gathers in flight at a time (a two-row pipeline was
+  // measured no faster and costs ~20 registers).
   float sumy = 0, sumyy = 0, sumxy = 0;
 #pragma unroll
   for (int dy = -hpz; dy <= hpz; ++dy) {
@@ -336,9 +353,8 @@ __device__ __forceinline__ float score_fast_affine(
       const float x2 = __builtin_fmaf(dfdx_x, (float)dx, rx);
       const float y2 = __builtin_fmaf(dfdx_y, (float)dx, ry);
       const float flx = floorf(x2), fly = floorf(y2);
-      const uint32_t t = fetch_quad(tex, flx, fly);
-      const float fx = x2 - flx;
-      const float fy = y2 - fly;
+      const uint32_t t = fetch_texel(flx, fly);
+      const float fx = x2 - flx, fy = y2 - fly;
       const float a = ub0(t), b = ub1(t), c = ub2(t), d = ub3(t);
       const float im0 = __builtin_fmaf(fx, b - a, a);
       const float im1 = __builtin_fmaf(fx, d - c, c);
@@ -346,8 +362,10 @@ __device__ __forceinline__ float score_fast_affine(
       const float wy = rp.w[k] * y;
       sumy += wy;
       sumyy = __builtin_fmaf(wy, y, sumyy);
-      sumxy = __builtin_fmaf(xk(k), wy, sumxy);
+      sumxy = __builtin_fmaf(rp.wx[k], y, sumxy);
     }
+    // the row's sums materialised here (otherwise the optimiser sinks the
+    // accumulation chains past the last row and keeps every sample live)
     asm volatile("" : "+v"(sumy), "+v"(sumyy), "+v"(sumxy));
     __builtin_amdgcn_sched_barrier(0);
   }

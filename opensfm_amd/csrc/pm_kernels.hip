@@ -21,31 +21,30 @@ __device__ __forceinline__ void wave_count(unsigned long long* c, unsigned v) {
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(c, (unsigned long long)v);
 }
 
-// XCD-aware tile order.  The MI355X deals workgroups of a launch round-robin
-// to its 8 XCDs, each with its own L2.  Renumbering so that XCD x walks the
-// contiguous row-major tile range [x * per, (x + 1) * per) keeps the
-// neighbour-view footprint of the tiles in flight on one XCD inside that
-// XCD's L2.  Launch with a 1-D grid of 8 * per workgroups (xcd_grid).
+// Tile order.  The MI355X deals the workgroups of a launch round-robin to its
+// 8 XCDs (each with its own L2), so consecutive row-major tiles land on
+// different XCDs.  Renumbering so that each XCD walks one contiguous band of
+// the image was measured 4 % slower on the 1600x1200 workload: the per-pixel
+// work varies across the image (masked pixels, skipped duplicate
+// candidates) and contiguous bands leave XCDs unevenly loaded, while the
+// round-robin order samples the whole image evenly.  Tiles therefore stay in
+// plain row-major order on a 1-D grid.
 struct Tile {
   int x, y;
   bool ok;
 };
 
-__device__ __forceinline__ Tile xcd_tile(int tiles_x, int tiles_y) {
-  const int total = tiles_x * tiles_y;
-  const int per = (total + 7) >> 3;
-  const int L = blockIdx.x;
-  const int t = (L & 7) * per + (L >> 3);
+__device__ __forceinline__ Tile row_tile(int tiles_x, int tiles_y) {
+  const int t = blockIdx.x;
   Tile r;
-  r.ok = t < total;
+  r.ok = t < tiles_x * tiles_y;
   r.x = t % tiles_x;
   r.y = t / tiles_x;
   return r;
 }
 
-static inline dim3 xcd_grid(int tiles_x, int tiles_y) {
-  const int total = tiles_x * tiles_y;
-  return dim3(8 * ((total + 7) / 8));
+static inline dim3 tile_grid(int tiles_x, int tiles_y) {
+  return dim3(tiles_x * tiles_y);
 }
 
 // ---------------------------------------------------------------------------
@@ -80,7 +79,7 @@ __device__ __forceinline__ unsigned sweep_pixel(const PmArgs& a,
                                                 const float* __restrict__ planes,
                                                 int nplanes) {
   constexpr int hpz = (P - 1) / 2;
-  const Tile t = xcd_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
+  const Tile t = row_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
   if (!t.ok) return 0;
   const int j = t.x * 16 + (threadIdx.x & 15);
   const int i = t.y * 16 + (threadIdx.x >> 4);
@@ -188,10 +187,61 @@ __device__ __forceinline__ void plane_from_depth_normal(const PixelRay& r,
   v[2] = n2 * inv;
 }
 
-// Score a plane against every neighbour view: max NCC with the lowest view
-// index winning ties (ComputePlaneScore, depthmap.cc:408-419).  Returns the
-// score; *nb gets the view and *depth the plane's depth at the pixel
-// (DepthOfPlaneBackprojection, depthmap.cc:114-118).
+// A plane v in the reference camera frame, prepared for scoring at one
+// pixel: g = K_0^-T v (first two components) and gq = v . K_0^-1 (j, i, 1).
+// Its depth at the pixel is 1 / max(1e-6, -gq) (DepthOfPlaneBackprojection,
+// depthmap.cc:114-118).
+struct PlaneAtPixel {
+  float g0, g1, gq;
+};
+
+__device__ __forceinline__ PlaneAtPixel plane_at_pixel(const PmArgs& a,
+                                                       const PixelRay& r,
+                                                       float v0, float v1,
+                                                       float v2) {
+  const float* Ki = a.Kinv0f;
+  PlaneAtPixel p;
+  p.g0 = __builtin_fmaf(v0, Ki[0], __builtin_fmaf(v1, Ki[3], v2 * Ki[6]));
+  p.g1 = __builtin_fmaf(v0, Ki[1], __builtin_fmaf(v1, Ki[4], v2 * Ki[7]));
+  p.gq = __builtin_fmaf(v0, r.x, __builtin_fmaf(v1, r.y, v2 * r.z));
+  return p;
+}
+
+// Local-affine warp of pixel (j, i) into view V under the plane:
+// H = M_v + b_v g^T (the FP32 form of PlaneInducedHomographyBaked), then
+// the image point and Jacobian of ComputePlaneImageScore
+// (depthmap.cc:448-462).  False when w == 0 or the warp is not finite
+// (the scorer then returns -1, as the reference does for w == 0).
+__device__ __forceinline__ bool warp_to_view(const PmView& V,
+                                             const PlaneAtPixel& p, float fj,
+                                             float fi, Warp* wp) {
+  const float* M = V.M;
+  const float* b = V.b;
+  const float u = __builtin_fmaf(b[0], p.gq, __builtin_fmaf(M[0], fj, __builtin_fmaf(M[1], fi, M[2])));
+  const float vv = __builtin_fmaf(b[1], p.gq, __builtin_fmaf(M[3], fj, __builtin_fmaf(M[4], fi, M[5])));
+  const float w = __builtin_fmaf(b[2], p.gq, __builtin_fmaf(M[6], fj, __builtin_fmaf(M[7], fi, M[8])));
+  const float H00 = __builtin_fmaf(b[0], p.g0, M[0]);
+  const float H01 = __builtin_fmaf(b[0], p.g1, M[1]);
+  const float H10 = __builtin_fmaf(b[1], p.g0, M[3]);
+  const float H11 = __builtin_fmaf(b[1], p.g1, M[4]);
+  const float H20 = __builtin_fmaf(b[2], p.g0, M[6]);
+  const float H21 = __builtin_fmaf(b[2], p.g1, M[7]);
+  const float rw = __frcp_rn(w);
+  wp->Hx0 = u * rw;
+  wp->Hy0 = vv * rw;
+  // (H00 w - H20 u) / w^2 = (H00 - H20 u/w) / w
+  wp->dfdx_x = __builtin_fmaf(-H20, wp->Hx0, H00) * rw;
+  wp->dfdx_y = __builtin_fmaf(-H20, wp->Hy0, H10) * rw;
+  wp->dfdy_x = __builtin_fmaf(-H21, wp->Hx0, H01) * rw;
+  wp->dfdy_y = __builtin_fmaf(-H21, wp->Hy0, H11) * rw;
+  const float s = wp->Hx0 + wp->Hy0 + wp->dfdx_x + wp->dfdx_y + wp->dfdy_x + wp->dfdy_y;
+  return w != 0.0f && __builtin_isfinite(s);
+}
+
+// Score a plane against every neighbour view from global memory: max NCC
+// with the lowest view index winning ties (ComputePlaneScore,
+// depthmap.cc:408-419).  Returns the score; *nb gets the view and *depth the
+// plane's depth at the pixel.
 template <int P>
 __device__ __forceinline__ float score_plane_all(const PmArgs& a,
                                                  const RefPatch<P>& rp,
@@ -199,40 +249,19 @@ __device__ __forceinline__ float score_plane_all(const PmArgs& a,
                                                  int j, float v0, float v1,
                                                  float v2, int* nb,
                                                  float* depth) {
-  const float* Ki = a.Kinv0f;
-  // g = K_0^-T v, gq = g . (j, i, 1) = v . ray
-  const float g0 = __builtin_fmaf(v0, Ki[0], __builtin_fmaf(v1, Ki[3], v2 * Ki[6]));
-  const float g1 = __builtin_fmaf(v0, Ki[1], __builtin_fmaf(v1, Ki[4], v2 * Ki[7]));
-  const float gq = __builtin_fmaf(v0, r.x, __builtin_fmaf(v1, r.y, v2 * r.z));
-  *depth = __frcp_rn(fmaxf(1e-6f, -gq));
+  const PlaneAtPixel p = plane_at_pixel(a, r, v0, v1, v2);
+  *depth = __frcp_rn(fmaxf(1e-6f, -p.gq));
   const float fj = (float)j, fi = (float)i;
   float best = -1.0f;
   int bn = 0;
   for (int v = 1; v < a.nviews; ++v) {
     const PmView& V = a.views[v];
-    const float* M = V.M;
-    const float* b = V.b;
-    const float u = __builtin_fmaf(b[0], gq, __builtin_fmaf(M[0], fj, __builtin_fmaf(M[1], fi, M[2])));
-    const float vv = __builtin_fmaf(b[1], gq, __builtin_fmaf(M[3], fj, __builtin_fmaf(M[4], fi, M[5])));
-    const float w = __builtin_fmaf(b[2], gq, __builtin_fmaf(M[6], fj, __builtin_fmaf(M[7], fi, M[8])));
+    Warp wp;
     float s = -1.0f;
-    if (w != 0.0f) {
-      const float H00 = __builtin_fmaf(b[0], g0, M[0]);
-      const float H01 = __builtin_fmaf(b[0], g1, M[1]);
-      const float H10 = __builtin_fmaf(b[1], g0, M[3]);
-      const float H11 = __builtin_fmaf(b[1], g1, M[4]);
-      const float H20 = __builtin_fmaf(b[2], g0, M[6]);
-      const float H21 = __builtin_fmaf(b[2], g1, M[7]);
-      const float rw = __frcp_rn(w);
-      const float Hx0 = u * rw, Hy0 = vv * rw;
-      // (H00 w - H20 u) / w^2 = (H00 - H20 u/w) / w
-      const float dfdx_x = __builtin_fmaf(-H20, Hx0, H00) * rw;
-      const float dfdx_y = __builtin_fmaf(-H20, Hy0, H10) * rw;
-      const float dfdy_x = __builtin_fmaf(-H21, Hx0, H01) * rw;
-      const float dfdy_y = __builtin_fmaf(-H21, Hy0, H11) * rw;
-      const QuadTex tex = make_tex(V.quad, V.width, V.height);
-      s = score_fast_affine<P>(rp, Hx0, Hy0, dfdx_x, dfdx_y, dfdy_x, dfdy_y,
-                               tex);
+    if (warp_to_view(V, p, fj, fi, &wp)) {
+      GlobalFetch g;
+      g.tex = make_tex(V.quad, V.width, V.height);
+      s = score_fast_affine<P>(rp, wp, g);
     }
     if (s > best) {
       best = s;
@@ -253,7 +282,7 @@ __device__ __forceinline__ bool is_interior(const PmArgs& a, int i, int j) {
 template <int P>
 __device__ __forceinline__ unsigned init_pixel(const PmArgs& a) {
   constexpr int hpz = (P - 1) / 2;
-  const Tile t = xcd_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
+  const Tile t = row_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
   if (!t.ok) return 0;
   const int j = t.x * 16 + (threadIdx.x & 15);
   const int i = t.y * 16 + (threadIdx.x >> 4);
@@ -306,17 +335,28 @@ __global__ __launch_bounds__(256) void k_pm_init(PmArgs a) {
   if (a.counter) wave_count(a.counter, n);
 }
 
-// One half of a red/black PatchMatch iteration: every interior pixel with
-// (i + j) % 2 == colour tests the planes of its four direct neighbours, the
-// best-scoring pixel among n_far odd-distance probes in each direction, and
-// then six random refinements of its current plane with the reference's
-// schedule (depth * exp(0.02 * 0.3^k N), slopes + 0.5 * 0.8^k N,
-// depthmap.cc:333-360).  Only opposite-colour pixels are read, so the
-// result does not depend on the order in which lanes run.
+// ---------------------------------------------------------------------------
+// Red/black half-iteration.
+//
+// Every interior pixel with (i + j) % 2 == colour tests the planes of its
+// four direct neighbours, the best-scoring pixel among n_far odd-distance
+// probes in each direction, and six random refinements of its current plane
+// with the reference's schedule (depth * exp(0.02 * 0.3^k N), slopes +
+// 0.5 * 0.8^k N, depthmap.cc:333-360), each scored over all views.  Only
+// opposite-colour pixels are read, so the result does not depend on the
+// order in which lanes run.
+//
+// Texels are gathered straight from global memory.  Staging each view's
+// footprint in LDS was measured slower on the 1600x1200 workload (per
+// converged iteration: 11.7 ms direct, 13.2 ms with per-wave LDS tiles,
+// 16.2 ms with per-workgroup tiles behind barriers): once neighbouring
+// pixels hold similar planes, a wave's gathers stay within a few L1 lines,
+// and building the footprint box costs more than the LDS reads save.
+
 template <int P>
 __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
                                                int iter) {
-  const Tile t = xcd_tile(((a.W + 1) / 2 + 15) >> 4, (a.H + 15) >> 4);
+  const Tile t = row_tile(((a.W + 1) / 2 + 15) >> 4, (a.H + 15) >> 4);
   if (!t.ok) return 0;
   const int cx = t.x * 16 + (threadIdx.x & 15);
   const int i = t.y * 16 + (threadIdx.x >> 4);
@@ -333,13 +373,10 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   int best_n = a.nghbr[idx];
 
   // One loop over 8 propagation + 6 refinement candidates, so the scorer is
-  // inlined once.  Propagation: the 4 direct neighbours, then per direction
-  // the best-scoring of n_far probes at odd distances 3, 5, 7, ...
-  // Refinement k perturbs the current best plane with the reference's
-  // schedule: depth * exp(0.02 * 0.3^k N), slopes + 0.5 * 0.8^k N
-  // (depthmap.cc:333-360).
+  // inlined once.
   unsigned tries = 0;
   float depth_range = 0.02f, normal_range = 0.5f;
+  float tx[8], ty[8], tz[8];  // propagation planes seen so far
   for (int c = 0; c < 14; ++c) {
     float v0, v1, v2;
     bool go;
@@ -369,7 +406,21 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
       v0 = nb.x;
       v1 = nb.y;
       v2 = nb.z;
-      go = ok && nb.w != 0.0f && !(v0 == cur.x && v1 == cur.y && v2 == cur.z);
+      // A plane bit-equal to the current one or to an earlier candidate
+      // scores the same and cannot win a strict comparison: skip it.
+      bool dup = v0 == cur.x && v1 == cur.y && v2 == cur.z;
+#pragma unroll
+      for (int m = 0; m < 7; ++m)
+        dup |= m < c && tx[m] == v0 && ty[m] == v1 && tz[m] == v2;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        if (m == c) {
+          tx[m] = v0;
+          ty[m] = v1;
+          tz[m] = v2;
+        }
+      }
+      go = ok && nb.w != 0.0f && !dup;
     } else {
       const int k = c - 8;
       const u4 r = philox4x32_10(a.seed_lo ^ (uint32_t)idx, a.seed_hi,
@@ -395,7 +446,7 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
       int nb;
       float dep;
       const float s = score_plane_all<P>(a, rp, ray, i, j, v0, v1, v2, &nb, &dep);
-      ++tries;
+      tries += (unsigned)(a.nviews - 1);
       if (s > best_s) {
         best_s = s;
         best_n = nb;
@@ -406,7 +457,7 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   reinterpret_cast<float4*>(a.state)[idx] = cur;
   a.score[idx] = best_s;
   a.nghbr[idx] = best_n;
-  return tries * (unsigned)(a.nviews - 1);
+  return tries;
 }
 
 template <int P>
@@ -573,7 +624,7 @@ hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
 hipError_t launch_sweep(const PmArgs& a, const float* Htab, const float* planes,
                         int nplanes, hipStream_t s) {
   const int P = 2 * a.hpz + 1;
-  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_sweep<P>, xcd_grid((a.W + 15) / 16, (a.H + 15) / 16), dim3(256),
+  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_sweep<P>, tile_grid((a.W + 15) / 16, (a.H + 15) / 16), dim3(256),
                                       0, s, a, Htab, planes, nplanes));
   return hipGetLastError();
 }
@@ -588,7 +639,7 @@ hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
 
 hipError_t launch_pm_init(const PmArgs& a, hipStream_t s) {
   const int P = 2 * a.hpz + 1;
-  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_init<P>, xcd_grid((a.W + 15) / 16, (a.H + 15) / 16), dim3(256),
+  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_init<P>, tile_grid((a.W + 15) / 16, (a.H + 15) / 16), dim3(256),
                                       0, s, a));
   return hipGetLastError();
 }
@@ -597,7 +648,7 @@ hipError_t launch_pm_half(const PmArgs& a, int colour, int iter,
                           hipStream_t s) {
   const int P = 2 * a.hpz + 1;
   PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_half<P>,
-                                      xcd_grid(((a.W + 1) / 2 + 15) / 16, (a.H + 15) / 16), dim3(256), 0,
+                                      tile_grid(((a.W + 1) / 2 + 15) / 16, (a.H + 15) / 16), dim3(256), 0,
                                       s, a, colour, iter));
   return hipGetLastError();
 }

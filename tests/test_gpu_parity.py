@@ -165,3 +165,40 @@ def test_patchmatch_accuracy_vs_oracle(cfg1_scene):
           f"gpu acc {np.mean(g_acc):.4f} score {np.mean(g_score):.4f}", flush=True)
     assert np.mean(g_acc) >= np.mean(o_acc) - 0.005
     assert np.mean(g_score) >= np.mean(o_score) - 0.005
+
+
+def test_plane_sweep_full_size_sampled_pixels():
+    """1600x1200 + 8 neighbours, 64 planes (the single-view workload): the
+    GPU sweep at 150 sampled interior pixels equals the oracle's per-pixel
+    brute force (ComputeBruteForce, depthmap.cc:184-200)."""
+    print("plane sweep 1600x1200, 64 planes, sampled pixels", flush=True)
+    s = scenes.scene_config2()
+    g = _setup(DepthmapEstimator(), s, planes=64)
+    depth, plane, score, nghbr = g.compute_plane_sweep()
+    o = _setup(oracle.OracleEstimator(), s, planes=64)
+    K = s.ref.K
+    rng = np.random.default_rng(17)
+    D = 64
+    n_views = len(s.views)
+    for _ in range(150):
+        i, j = int(rng.integers(3, 1197)), int(rng.integers(3, 1597))
+        best, bn, bd = np.float32(0), 0, -1
+        for d in range(D):
+            dep = np.float32(1 / (1 / 1.0 + d * (1 / 40.0 - 1 / 1.0) / (D - 1)))
+            v = oracle.plane_from_depth_and_normal(j, i, K, dep, np.array([0, 0, -1], np.float32))
+            sc = np.float32(-1)
+            nb = 0
+            for view in range(1, n_views):
+                x = np.float32(o.score(i, j, v, view))
+                if x > sc:
+                    sc, nb = x, view
+            if sc > best:
+                best, bn, bd = sc, nb, d
+        assert abs(score[i, j] - best) <= 1e-4
+        assert nghbr[i, j] == bn
+        if bd >= 0:
+            dep = np.float32(1 / (1 / 1.0 + bd * (1 / 40.0 - 1 / 1.0) / (D - 1)))
+            assert abs(depth[i, j] - dep) <= 1e-5 * dep
+        else:
+            assert depth[i, j] == 0
+
