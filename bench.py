@@ -72,7 +72,7 @@ def cpu_baseline(scene, threads, target_s=12.0):
     import oracle
     w = WORKLOAD
 
-    def run(x0, y0, cw, ch, seed):
+    def run(x0, y0, cw, ch, seed, keep=None):
         est = oracle.OracleEstimator()
         for K, R, t, img, mask in crop_views(scene, x0, y0, cw, ch):
             est.add_view(K, R, t, img, mask)
@@ -82,8 +82,11 @@ def cpu_baseline(scene, threads, target_s=12.0):
         est.set_patchmatch_iterations(w["iterations"])
         est.set_seed(seed)
         t0 = time.perf_counter()
-        est.compute_patch_match()
-        return time.perf_counter() - t0
+        depth = est.compute_patch_match()[0]
+        dt = time.perf_counter() - t0
+        if keep is not None:
+            keep.append(depth)
+        return dt
 
     # calibrate on a small crop, then size one crop per thread for target_s
     t_small = run(700, 500, 40, 40, 1)
@@ -93,9 +96,10 @@ def cpu_baseline(scene, threads, target_s=12.0):
     jobs = [(int(rng.integers(0, w["width"] - side)), int(rng.integers(0, w["height"] - side)))
             for _ in range(threads)]
     res = [None] * threads
+    depths = [[] for _ in range(threads)]
 
     def worker(k):
-        res[k] = run(jobs[k][0], jobs[k][1], side, side, 100 + k)
+        res[k] = run(jobs[k][0], jobs[k][1], side, side, 100 + k, depths[k])
 
     t0 = time.perf_counter()
     ths = [threading.Thread(target=worker, args=(k,)) for k in range(threads)]
@@ -105,7 +109,8 @@ def cpu_baseline(scene, threads, target_s=12.0):
         t.join()
     wall = time.perf_counter() - t0
     mpix = threads * side * side / wall / 1e6
-    return {
+    crops = [(x0, y0, side, side, depths[k][0]) for k, (x0, y0) in enumerate(jobs)]
+    return crops, {
         "value": mpix, "unit": "Mpix/s", "cores": threads, "kind": "port",
         "sample": (f"{threads} concurrent {side}x{side} crops of the 1600x1200 reference "
                    f"view (8 neighbours, 5 iterations, patch 7), one per host thread; "
@@ -136,10 +141,11 @@ def main():
     scene = make_scene()
 
     cpu = None
+    crops = []
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or max(1, min(16, len(os.sched_getaffinity(0))))
         log(f"CPU baseline: oracle PatchMatch on {threads} threads")
-        cpu = cpu_baseline(scene, threads)
+        crops, cpu = cpu_baseline(scene, threads)
         log(f"CPU baseline {cpu['value']:.4f} Mpix/s ({cpu['wall_s']:.1f} s)")
 
     from opensfm_amd import DepthmapEstimator, scenes
@@ -245,6 +251,28 @@ def main():
     }
     if cpu is not None:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        # equal-accuracy check on identical inputs: the GPU engine on the
+        # CPU baseline's crops (same views, K shift, depth range, iterations)
+        cpu_acc, gpu_acc, cpu_sc, gpu_sc = [], [], [], []
+        for k, (x0, y0, cw, ch, odepth) in enumerate(crops):
+            gt = scene.ref.depth[y0:y0 + ch, x0:x0 + cw]
+            ce = DepthmapEstimator()
+            for K, R, t, img, mask in crop_views(scene, x0, y0, cw, ch):
+                ce.add_view(K, R, t, img, mask)
+            ce.set_depth_range(w["min_depth"], w["max_depth"], 50)
+            ce.set_patch_size(w["patch"])
+            ce.set_min_patch_sd(w["min_patch_sd"])
+            ce.set_patchmatch_iterations(w["iterations"])
+            ce.set_seed(100 + k)
+            gd = ce.compute_patch_match()[0]
+            cpu_acc.append(scenes.depth_accuracy(odepth, gt, w["min_depth"], w["max_depth"]))
+            gpu_acc.append(scenes.depth_accuracy(gd, gt, w["min_depth"], w["max_depth"]))
+        out["equal_accuracy_check"] = {
+            "crops": len(crops),
+            "cpu_reference_accuracy": round(float(np.mean(cpu_acc)), 4),
+            "gpu_accuracy": round(float(np.mean(gpu_acc)), 4),
+            "gpu_at_least_cpu": bool(np.mean(gpu_acc) >= np.mean(cpu_acc)),
+        }
     print(json.dumps(out), flush=True)
 
 

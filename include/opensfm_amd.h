@@ -92,6 +92,10 @@ int pm_estimator_set_seed(pm_estimator* est, uint64_t seed);
 /* Far-propagation probes per direction of the red/black kernel (default 6;
  * 0 = direct neighbours only). */
 int pm_estimator_set_far_probes(pm_estimator* est, int n);
+/* Random refinements per pixel update (default 6, the reference's count,
+ * depthmap.cc:337; the 0.02 * 0.3^k / 0.5 * 0.8^k schedule restarts every
+ * 6 steps). */
+int pm_estimator_set_refinement_steps(pm_estimator* est, int n);
 /* Run PostProcess (5x5 median, depthmap.cc:489-502) after PatchMatch
  * (default 1, as the reference always does; 0 returns the raw depth). */
 int pm_estimator_set_post_process(pm_estimator* est, int enable);

@@ -55,7 +55,9 @@ SIGNATURES = {
     "pm_estimator_set_patch_size": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_min_patch_sd": (C.c_int, [_vp, C.c_float]),
     "pm_estimator_set_seed": (C.c_int, [_vp, C.c_uint64]),
-    "pm_estimator_set_far_probes": (C.c_int, [_vp, C.c_int]),    "pm_estimator_set_post_process": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_far_probes": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_refinement_steps": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_compute": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _ip]),
     "pm_estimator_run": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_download": (C.c_int, [_vp, _fp, _fp, _fp, _ip]),

@@ -188,6 +188,7 @@ struct pm_estimator {
   float min_patch_variance = 5 * 5;
   uint64_t seed = 0x5EEDull;
   int n_far = 6;
+  int n_refine = 6;
   int post_process = 1;
   std::vector<std::unique_ptr<HostView>> views;
   DevBuf mask0;
@@ -346,6 +347,7 @@ pm::PmArgs prepare(pm_estimator* e) {
   a.seed_lo = (uint32_t)(e->seed & 0xffffffffu);
   a.seed_hi = (uint32_t)(e->seed >> 32);
   a.n_far = e->n_far;
+  a.n_refine = e->n_refine;
   a.counter = e->counter.as<unsigned long long>();
   return a;
 }
@@ -356,8 +358,6 @@ int run(pm_estimator* e, int mode) {
   if (mode != PM_MODE_PLANE_SWEEP && mode != PM_MODE_PATCHMATCH &&
       mode != PM_MODE_PATCHMATCH_SAMPLE)
     return fail(PM_ERR_INVALID, "unknown compute mode " + std::to_string(mode));
-  if (mode == PM_MODE_PATCHMATCH_SAMPLE)
-    return fail(PM_ERR_INVALID, "compute_patch_match_sample is not implemented on the HIP engine yet");
   DeviceGuard dg(e->device);
   e->have_result = false;
   pm::PmArgs a = prepare(e);
@@ -402,13 +402,14 @@ int run(pm_estimator* e, int mode) {
     HIP_CHECK(hipEventElapsedTime(&dom_ms, e->ev[2], e->ev[3]));
     launches = 1;
   } else {
-    HIP_CHECK(pm::launch_pm_init(a, e->stream));
+    const bool sample = mode == PM_MODE_PATCHMATCH_SAMPLE;
+    HIP_CHECK(pm::launch_pm_init(a, sample, e->stream));
     pm::PmArgs ah = a;  // the red/black launches count into slot 1
     ah.counter = a.counter + 1;
     for (int it = 0; it < e->iterations; ++it) {
       for (int colour = 0; colour < 2; ++colour) {
         HIP_CHECK(hipEventRecord(e->ev[2], e->stream));
-        HIP_CHECK(pm::launch_pm_half(ah, colour, it, e->stream));
+        HIP_CHECK(pm::launch_pm_half(ah, sample, colour, it, e->stream));
         HIP_CHECK(hipEventRecord(e->ev[3], e->stream));
         HIP_CHECK(hipEventSynchronize(e->ev[3]));
         float ms = 0;
@@ -592,6 +593,13 @@ int pm_estimator_set_min_patch_sd(pm_estimator* e, float sd) {
 int pm_estimator_set_seed(pm_estimator* e, uint64_t seed) {
   if (!e) return fail(PM_ERR_INVALID, "null estimator");
   e->seed = seed;
+  return PM_OK;
+}
+
+int pm_estimator_set_refinement_steps(pm_estimator* e, int n) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (n < 0 || n > 48) return fail(PM_ERR_INVALID, "refinement steps must be in [0, 48]");
+  e->n_refine = n;
   return PM_OK;
 }
 

@@ -13,8 +13,9 @@ hipError_t launch_sweep(const PmArgs& a, const float* Htab, const float* planes,
                         int nplanes, hipStream_t s);
 hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
                             const float* Hs, float* out, hipStream_t s);
-hipError_t launch_pm_init(const PmArgs& a, hipStream_t s);
-hipError_t launch_pm_half(const PmArgs& a, int colour, int iter, hipStream_t s);
+hipError_t launch_pm_init(const PmArgs& a, bool sample, hipStream_t s);
+hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
+                          hipStream_t s);
 hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
                        float* score, int* nghbr, hipStream_t s);
 hipError_t launch_clean(const ClView* views, const ClRef* ref, int n, int W,

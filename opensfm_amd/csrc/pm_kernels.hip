@@ -272,6 +272,41 @@ __device__ __forceinline__ float score_plane_all(const PmArgs& a,
   return best;
 }
 
+// Score a plane against one view per lane (ComputePlaneImageScore for the
+// sample-mode PatchMatch, depthmap.cc:388-396), where the view may differ
+// between lanes.  The wave loops over the distinct views its pending lanes
+// ask for, so every texture access uses a wave-uniform view.  Lanes with
+// pend == false skip.  Returns the score (-1 if not scored) and the plane's
+// depth at the pixel in *depth.
+template <int P>
+__device__ __forceinline__ float score_plane_view(const PmArgs& a,
+                                                  const RefPatch<P>& rp,
+                                                  const PixelRay& r, int i,
+                                                  int j, float v0, float v1,
+                                                  float v2, int view,
+                                                  bool pend, float* depth) {
+  const PlaneAtPixel p = plane_at_pixel(a, r, v0, v1, v2);
+  *depth = __frcp_rn(fmaxf(1e-6f, -p.gq));
+  const float fj = (float)j, fi = (float)i;
+  float s = -1.0f;
+  while (true) {
+    const unsigned long long m = __ballot(pend);
+    if (m == 0) break;
+    const int vv = __shfl(view, __ffsll((long long)m) - 1);
+    if (pend && view == vv) {
+      const PmView& V = a.views[vv];
+      Warp wp;
+      if (warp_to_view(V, p, fj, fi, &wp)) {
+        GlobalFetch g;
+        g.tex = make_tex(V.quad, V.width, V.height);
+        s = score_fast_affine<P>(rp, wp, g);
+      }
+      pend = false;
+    }
+  }
+  return s;
+}
+
 __device__ __forceinline__ bool is_interior(const PmArgs& a, int i, int j) {
   return i >= a.hpz && i < a.H - a.hpz && j >= a.hpz && j < a.W - a.hpz;
 }
@@ -279,7 +314,7 @@ __device__ __forceinline__ bool is_interior(const PmArgs& a, int i, int j) {
 // Random initialisation + ignore mask (RandomInitialization and
 // ComputeIgnoreMask, depthmap.cc:237-281).  Masked and low-variance pixels
 // keep the zero state (the reference scores them and then zeroes them).
-template <int P>
+template <int P, bool kSample>
 __device__ __forceinline__ unsigned init_pixel(const PmArgs& a) {
   constexpr int hpz = (P - 1) / 2;
   const Tile t = row_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
@@ -318,8 +353,14 @@ __device__ __forceinline__ unsigned init_pixel(const PmArgs& a) {
   float s = -1.0f;
   int nb = 0;
   unsigned count = 0;
-  if (load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) {
-    float dtmp;
+  const bool textured = load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut);
+  float dtmp;
+  if constexpr (kSample) {
+    // one random neighbour view (uni_(rng_), depthmap.cc:247-249)
+    nb = 1 + (int)(r.w % (uint32_t)(a.nviews - 1));
+    s = score_plane_view<P>(a, rp, ray, i, j, v[0], v[1], v[2], nb, textured, &dtmp);
+    count = textured ? 1 : 0;
+  } else if (textured) {
     s = score_plane_all<P>(a, rp, ray, i, j, v[0], v[1], v[2], &nb, &dtmp);
     count = a.nviews - 1;
   }
@@ -329,9 +370,9 @@ __device__ __forceinline__ unsigned init_pixel(const PmArgs& a) {
   return count;
 }
 
-template <int P>
+template <int P, bool kSample>
 __global__ __launch_bounds__(256) void k_pm_init(PmArgs a) {
-  const unsigned n = init_pixel<P>(a);
+  const unsigned n = init_pixel<P, kSample>(a);
   if (a.counter) wave_count(a.counter, n);
 }
 
@@ -353,7 +394,7 @@ __global__ __launch_bounds__(256) void k_pm_init(PmArgs a) {
 // pixels hold similar planes, a wave's gathers stay within a few L1 lines,
 // and building the footprint box costs more than the LDS reads save.
 
-template <int P>
+template <int P, bool kSample>
 __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
                                                int iter) {
   const Tile t = row_tile(((a.W + 1) / 2 + 15) >> 4, (a.H + 15) >> 4);
@@ -372,14 +413,22 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   float best_s = a.score[idx];
   int best_n = a.nghbr[idx];
 
-  // One loop over 8 propagation + 6 refinement candidates, so the scorer is
-  // inlined once.
+  // One loop over 8 propagation + 6 refinement candidates (+ in sample
+  // mode the current plane against a random other view), so the scorer is
+  // inlined once.  In sample mode every candidate is scored against one view
+  // only (depthmap.cc:325-373): a propagated plane against its source
+  // pixel's view, a refinement against the pixel's view after propagation.
+  const int ncand = 8 + a.n_refine + (kSample ? 1 : 0);
   unsigned tries = 0;
   float depth_range = 0.02f, normal_range = 0.5f;
   float tx[8], ty[8], tz[8];  // propagation planes seen so far
-  for (int c = 0; c < 14; ++c) {
+  int tvw[8];                 // and their views (sample mode)
+  int cur_view = best_n;      // current_nghbr of depthmap.cc:336
+  for (int c = 0; c < ncand; ++c) {
     float v0, v1, v2;
     bool go;
+    int view = 0;
+    if (c == 8) cur_view = best_n;
     if (c < 8) {
       const int k = c & 3;
       const int di = k == 0 ? -1 : (k == 1 ? 1 : 0);
@@ -406,23 +455,44 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
       v0 = nb.x;
       v1 = nb.y;
       v2 = nb.z;
-      // A plane bit-equal to the current one or to an earlier candidate
-      // scores the same and cannot win a strict comparison: skip it.
-      bool dup = v0 == cur.x && v1 == cur.y && v2 == cur.z;
+      if constexpr (kSample) view = a.nghbr[q];
+      // A candidate bit-equal to the current one or to an earlier candidate
+      // (plane, and in sample mode view) scores the same and cannot win a
+      // strict comparison: skip it.
+      bool dup = v0 == cur.x && v1 == cur.y && v2 == cur.z &&
+                 (!kSample || view == best_n);
 #pragma unroll
       for (int m = 0; m < 7; ++m)
-        dup |= m < c && tx[m] == v0 && ty[m] == v1 && tz[m] == v2;
+        dup |= m < c && tx[m] == v0 && ty[m] == v1 && tz[m] == v2 &&
+               (!kSample || tvw[m] == view);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         if (m == c) {
           tx[m] = v0;
           ty[m] = v1;
           tz[m] = v2;
+          tvw[m] = view;
         }
       }
       go = ok && nb.w != 0.0f && !dup;
+    } else if (kSample && c == ncand - 1) {
+      // the current plane against a random view other than current_nghbr
+      // (depthmap.cc:362-373), only with at least two neighbour views
+      const u4 r = philox4x32_10(a.seed_lo ^ (uint32_t)idx, a.seed_hi,
+                                 u4{(uint32_t)iter, 14u, 2u, 0u});
+      v0 = cur.x;
+      v1 = cur.y;
+      v2 = cur.z;
+      go = a.nviews > 2 && cur_view >= 1;
+      int other = 1 + (int)(r.x % (uint32_t)max(1, a.nviews - 2));
+      if (other >= cur_view) ++other;
+      view = other;
     } else {
       const int k = c - 8;
+      if (k > 0 && k % 6 == 0) {  // the 6-step schedule restarts
+        depth_range = 0.02f;
+        normal_range = 0.5f;
+      }
       const u4 r = philox4x32_10(a.seed_lo ^ (uint32_t)idx, a.seed_hi,
                                  u4{(uint32_t)iter, (uint32_t)k, 1u, 0u});
       float g0, g1, g2, g3;
@@ -441,8 +511,20 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
       v0 = v[0];
       v1 = v[1];
       v2 = v[2];
+      view = cur_view;
     }
-    if (go) {
+    if constexpr (kSample) {
+      float dep;
+      const float s = score_plane_view<P>(a, rp, ray, i, j, v0, v1, v2, view, go, &dep);
+      if (go) {
+        tries += 1;
+        if (s > best_s) {
+          best_s = s;
+          best_n = view;
+          cur = make_float4(v0, v1, v2, dep);
+        }
+      }
+    } else if (go) {
       int nb;
       float dep;
       const float s = score_plane_all<P>(a, rp, ray, i, j, v0, v1, v2, &nb, &dep);
@@ -460,10 +542,10 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   return tries;
 }
 
-template <int P>
+template <int P, bool kSample>
 __global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour,
                                                  int iter) {
-  const unsigned n = half_pixel<P>(a, colour, iter);
+  const unsigned n = half_pixel<P, kSample>(a, colour, iter);
   if (a.counter) wave_count(a.counter, n);
 }
 
@@ -637,17 +719,29 @@ hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
   return hipGetLastError();
 }
 
-hipError_t launch_pm_init(const PmArgs& a, hipStream_t s) {
+hipError_t launch_pm_init(const PmArgs& a, bool sample, hipStream_t s) {
   const int P = 2 * a.hpz + 1;
-  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_init<P>, tile_grid((a.W + 15) / 16, (a.H + 15) / 16), dim3(256),
+  if (sample) {
+    PM_DISPATCH_P(P, hipLaunchKernelGGL((k_pm_init<P, true>),
+                                        tile_grid((a.W + 15) / 16, (a.H + 15) / 16),
+                                        dim3(256), 0, s, a));
+    return hipGetLastError();
+  }
+  PM_DISPATCH_P(P, hipLaunchKernelGGL((k_pm_init<P, false>), tile_grid((a.W + 15) / 16, (a.H + 15) / 16), dim3(256),
                                       0, s, a));
   return hipGetLastError();
 }
 
-hipError_t launch_pm_half(const PmArgs& a, int colour, int iter,
+hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
                           hipStream_t s) {
   const int P = 2 * a.hpz + 1;
-  PM_DISPATCH_P(P, hipLaunchKernelGGL(k_pm_half<P>,
+  if (sample) {
+    PM_DISPATCH_P(P, hipLaunchKernelGGL((k_pm_half<P, true>),
+                                        tile_grid(((a.W + 1) / 2 + 15) / 16, (a.H + 15) / 16),
+                                        dim3(256), 0, s, a, colour, iter));
+    return hipGetLastError();
+  }
+  PM_DISPATCH_P(P, hipLaunchKernelGGL((k_pm_half<P, false>),
                                       tile_grid(((a.W + 1) / 2 + 15) / 16, (a.H + 15) / 16), dim3(256), 0,
                                       s, a, colour, iter));
   return hipGetLastError();

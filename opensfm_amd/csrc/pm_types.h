@@ -38,6 +38,7 @@ struct PmArgs {
   float min_patch_var;
   uint32_t seed_lo, seed_hi;
   int n_far;           // far propagation probes per direction
+  int n_refine;        // random refinements per pixel update (6-step schedule)
   unsigned long long* counter;  // scorings executed (nullptr: not counted)
 };
 

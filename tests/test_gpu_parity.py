@@ -202,3 +202,32 @@ def test_plane_sweep_full_size_sampled_pixels():
         else:
             assert depth[i, j] == 0
 
+
+
+def test_patchmatch_sample_mode(cfg1_scene):
+    """ComputePatchMatchSample (depthmap.cc:215-227): one view per candidate.
+    Invariants, determinism, and accuracy/mean score at least the oracle's."""
+    print("PatchMatch sample mode config 1", flush=True)
+    gt = cfg1_scene.ref.depth
+    n = len(cfg1_scene.views)
+    o_acc, g_acc, o_sc, g_sc = [], [], [], []
+    for seed in (3, 4):
+        g = _setup(DepthmapEstimator(), cfg1_scene)
+        g.set_seed(seed)
+        d, p, s, nb = g.compute_patch_match_sample()
+        d2 = g.compute_patch_match_sample()[0]
+        assert np.array_equal(d, d2)
+        live = p[..., 2] != 0
+        assert np.all((nb[live] >= 1) & (nb[live] < n))
+        assert np.all((s >= -1) & (s <= 1 + 1e-5))
+        g_acc.append(scenes.depth_accuracy(d, gt, 2.0, 20.0))
+        g_sc.append(float(s[d > 0].mean()))
+        o = _setup(oracle.OracleEstimator(), cfg1_scene)
+        o.set_seed(seed)
+        od, _, osc, _ = o.compute_patch_match_sample()
+        o_acc.append(scenes.depth_accuracy(od, gt, 2.0, 20.0))
+        o_sc.append(float(osc[od > 0].mean()))
+    print(f"sample mode: oracle acc {np.mean(o_acc):.4f} score {np.mean(o_sc):.4f} | "
+          f"gpu acc {np.mean(g_acc):.4f} score {np.mean(g_sc):.4f}", flush=True)
+    assert np.mean(g_acc) >= np.mean(o_acc) - 0.01
+    assert np.mean(g_sc) >= np.mean(o_sc) - 0.01
