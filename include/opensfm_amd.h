@@ -96,6 +96,15 @@ int pm_estimator_set_far_probes(pm_estimator* est, int n);
  * depthmap.cc:337; the 0.02 * 0.3^k / 0.5 * 0.8^k schedule restarts every
  * 6 steps). */
 int pm_estimator_set_refinement_steps(pm_estimator* est, int n);
+/* Exploration samples per pixel update (0..8, default 3; 0 = the
+ * reference's candidate list, depthmap.cc:305-374).  Not in the reference:
+ * sample 0 and 1 are the planes through the 3-D points of the four
+ * neighbours at distance 1 and 3, further samples keep the current depth
+ * with a normal drawn within 87 degrees of the optical axis.  Scored after
+ * propagation and before the refinements, only at pixels whose best score
+ * is below 0.95.  They reach strongly slanted surfaces that the reference
+ * reaches by chaining refinements along its sequential sweep. */
+int pm_estimator_set_exploration(pm_estimator* est, int n);
 /* Run PostProcess (5x5 median, depthmap.cc:489-502) after PatchMatch
  * (default 1, as the reference always does; 0 returns the raw depth). */
 int pm_estimator_set_post_process(pm_estimator* est, int enable);

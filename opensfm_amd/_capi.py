@@ -57,6 +57,7 @@ SIGNATURES = {
     "pm_estimator_set_seed": (C.c_int, [_vp, C.c_uint64]),
     "pm_estimator_set_far_probes": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_refinement_steps": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_exploration": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_compute": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _ip]),
     "pm_estimator_run": (C.c_int, [_vp, C.c_int]),

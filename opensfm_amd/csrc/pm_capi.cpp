@@ -189,6 +189,7 @@ struct pm_estimator {
   uint64_t seed = 0x5EEDull;
   int n_far = 6;
   int n_refine = 6;
+  int n_explore = 3;
   int post_process = 1;
   std::vector<std::unique_ptr<HostView>> views;
   DevBuf mask0;
@@ -200,9 +201,12 @@ struct pm_estimator {
   pm_stats stats{};
   int wlut_patch = -1;
 
+  std::vector<hipEvent_t> half_ev;  // around the red/black launches
+
   ~pm_estimator() {
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
+    for (auto& e : half_ev) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -348,6 +352,7 @@ pm::PmArgs prepare(pm_estimator* e) {
   a.seed_hi = (uint32_t)(e->seed >> 32);
   a.n_far = e->n_far;
   a.n_refine = e->n_refine;
+  a.n_explore = e->n_explore;
   a.counter = e->counter.as<unsigned long long>();
   return a;
 }
@@ -406,17 +411,25 @@ int run(pm_estimator* e, int mode) {
     HIP_CHECK(pm::launch_pm_init(a, sample, e->stream));
     pm::PmArgs ah = a;  // the red/black launches count into slot 1
     ah.counter = a.counter + 1;
+    // one event between consecutive launches; no host wait inside the loop
+    const int nl = 2 * e->iterations;
+    while ((int)e->half_ev.size() < nl + 1) {
+      hipEvent_t ev;
+      HIP_CHECK(hipEventCreate(&ev));
+      e->half_ev.push_back(ev);
+    }
+    HIP_CHECK(hipEventRecord(e->half_ev[0], e->stream));
     for (int it = 0; it < e->iterations; ++it) {
       for (int colour = 0; colour < 2; ++colour) {
-        HIP_CHECK(hipEventRecord(e->ev[2], e->stream));
         HIP_CHECK(pm::launch_pm_half(ah, sample, colour, it, e->stream));
-        HIP_CHECK(hipEventRecord(e->ev[3], e->stream));
-        HIP_CHECK(hipEventSynchronize(e->ev[3]));
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, e->ev[2], e->ev[3]));
-        dom_ms += ms;
-        ++launches;
+        HIP_CHECK(hipEventRecord(e->half_ev[++launches], e->stream));
       }
+    }
+    HIP_CHECK(hipEventSynchronize(e->half_ev[nl]));
+    for (int l = 0; l < nl; ++l) {
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, e->half_ev[l], e->half_ev[l + 1]));
+      dom_ms += ms;
     }
   }
   const size_t np = (size_t)r.width * r.height;
@@ -600,6 +613,13 @@ int pm_estimator_set_refinement_steps(pm_estimator* e, int n) {
   if (!e) return fail(PM_ERR_INVALID, "null estimator");
   if (n < 0 || n > 48) return fail(PM_ERR_INVALID, "refinement steps must be in [0, 48]");
   e->n_refine = n;
+  return PM_OK;
+}
+
+int pm_estimator_set_exploration(pm_estimator* e, int n) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (n < 0 || n > 8) return fail(PM_ERR_INVALID, "exploration samples must be in [0, 8]");
+  e->n_explore = n;
   return PM_OK;
 }
 

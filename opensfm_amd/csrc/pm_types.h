@@ -39,6 +39,7 @@ struct PmArgs {
   uint32_t seed_lo, seed_hi;
   int n_far;           // far propagation probes per direction
   int n_refine;        // random refinements per pixel update (6-step schedule)
+  int n_explore;       // wide-cone normal samples per pixel update (0: none)
   unsigned long long* counter;  // scorings executed (nullptr: not counted)
 };
 

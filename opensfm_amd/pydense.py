@@ -88,6 +88,9 @@ class DepthmapEstimator:
     def set_refinement_steps(self, n: int) -> None:
         _capi.check(self._lib.pm_estimator_set_refinement_steps(self._h, int(n)))
 
+    def set_exploration(self, n: int) -> None:
+        _capi.check(self._lib.pm_estimator_set_exploration(self._h, int(n)))
+
     def set_post_process(self, enable: bool) -> None:
         _capi.check(self._lib.pm_estimator_set_post_process(self._h, int(bool(enable))))
 
