@@ -155,6 +155,76 @@ int pm_cleaner_clean(pm_cleaner* cl, float* clean_depth);
 /* HIP-event time of the last clean kernel, milliseconds. */
 int pm_cleaner_last_ms(pm_cleaner* cl, float* ms);
 
+/* ---- Batched estimation ---------------------------------------------- */
+
+/* Many reference views ("jobs") of one pool of views, run together: each
+ * launch walks the tiles of every job, and a whole run (state clears,
+ * random init, 2 x iterations red/black launches, output packing) is
+ * captured in one hipGraph that is replayed while the configuration stays
+ * the same.  Job k computes exactly what one estimator computes with
+ * add_view(ref), add_view(neighbours[0..n-1]), the same settings and
+ * set_seed(seed + k).  No counterpart in the reference, whose callers run
+ * one DepthmapEstimator per reference view in a thread pool
+ * (opensfm/dense.py:187-241). */
+typedef struct pm_batch pm_batch;
+
+/* lane layouts of the red/black launches (lanes across x down; each lane
+ * takes one pixel of the colour in a pair of columns) */
+#define PM_TILE_8x8 0
+#define PM_TILE_16x16 1
+#define PM_TILE_32x8 2
+
+int pm_batch_create(int device, pm_batch** out);
+int pm_batch_destroy(pm_batch* b);
+/* Adds a camera (world->camera K, R, t as in pm_estimator_add_view) and its
+ * image and mask to the pool; *view_id receives its index. */
+int pm_batch_add_view(pm_batch* b, const double* K, const double* R,
+                      const double* t, const uint8_t* image,
+                      const uint8_t* mask, int width, int height, int stride,
+                      int* view_id);
+/* Adds a job: reference pool view `ref_view` against `n_neighbours` pool
+ * views, depth range [min_depth, max_depth]; *job_id receives its index. */
+int pm_batch_add_job(pm_batch* b, int ref_view, const int* neighbours,
+                     int n_neighbours, double min_depth, double max_depth,
+                     int* job_id);
+/* Settings shared by every job (as the pm_estimator_set_* of the same
+ * name; defaults equal the estimator's). */
+int pm_batch_set_patchmatch_iterations(pm_batch* b, int n);
+int pm_batch_set_patch_size(pm_batch* b, int size);
+int pm_batch_set_min_patch_sd(pm_batch* b, float sd);
+int pm_batch_set_seed(pm_batch* b, uint64_t seed);
+int pm_batch_set_exploration(pm_batch* b, int n);
+int pm_batch_set_post_process(pm_batch* b, int enable);
+/* Lane layout of the red/black launches (default PM_TILE_16x16).  The maps
+ * do not depend on it. */
+int pm_batch_set_tile(pm_batch* b, int tile);
+/* 1 (default): run through the captured hipGraph; 0: launch eagerly. */
+int pm_batch_set_graph(pm_batch* b, int enable);
+/* Times init + one red/black iteration on the first jobs (about 8 Mpx) for
+ * every tile and every patch size in patch_sizes[0..n-1] (n = 0: the
+ * current size only) and keeps the fastest pair; *patch_out and *tile_out
+ * receive it.  Patch size changes the maps: list only sizes acceptable to
+ * the caller. */
+int pm_batch_autotune(pm_batch* b, int mode, const int* patch_sizes, int n,
+                      int* patch_out, int* tile_out);
+/* Runs every job (PM_MODE_PATCHMATCH or PM_MODE_PATCHMATCH_SAMPLE); the
+ * maps stay in HBM. */
+int pm_batch_run(pm_batch* b, int mode);
+/* Maps of job `job` of the last run, layouts as pm_estimator_compute. */
+int pm_batch_download(pm_batch* b, int job, float* depth, float* plane,
+                      float* score, int32_t* nghbr);
+/* DepthmapCleaner::Clean (depthmap.cc:524-555) of every job's depth against
+ * the depths of its neighbour views that are themselves the reference view
+ * of a job, on the device. */
+int pm_batch_clean(pm_batch* b, float same_depth_threshold,
+                   int min_consistent_views);
+int pm_batch_download_clean(pm_batch* b, int job, float* depth);
+/* Last run: scorings, launches = red/black launches, total_ms = dominant_ms
+ * = the whole run on the device. */
+int pm_batch_stats(pm_batch* b, pm_stats* out);
+/* Device time of the last pm_batch_clean, milliseconds. */
+int pm_batch_clean_ms(pm_batch* b, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,7 @@ pydense.DepthmapCleaner, opensfm/src/lib/dense/pydense.pyi:28-60), computed
 by hand-written HIP kernels for gfx950 behind the C ABI in
 include/opensfm_amd.h.
 """
+from .batch import DepthmapBatch  # noqa: F401
 from .pydense import DepthmapCleaner, DepthmapEstimator  # noqa: F401
 
-__all__ = ["DepthmapEstimator", "DepthmapCleaner"]
+__all__ = ["DepthmapEstimator", "DepthmapCleaner", "DepthmapBatch"]

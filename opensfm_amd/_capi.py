@@ -23,6 +23,11 @@ PM_MODE_PLANE_SWEEP = 0
 PM_MODE_PATCHMATCH = 1
 PM_MODE_PATCHMATCH_SAMPLE = 2
 
+PM_TILE_8x8 = 0
+PM_TILE_16x16 = 1
+PM_TILE_32x8 = 2
+TILE_NAMES = {PM_TILE_8x8: "8x8", PM_TILE_16x16: "16x16", PM_TILE_32x8: "32x8"}
+
 
 class pm_stats(C.Structure):
     _fields_ = [
@@ -71,6 +76,27 @@ SIGNATURES = {
     "pm_cleaner_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _fp, C.c_int, C.c_int]),
     "pm_cleaner_clean": (C.c_int, [_vp, _fp]),
     "pm_cleaner_last_ms": (C.c_int, [_vp, _fp]),
+    "pm_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "pm_batch_destroy": (C.c_int, [_vp]),
+    "pm_batch_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _u8p, _u8p, C.c_int, C.c_int,
+                                    C.c_int, _ip]),
+    "pm_batch_add_job": (C.c_int, [_vp, C.c_int, _ip, C.c_int, C.c_double, C.c_double,
+                                   _ip]),
+    "pm_batch_set_patchmatch_iterations": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_patch_size": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_min_patch_sd": (C.c_int, [_vp, C.c_float]),
+    "pm_batch_set_seed": (C.c_int, [_vp, C.c_uint64]),
+    "pm_batch_set_exploration": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_post_process": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_tile": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_graph": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _ip, _ip]),
+    "pm_batch_run": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_download": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _ip]),
+    "pm_batch_clean": (C.c_int, [_vp, C.c_float, C.c_int]),
+    "pm_batch_download_clean": (C.c_int, [_vp, C.c_int, _fp]),
+    "pm_batch_stats": (C.c_int, [_vp, C.POINTER(pm_stats)]),
+    "pm_batch_clean_ms": (C.c_int, [_vp, _fp]),
 }
 
 _lock = threading.Lock()

@@ -1,0 +1,673 @@
+// Host side of the batched C ABI (pm_batch_* in include/opensfm_amd.h).
+//
+// A batch holds a pool of views (camera + image, uploaded once) and a list
+// of jobs, each a reference view with its neighbour views and depth range:
+// job k computes what one DepthmapEstimator with add_view(ref),
+// add_view(neighbours...) and set_seed(seed + k) computes (depthmap.cc:
+// 146-227), bit for bit.  All jobs run together: per-pixel state lives in
+// flat device buffers at per-job offsets, and each launch walks the tiles of
+// every job (PmJob table, pm_types.h).  A run -- clears, random init,
+// 2 x iterations red/black launches, output packing -- is captured once in a
+// hipGraph and replayed while the configuration is unchanged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "pm_host.h"
+#include "pm_kernels.h"
+#include "pm_types.h"
+
+using namespace pmhost;
+
+namespace {
+
+struct PoolView {
+  double K[9], R[9], t[3], Kinv[9];
+  int width = 0, height = 0;
+  DevBuf img, mask, quad;
+};
+
+struct BatchJob {
+  int ref = 0;
+  std::vector<int> nb;
+  double min_depth = 0, max_depth = 0;
+  size_t off = 0;  // first pixel in the flat per-pixel buffers
+};
+
+}  // namespace
+
+struct pm_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // DepthmapEstimator defaults (depthmap.cc:134-144) plus the engine's own
+  int patch_size = 7;
+  int iterations = 3;
+  float min_patch_variance = 5 * 5;
+  uint64_t seed = 0x5EEDull;
+  int n_far = 6, n_refine = 6, n_explore = 3;
+  int post_process = 1;
+  int tile = 1;
+  int use_graph = 1;
+  std::vector<std::unique_ptr<PoolView>> views;
+  std::vector<BatchJob> jobs;
+  // device state of the prepared configuration
+  DevBuf viewtab, jobtab, wlut, state, score, nghbr;
+  DevBuf out_depth, out_plane, out_score, out_nghbr, queue, counter;
+  DevBuf cl_views, cl_jobs, cl_out;
+  int wlut_patch = -1;
+  int prepared_jobs = -1;  // jobs in the uploaded table (-1: stale)
+  size_t total_px = 0;
+  int half_total = 0, pix_total = 0;
+  hipGraphExec_t exec = nullptr;
+  int graph_mode = -1;  // mode the graph was captured for (-1: none)
+  bool have_result = false, have_clean = false;
+  pm_stats stats{};
+  float clean_ms = 0;
+
+  void invalidate() {
+    prepared_jobs = -1;
+    drop_graph();
+    have_result = have_clean = false;
+  }
+  void drop_graph() {
+    if (exec) hipGraphExecDestroy(exec);
+    exec = nullptr;
+    graph_mode = -1;
+  }
+  ~pm_batch() {
+    drop_graph();
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int validate(pm_batch* b) {
+  if (b->jobs.empty()) return fail(PM_ERR_INVALID, "batch has no jobs");
+  const int hpz = (b->patch_size - 1) / 2;
+  for (size_t k = 0; k < b->jobs.size(); ++k) {
+    const PoolView& r = *b->views[b->jobs[k].ref];
+    if (r.width < 2 * hpz + 1 || r.height < 2 * hpz + 1)
+      return fail(PM_ERR_INVALID, "job " + std::to_string(k) +
+                                      ": reference view smaller than one patch");
+  }
+  return PM_OK;
+}
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Uploads the view and job tables for the first `count` jobs and sizes the
+// per-pixel buffers.  Host-synchronous.
+void prepare(pm_batch* b, int count) {
+  const int hpz = (b->patch_size - 1) / 2;
+  if (b->wlut_patch != b->patch_size) {
+    std::vector<float> t = bilateral_table(hpz);
+    b->wlut.ensure(t.size() * sizeof(float));
+    HIP_CHECK(hipMemcpy(b->wlut.p, t.data(), t.size() * sizeof(float),
+                        hipMemcpyHostToDevice));
+    b->wlut_patch = b->patch_size;
+  }
+  int tx, ty;
+  pm::tile_shape(b->tile, &tx, &ty);
+  std::vector<pm::PmView> vt;
+  std::vector<int> view0(count);
+  size_t px = 0;
+  for (int k = 0; k < count; ++k) {
+    BatchJob& J = b->jobs[k];
+    const PoolView& R = *b->views[J.ref];
+    J.off = px;
+    px += (size_t)R.width * R.height;
+    view0[k] = (int)vt.size();
+    // the reference itself first, with its pose relative to itself as
+    // AddView computes it for view 0
+    double Q[9], a[3];
+    relative_pose(R.R, R.t, R.R, R.t, Q, a);
+    vt.emplace_back();
+    fill_view(vt.back(), R.quad.as<uint32_t>(), R.width, R.height, R.K, Q, a,
+              R.Kinv);
+    for (int v : J.nb) {
+      const PoolView& V = *b->views[v];
+      relative_pose(R.R, R.t, V.R, V.t, Q, a);
+      vt.emplace_back();
+      fill_view(vt.back(), V.quad.as<uint32_t>(), V.width, V.height, V.K, Q,
+                a, R.Kinv);
+    }
+  }
+  b->total_px = px;
+  b->viewtab.ensure(vt.size() * sizeof(pm::PmView));
+  HIP_CHECK(hipMemcpy(b->viewtab.p, vt.data(), vt.size() * sizeof(pm::PmView),
+                      hipMemcpyHostToDevice));
+  b->state.ensure(px * 4 * sizeof(float));
+  b->score.ensure(px * sizeof(float));
+  b->nghbr.ensure(px * sizeof(int));
+  b->out_depth.ensure(px * sizeof(float));
+  b->out_plane.ensure(px * 3 * sizeof(float));
+  b->out_score.ensure(px * sizeof(float));
+  b->out_nghbr.ensure(px * sizeof(int));
+  b->queue.ensure((2 * (size_t)std::max(b->iterations, 1) + 1) * sizeof(unsigned));
+  b->counter.ensure(2 * sizeof(unsigned long long));
+
+  std::vector<pm::PmJob> jt(count);
+  int half_t = 0, pix_t = 0;
+  for (int k = 0; k < count; ++k) {
+    const BatchJob& J = b->jobs[k];
+    const PoolView& R = *b->views[J.ref];
+    pm::PmJob& T = jt[k];
+    memset(&T, 0, sizeof(T));
+    pm::PmArgs& a = T.a;
+    a.W = R.width;
+    a.H = R.height;
+    a.nviews = 1 + (int)J.nb.size();
+    a.hpz = hpz;
+    a.views = b->viewtab.as<pm::PmView>() + view0[k];
+    a.ref_img = R.img.as<uint8_t>();
+    a.ref_mask = R.mask.as<uint8_t>();
+    a.wlut = b->wlut.as<float>();
+    a.state = b->state.as<float>() + 4 * J.off;
+    a.score = b->score.as<float>() + J.off;
+    a.nghbr = b->nghbr.as<int>() + J.off;
+    for (int m = 0; m < 9; ++m) {
+      a.Kinv0f[m] = (float)R.Kinv[m];
+      a.Kinv0[m] = R.Kinv[m];
+    }
+    a.log_min = (float)log(J.min_depth);
+    a.log_max = (float)log(J.max_depth);
+    a.min_patch_var = b->min_patch_variance;
+    const uint64_t s = b->seed + (uint64_t)k;
+    a.seed_lo = (uint32_t)(s & 0xffffffffu);
+    a.seed_hi = (uint32_t)(s >> 32);
+    a.n_far = b->n_far;
+    a.n_refine = b->n_refine;
+    a.n_explore = b->n_explore;
+    a.counter = nullptr;
+    T.half_tx = ceil_div((R.width + 1) / 2, tx);
+    T.half_ty = ceil_div(R.height, ty);
+    T.half_t0 = half_t;
+    half_t += T.half_tx * T.half_ty;
+    T.pix_tx = ceil_div(R.width, 16);
+    T.pix_ty = ceil_div(R.height, 16);
+    T.pix_t0 = pix_t;
+    pix_t += T.pix_tx * T.pix_ty;
+    T.out_depth = b->out_depth.as<float>() + J.off;
+    T.out_plane = b->out_plane.as<float>() + 3 * J.off;
+    T.out_score = b->out_score.as<float>() + J.off;
+    T.out_nghbr = b->out_nghbr.as<int>() + J.off;
+  }
+  b->half_total = half_t;
+  b->pix_total = pix_t;
+  b->jobtab.ensure(jt.size() * sizeof(pm::PmJob));
+  HIP_CHECK(hipMemcpy(b->jobtab.p, jt.data(), jt.size() * sizeof(pm::PmJob),
+                      hipMemcpyHostToDevice));
+  b->prepared_jobs = count;
+}
+
+// Enqueues one whole run on b->stream.  Only kernel launches (no memsets or
+// host work), so that it can be captured in a graph.  half_ev, if given,
+// receives 2 * iterations + 1 events around the red/black launches.
+void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
+  const bool sample = mode == PM_MODE_PATCHMATCH_SAMPLE;
+  const int nj = b->prepared_jobs;
+  const pm::PmJob* jt = b->jobtab.as<pm::PmJob>();
+  unsigned* q = b->queue.as<unsigned>();
+  unsigned long long* cnt = b->counter.as<unsigned long long>();
+  const int nq = 2 * b->iterations + 1;
+  HIP_CHECK(pm::launch_fill_u32(q, nq, 0u, b->stream));
+  HIP_CHECK(pm::launch_fill_u32(reinterpret_cast<unsigned*>(cnt), 4, 0u, b->stream));
+  // every pixel starts at the zero state (border, masked and low-variance
+  // pixels keep it)
+  HIP_CHECK(pm::launch_fill_u32(b->state.as<unsigned>(), (int)(4 * b->total_px), 0u, b->stream));
+  HIP_CHECK(pm::launch_fill_u32(b->score.as<unsigned>(), (int)b->total_px, 0u, b->stream));
+  HIP_CHECK(pm::launch_fill_u32(b->nghbr.as<unsigned>(), (int)b->total_px, 0u, b->stream));
+  HIP_CHECK(pm::launch_pm_init_batch(jt, nj, b->pix_total, b->patch_size, sample,
+                                     q, cnt, b->stream));
+  if (half_ev) HIP_CHECK(hipEventRecord(half_ev[0], b->stream));
+  int l = 0;
+  for (int it = 0; it < b->iterations; ++it) {
+    for (int colour = 0; colour < 2; ++colour) {
+      HIP_CHECK(pm::launch_pm_half_batch(jt, nj, b->half_total, b->patch_size,
+                                         sample, b->tile, colour, it, q + 1 + l,
+                                         cnt + 1, b->stream));
+      ++l;
+      if (half_ev) HIP_CHECK(hipEventRecord(half_ev[l], b->stream));
+    }
+  }
+  HIP_CHECK(pm::launch_pack_batch(jt, nj, b->pix_total, b->post_process, b->stream));
+}
+
+int run(pm_batch* b, int mode) {
+  if (mode != PM_MODE_PATCHMATCH && mode != PM_MODE_PATCHMATCH_SAMPLE)
+    return fail(PM_ERR_INVALID, "batched runs support PM_MODE_PATCHMATCH and "
+                                "PM_MODE_PATCHMATCH_SAMPLE");
+  int rc = validate(b);
+  if (rc) return rc;
+  DeviceGuard dg(b->device);
+  if (b->prepared_jobs != (int)b->jobs.size()) {
+    b->drop_graph();
+    prepare(b, (int)b->jobs.size());
+  }
+  if ((size_t)4 * b->total_px > 0x7fffffffu)
+    return fail(PM_ERR_INVALID, "batch too large: at most 536870911 reference pixels");
+  b->have_result = b->have_clean = false;
+  HIP_CHECK(hipEventRecord(b->ev[0], b->stream));
+  if (b->use_graph) {
+    if (!b->exec || b->graph_mode != mode) {
+      b->drop_graph();
+      // one eager run first: it is the result of this call, and it resolves
+      // the persistent grid sizes outside the capture
+      enqueue(b, mode, nullptr);
+      hipGraph_t g = nullptr;
+      HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(b, mode, nullptr);
+      } catch (...) {
+        hipStreamEndCapture(b->stream, &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+      }
+      HIP_CHECK(hipStreamEndCapture(b->stream, &g));
+      hipError_t e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      HIP_CHECK(e);
+      b->graph_mode = mode;
+    } else {
+      HIP_CHECK(hipGraphLaunch(b->exec, b->stream));
+    }
+  } else {
+    enqueue(b, mode, nullptr);
+  }
+  HIP_CHECK(hipEventRecord(b->ev[1], b->stream));
+  HIP_CHECK(hipEventSynchronize(b->ev[1]));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  unsigned long long cnt[2] = {0, 0};
+  HIP_CHECK(hipMemcpy(cnt, b->counter.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  b->stats = pm_stats{};
+  b->stats.scorings = (int64_t)(cnt[0] + cnt[1]);
+  b->stats.dominant_scorings = (int64_t)cnt[1];
+  b->stats.patch_size = b->patch_size;
+  b->stats.width = 0;
+  b->stats.height = 0;
+  b->stats.launches = 2 * b->iterations;
+  b->stats.dominant_ms = ms;
+  b->stats.total_ms = ms;
+  b->have_result = true;
+  return PM_OK;
+}
+
+// Times init + one red/black iteration over the first jobs (up to about
+// 8 Mpx) for one (patch, tile) candidate; milliseconds per Mpx of the red/
+// black launches.
+float probe(pm_batch* b, int mode) {
+  size_t px = 0;
+  int count = 0;
+  while (count < (int)b->jobs.size() && (count == 0 || px < (size_t)8 << 20)) {
+    const PoolView& R = *b->views[b->jobs[count].ref];
+    px += (size_t)R.width * R.height;
+    ++count;
+  }
+  const int iters = b->iterations;
+  b->iterations = 1;
+  std::vector<hipEvent_t> ev(3, nullptr);
+  float ms = 0;
+  try {
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    prepare(b, count);
+    enqueue(b, mode, ev.data());
+    HIP_CHECK(hipEventSynchronize(ev[2]));
+    HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[2]));
+  } catch (...) {
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    b->iterations = iters;
+    b->prepared_jobs = -1;
+    throw;
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  b->iterations = iters;
+  b->prepared_jobs = -1;
+  return ms / (float)(px / 1e6);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pm_batch_create(int device, pm_batch** out) {
+  if (!out) return fail(PM_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  int rc = check_device(device);
+  if (rc) return rc;
+  return guarded([&] {
+    std::unique_ptr<pm_batch> b(new pm_batch());
+    b->device = device;
+    DeviceGuard dg(device);
+    HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (auto& e : b->ev) HIP_CHECK(hipEventCreate(&e));
+    *out = b.release();
+    return PM_OK;
+  });
+}
+
+int pm_batch_destroy(pm_batch* b) {
+  if (!b) return PM_OK;
+  return guarded([&] {
+    DeviceGuard dg(b->device);
+    delete b;
+    return PM_OK;
+  });
+}
+
+int pm_batch_add_view(pm_batch* b, const double* K, const double* R,
+                      const double* t, const uint8_t* image, const uint8_t* mask,
+                      int width, int height, int stride, int* view_id) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (!K || !R || !t || !image || !mask)
+    return fail(PM_ERR_INVALID, "null K, R, t, image or mask");
+  if (width < 2 || height < 2) return fail(PM_ERR_INVALID, "image must be at least 2x2");
+  if (stride < width) return fail(PM_ERR_INVALID, "stride smaller than width");
+  if (!all_finite(K, 9) || !all_finite(R, 9) || !all_finite(t, 3))
+    return fail(PM_ERR_INVALID, "K, R and t must be finite");
+  return guarded([&] {
+    std::unique_ptr<PoolView> v(new PoolView());
+    memcpy(v->K, K, sizeof(v->K));
+    memcpy(v->R, R, sizeof(v->R));
+    memcpy(v->t, t, sizeof(v->t));
+    if (!inv3(v->K, v->Kinv)) return fail(PM_ERR_INVALID, "K is singular");
+    v->width = width;
+    v->height = height;
+    DeviceGuard dg(b->device);
+    const size_t np = (size_t)width * height;
+    v->img.ensure(np);
+    v->mask.ensure(np);
+    HIP_CHECK(hipMemcpy2DAsync(v->img.p, width, image, stride, width, height,
+                               hipMemcpyHostToDevice, b->stream));
+    HIP_CHECK(hipMemcpy2DAsync(v->mask.p, width, mask, stride, width, height,
+                               hipMemcpyHostToDevice, b->stream));
+    v->quad.ensure((size_t)(width + 1) * (height + 1) * sizeof(uint32_t));
+    HIP_CHECK(pm::launch_build_quad(v->img.as<uint8_t>(), v->quad.as<uint32_t>(),
+                                    width, height, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (view_id) *view_id = (int)b->views.size();
+    b->views.push_back(std::move(v));
+    return PM_OK;
+  });
+}
+
+int pm_batch_add_job(pm_batch* b, int ref_view, const int* neighbours,
+                     int n_neighbours, double min_depth, double max_depth,
+                     int* job_id) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  const int nv = (int)b->views.size();
+  if (ref_view < 0 || ref_view >= nv)
+    return fail(PM_ERR_RANGE, "reference view " + std::to_string(ref_view) +
+                                  " out of range [0, " + std::to_string(nv) + ")");
+  if (n_neighbours < 1 || !neighbours)
+    return fail(PM_ERR_INVALID, "a job needs at least one neighbour view");
+  if (n_neighbours > pm::kMaxViews - 1)
+    return fail(PM_ERR_INVALID, "at most " + std::to_string(pm::kMaxViews - 1) +
+                                    " neighbour views per job");
+  for (int k = 0; k < n_neighbours; ++k)
+    if (neighbours[k] < 0 || neighbours[k] >= nv)
+      return fail(PM_ERR_RANGE, "neighbour view " + std::to_string(neighbours[k]) +
+                                    " out of range [0, " + std::to_string(nv) + ")");
+  if (!(min_depth > 0) || !(max_depth > min_depth) || !std::isfinite(max_depth))
+    return fail(PM_ERR_INVALID, "depth range must satisfy 0 < min_depth < max_depth");
+  BatchJob J;
+  J.ref = ref_view;
+  J.nb.assign(neighbours, neighbours + n_neighbours);
+  J.min_depth = min_depth;
+  J.max_depth = max_depth;
+  if (job_id) *job_id = (int)b->jobs.size();
+  b->jobs.push_back(std::move(J));
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_patchmatch_iterations(pm_batch* b, int n) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (n < 0 || n > 1000) return fail(PM_ERR_INVALID, "iterations must be in [0, 1000]");
+  b->iterations = n;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_patch_size(pm_batch* b, int size) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (size < 3 || size > 11 || size % 2 == 0)
+    return fail(PM_ERR_INVALID, "patch size must be odd and in [3, 11]");
+  b->patch_size = size;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_min_patch_sd(pm_batch* b, float sd) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  b->min_patch_variance = sd * sd;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_seed(pm_batch* b, uint64_t seed) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  b->seed = seed;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_exploration(pm_batch* b, int n) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (n < 0 || n > 8) return fail(PM_ERR_INVALID, "exploration samples must be in [0, 8]");
+  b->n_explore = n;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_post_process(pm_batch* b, int enable) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  b->post_process = enable ? 1 : 0;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_tile(pm_batch* b, int tile) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (tile < PM_TILE_8x8 || tile > PM_TILE_32x8)
+    return fail(PM_ERR_INVALID, "unknown tile " + std::to_string(tile));
+  b->tile = tile;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_graph(pm_batch* b, int enable) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  b->use_graph = enable ? 1 : 0;
+  b->drop_graph();
+  return PM_OK;
+}
+
+int pm_batch_autotune(pm_batch* b, int mode, const int* patch_sizes, int n,
+                      int* patch_out, int* tile_out) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (mode != PM_MODE_PATCHMATCH && mode != PM_MODE_PATCHMATCH_SAMPLE)
+    return fail(PM_ERR_INVALID, "autotune supports PM_MODE_PATCHMATCH and "
+                                "PM_MODE_PATCHMATCH_SAMPLE");
+  std::vector<int> patches;
+  if (n > 0 && !patch_sizes) return fail(PM_ERR_INVALID, "null patch size list");
+  for (int k = 0; k < n; ++k) {
+    const int p = patch_sizes[k];
+    if (p < 3 || p > 11 || p % 2 == 0)
+      return fail(PM_ERR_INVALID, "patch sizes must be odd and in [3, 11]");
+    patches.push_back(p);
+  }
+  if (patches.empty()) patches.push_back(b->patch_size);
+  int rc = validate(b);
+  if (rc) return rc;
+  return guarded([&] {
+    DeviceGuard dg(b->device);
+    b->invalidate();
+    const int keep_patch = b->patch_size, keep_tile = b->tile;
+    float best = 0;
+    int best_p = keep_patch, best_t = keep_tile;
+    bool first = true;
+    try {
+      for (int p : patches) {
+        for (int t = PM_TILE_8x8; t <= PM_TILE_32x8; ++t) {
+          b->patch_size = p;
+          b->tile = t;
+          // smaller views than the patch cannot run at this size
+          if (validate(b)) continue;
+          const float ms = probe(b, mode);
+          if (first || ms < best) {
+            best = ms;
+            best_p = p;
+            best_t = t;
+            first = false;
+          }
+        }
+      }
+    } catch (...) {
+      b->patch_size = keep_patch;
+      b->tile = keep_tile;
+      throw;
+    }
+    if (first) return fail(PM_ERR_INVALID, "no candidate patch size fits every job");
+    b->patch_size = best_p;
+    b->tile = best_t;
+    b->invalidate();
+    if (patch_out) *patch_out = best_p;
+    if (tile_out) *tile_out = best_t;
+    return PM_OK;
+  });
+}
+
+int pm_batch_run(pm_batch* b, int mode) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  return guarded([&] { return run(b, mode); });
+}
+
+int pm_batch_download(pm_batch* b, int job, float* depth, float* plane,
+                      float* score, int32_t* nghbr) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (!b->have_result) return fail(PM_ERR_INVALID, "no computed maps to download");
+  if (job < 0 || job >= (int)b->jobs.size())
+    return fail(PM_ERR_RANGE, "job " + std::to_string(job) + " out of range");
+  if (!depth || !plane || !score || !nghbr) return fail(PM_ERR_INVALID, "null output buffer");
+  return guarded([&] {
+    DeviceGuard dg(b->device);
+    const BatchJob& J = b->jobs[job];
+    const PoolView& R = *b->views[J.ref];
+    const size_t np = (size_t)R.width * R.height;
+    HIP_CHECK(hipMemcpyAsync(depth, b->out_depth.as<float>() + J.off, np * sizeof(float),
+                             hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipMemcpyAsync(plane, b->out_plane.as<float>() + 3 * J.off,
+                             3 * np * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipMemcpyAsync(score, b->out_score.as<float>() + J.off, np * sizeof(float),
+                             hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipMemcpyAsync(nghbr, b->out_nghbr.as<int>() + J.off, np * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    return PM_OK;
+  });
+}
+
+// DepthmapCleaner::Clean (depthmap.cc:524-555) of every job's depthmap
+// against the depthmaps of its neighbour views that are themselves the
+// reference view of a job (their packed, post-processed depth).
+int pm_batch_clean(pm_batch* b, float same_depth_threshold, int min_consistent_views) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (!b->have_result) return fail(PM_ERR_INVALID, "run the batch before cleaning");
+  return guarded([&] {
+    DeviceGuard dg(b->device);
+    const int nj = (int)b->jobs.size();
+    std::vector<int> job_of(b->views.size(), -1);
+    for (int k = 0; k < nj; ++k)
+      if (job_of[b->jobs[k].ref] < 0) job_of[b->jobs[k].ref] = k;
+    std::vector<pm::ClView> cv;
+    std::vector<pm::ClJob> cj(nj);
+    int pix_t = 0;
+    for (int k = 0; k < nj; ++k) {
+      const BatchJob& J = b->jobs[k];
+      pm::ClJob& T = cj[k];
+      memset(&T, 0, sizeof(T));
+      T.view0 = (int)cv.size();
+      auto push = [&](int view, int job) {
+        const PoolView& V = *b->views[view];
+        pm::ClView c;
+        memset(&c, 0, sizeof(c));
+        c.depth = b->out_depth.as<float>() + b->jobs[job].off;
+        c.width = V.width;
+        c.height = V.height;
+        memcpy(c.K, V.K, sizeof(c.K));
+        memcpy(c.R, V.R, sizeof(c.R));
+        memcpy(c.t, V.t, sizeof(c.t));
+        cv.push_back(c);
+      };
+      push(J.ref, k);
+      for (int v : J.nb)
+        if (job_of[v] >= 0) push(v, job_of[v]);
+      T.nviews = (int)cv.size() - T.view0;
+      const PoolView& R = *b->views[J.ref];
+      memcpy(T.ref.Kinv, R.Kinv, sizeof(T.ref.Kinv));
+      mat3_transpose(R.R, T.ref.Rt);
+      T.pix_tx = ceil_div(R.width, 16);
+      T.pix_t0 = pix_t;
+      pix_t += T.pix_tx * ceil_div(R.height, 16);
+      T.out = nullptr;  // set below once cl_out is sized
+    }
+    b->cl_out.ensure(b->total_px * sizeof(float));
+    for (int k = 0; k < nj; ++k) cj[k].out = b->cl_out.as<float>() + b->jobs[k].off;
+    b->cl_views.ensure(cv.size() * sizeof(pm::ClView));
+    b->cl_jobs.ensure(cj.size() * sizeof(pm::ClJob));
+    HIP_CHECK(hipMemcpyAsync(b->cl_views.p, cv.data(), cv.size() * sizeof(pm::ClView),
+                             hipMemcpyHostToDevice, b->stream));
+    HIP_CHECK(hipMemcpyAsync(b->cl_jobs.p, cj.data(), cj.size() * sizeof(pm::ClJob),
+                             hipMemcpyHostToDevice, b->stream));
+    HIP_CHECK(hipEventRecord(b->ev[2], b->stream));
+    HIP_CHECK(pm::launch_clean_batch(b->cl_jobs.as<pm::ClJob>(), nj, pix_t,
+                                     b->cl_views.as<pm::ClView>(), same_depth_threshold,
+                                     min_consistent_views, b->stream));
+    HIP_CHECK(hipEventRecord(b->ev[3], b->stream));
+    HIP_CHECK(hipEventSynchronize(b->ev[3]));
+    HIP_CHECK(hipEventElapsedTime(&b->clean_ms, b->ev[2], b->ev[3]));
+    b->have_clean = true;
+    return PM_OK;
+  });
+}
+
+int pm_batch_download_clean(pm_batch* b, int job, float* depth) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (!b->have_clean) return fail(PM_ERR_INVALID, "no cleaned maps to download");
+  if (job < 0 || job >= (int)b->jobs.size())
+    return fail(PM_ERR_RANGE, "job " + std::to_string(job) + " out of range");
+  if (!depth) return fail(PM_ERR_INVALID, "null output buffer");
+  return guarded([&] {
+    DeviceGuard dg(b->device);
+    const BatchJob& J = b->jobs[job];
+    const PoolView& R = *b->views[J.ref];
+    HIP_CHECK(hipMemcpyAsync(depth, b->cl_out.as<float>() + J.off,
+                             (size_t)R.width * R.height * sizeof(float),
+                             hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    return PM_OK;
+  });
+}
+
+int pm_batch_stats(pm_batch* b, pm_stats* out) {
+  if (!b || !out) return fail(PM_ERR_INVALID, "null argument");
+  *out = b->stats;
+  return PM_OK;
+}
+
+int pm_batch_clean_ms(pm_batch* b, float* ms) {
+  if (!b || !ms) return fail(PM_ERR_INVALID, "null argument");
+  *ms = b->clean_ms;
+  return PM_OK;
+}
+
+}  // extern "C"

@@ -18,6 +18,22 @@ hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
                           hipStream_t s);
 hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
                        float* score, int* nghbr, hipStream_t s);
+// Batched launches over the jobs of a PmJob table (see pm_types.h).  `tile`
+// selects the red/black lane layout: 0 = 8x8, 1 = 16x16, 2 = 32x8 lanes.
+hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s);
+void tile_shape(int tile, int* tx, int* ty);
+hipError_t launch_pm_init_batch(const PmJob* jobs, int njobs, int total,
+                                int patch, bool sample, unsigned* queue,
+                                unsigned long long* count, hipStream_t s);
+hipError_t launch_pm_half_batch(const PmJob* jobs, int njobs, int total,
+                                int patch, bool sample, int tile, int colour,
+                                int iter, unsigned* queue,
+                                unsigned long long* count, hipStream_t s);
+hipError_t launch_pack_batch(const PmJob* jobs, int njobs, int total,
+                             int median, hipStream_t s);
+hipError_t launch_clean_batch(const ClJob* jobs, int njobs, int total,
+                              const ClView* views, float thr, int min_views,
+                              hipStream_t s);
 hipError_t launch_clean(const ClView* views, const ClRef* ref, int n, int W,
                         int H, float thr, int min_views, float* out,
                         hipStream_t s);

@@ -8,6 +8,11 @@
 // the whole kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "pm_device.h"
 #include "pm_kernels.h"
 
@@ -315,12 +320,11 @@ __device__ __forceinline__ bool is_interior(const PmArgs& a, int i, int j) {
 // ComputeIgnoreMask, depthmap.cc:237-281).  Masked and low-variance pixels
 // keep the zero state (the reference scores them and then zeroes them).
 template <int P, bool kSample>
-__device__ __forceinline__ unsigned init_pixel(const PmArgs& a) {
+__device__ __forceinline__ unsigned init_pixel(const PmArgs& a, int tile_x,
+                                               int tile_y) {
   constexpr int hpz = (P - 1) / 2;
-  const Tile t = row_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
-  if (!t.ok) return 0;
-  const int j = t.x * 16 + (threadIdx.x & 15);
-  const int i = t.y * 16 + (threadIdx.x >> 4);
+  const int j = tile_x * 16 + (threadIdx.x & 15);
+  const int i = tile_y * 16 + (threadIdx.x >> 4);
   if (!is_interior(a, i, j)) return 0;
   const size_t idx = (size_t)i * a.W + j;
   if (a.ref_mask[idx] == 0) return 0;
@@ -372,7 +376,8 @@ __device__ __forceinline__ unsigned init_pixel(const PmArgs& a) {
 
 template <int P, bool kSample>
 __global__ __launch_bounds__(256) void k_pm_init(PmArgs a) {
-  const unsigned n = init_pixel<P, kSample>(a);
+  const Tile t = row_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
+  const unsigned n = t.ok ? init_pixel<P, kSample>(a, t.x, t.y) : 0u;
   if (a.counter) wave_count(a.counter, n);
 }
 
@@ -397,13 +402,15 @@ __global__ __launch_bounds__(256) void k_pm_init(PmArgs a) {
 
 constexpr float kExploreBelow = 0.95f;
 
-template <int P, bool kSample>
+// Tile (tile_x, tile_y) of TX x TY lanes covers columns [2 TX tile_x,
+// 2 TX (tile_x + 1)) and rows [TY tile_y, TY (tile_y + 1)); each lane takes
+// the pixel of the given colour in its pair of columns.
+template <int P, bool kSample, int TX, int TY>
 __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
-                                               int iter) {
-  const Tile t = row_tile(((a.W + 1) / 2 + 15) >> 4, (a.H + 15) >> 4);
-  if (!t.ok) return 0;
-  const int cx = t.x * 16 + (threadIdx.x & 15);
-  const int i = t.y * 16 + (threadIdx.x >> 4);
+                                               int iter, int tile_x,
+                                               int tile_y) {
+  const int cx = tile_x * TX + (int)(threadIdx.x % TX);
+  const int i = tile_y * TY + (int)(threadIdx.x / TX);
   const int j = 2 * cx + ((i + colour) & 1);
   if (!is_interior(a, i, j)) return 0;
   const size_t idx = (size_t)i * a.W + j;
@@ -614,7 +621,9 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
 template <int P, bool kSample>
 __global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour,
                                                  int iter) {
-  const unsigned n = half_pixel<P, kSample>(a, colour, iter);
+  const Tile t = row_tile(((a.W + 1) / 2 + 15) >> 4, (a.H + 15) >> 4);
+  const unsigned n =
+      t.ok ? half_pixel<P, kSample, 16, 16>(a, colour, iter, t.x, t.y) : 0u;
   if (a.counter) wave_count(a.counter, n);
 }
 
@@ -629,13 +638,11 @@ __device__ __forceinline__ void cswap(float& x, float& y) {
   y = hi;
 }
 
-__global__ __launch_bounds__(256) void k_pack(PmArgs a, int median,
-                                              float* __restrict__ out_depth,
-                                              float* __restrict__ out_plane,
-                                              float* __restrict__ out_score,
-                                              int* __restrict__ out_nghbr) {
-  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+__device__ __forceinline__ void pack_pixel(const PmArgs& a, int median, int i,
+                                           int j, float* __restrict__ out_depth,
+                                           float* __restrict__ out_plane,
+                                           float* __restrict__ out_score,
+                                           int* __restrict__ out_nghbr) {
   if (i >= a.H || j >= a.W) return;
   const size_t idx = (size_t)i * a.W + j;
   const float4* st = reinterpret_cast<const float4*>(a.state);
@@ -669,6 +676,16 @@ __global__ __launch_bounds__(256) void k_pack(PmArgs a, int median,
   out_nghbr[idx] = a.nghbr[idx];
 }
 
+__global__ __launch_bounds__(256) void k_pack(PmArgs a, int median,
+                                              float* __restrict__ out_depth,
+                                              float* __restrict__ out_plane,
+                                              float* __restrict__ out_score,
+                                              int* __restrict__ out_nghbr) {
+  pack_pixel(a, median, blockIdx.y * 16 + (threadIdx.x >> 4),
+             blockIdx.x * 16 + (threadIdx.x & 15), out_depth, out_plane,
+             out_score, out_nghbr);
+}
+
 // ---------------------------------------------------------------------------
 // DepthmapCleaner::Clean (depthmap.cc:524-555): FP64 back-projection and
 // projection, FP32 bilinear of the other depthmaps, vote counting from 1.
@@ -685,13 +702,12 @@ __device__ __forceinline__ float lerp_depth(const float* __restrict__ img,
   return (1 - dy) * im0 + dy * im1;
 }
 
-__global__ __launch_bounds__(256) void k_clean(const ClView* __restrict__ views,
-                                               const ClRef* __restrict__ ref,
-                                               int n, float thr, int min_views,
-                                               float* __restrict__ out) {
+__device__ __forceinline__ void clean_pixel(const ClView* __restrict__ views,
+                                            const ClRef* __restrict__ ref,
+                                            int n, float thr, int min_views,
+                                            int i, int j,
+                                            float* __restrict__ out) {
   const ClView& V0 = views[0];
-  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (i >= V0.height || j >= V0.width) return;
   const size_t idx = (size_t)i * V0.width + j;
   const float depth = V0.depth[idx];
@@ -747,6 +763,109 @@ __global__ __launch_bounds__(256) void k_clean(const ClView* __restrict__ views,
   out[idx] = consistent >= min_views ? depth : 0.0f;
 }
 
+__global__ __launch_bounds__(256) void k_clean(const ClView* __restrict__ views,
+                                               const ClRef* __restrict__ ref,
+                                               int n, float thr, int min_views,
+                                               float* __restrict__ out) {
+  clean_pixel(views, ref, n, thr, min_views,
+              blockIdx.y * 16 + (threadIdx.x >> 4),
+              blockIdx.x * 16 + (threadIdx.x & 15), out);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: many reference views (jobs) of different sizes and view
+// counts in one launch.  The tiles of all jobs form one flat index space in
+// job order; a tile finds its job by binary search over the jobs' first
+// tile indices (wave-uniform, so the job's arguments come in through scalar
+// loads).  The red/black and init kernels are persistent: a grid sized to
+// the resident workgroups pulls tiles from an atomic queue until it runs
+// dry, so large and small views balance across the CUs.  Each launch has
+// its own queue slot, cleared by k_fill_u32 at the start of a run (a kernel,
+// not a memset, so that the run can be captured in a graph).
+
+__global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+// Index of the job whose tiles (first index at offset `field` of PmJob or
+// ClJob) contain tile t; t0 of job 0 is 0.
+template <typename J>
+__device__ __forceinline__ int find_job(const J* __restrict__ jobs, int njobs,
+                                        int t, int J::*first) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].*first <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Next tile of the workgroup from the queue (wave-uniform result).
+__device__ __forceinline__ int next_tile(unsigned* __restrict__ queue) {
+  __shared__ int s_tile;
+  if (threadIdx.x == 0) s_tile = (int)atomicAdd(queue, 1u);
+  __syncthreads();
+  const int t = __builtin_amdgcn_readfirstlane(s_tile);
+  __syncthreads();
+  return t;
+}
+
+template <int P, bool kSample>
+__global__ __launch_bounds__(256) void k_pm_init_batch(
+    const PmJob* __restrict__ jobs, int njobs, int total,
+    unsigned* __restrict__ queue, unsigned long long* __restrict__ count) {
+  unsigned n = 0;
+  for (;;) {
+    const int t = next_tile(queue);
+    if (t >= total) break;
+    const PmJob& J = jobs[find_job(jobs, njobs, t, &PmJob::pix_t0)];
+    const int lt = t - J.pix_t0;
+    n += init_pixel<P, kSample>(J.a, lt % J.pix_tx, lt / J.pix_tx);
+  }
+  if (count) wave_count(count, n);
+}
+
+template <int P, bool kSample, int TX, int TY>
+__global__ __launch_bounds__(256) void k_pm_half_batch(
+    const PmJob* __restrict__ jobs, int njobs, int total,
+    unsigned* __restrict__ queue, int colour, int iter,
+    unsigned long long* __restrict__ count) {
+  unsigned n = 0;
+  for (;;) {
+    const int t = next_tile(queue);
+    if (t >= total) break;
+    const PmJob& J = jobs[find_job(jobs, njobs, t, &PmJob::half_t0)];
+    const int lt = t - J.half_t0;
+    n += half_pixel<P, kSample, TX, TY>(J.a, colour, iter, lt % J.half_tx,
+                                        lt / J.half_tx);
+  }
+  if (count) wave_count(count, n);
+}
+
+// One workgroup per 16 x 16 pixel tile (memory-bound, no queue needed).
+__global__ __launch_bounds__(256) void k_pack_batch(const PmJob* __restrict__ jobs,
+                                                    int njobs, int median) {
+  const int t = blockIdx.x;
+  const PmJob& J = jobs[find_job(jobs, njobs, t, &PmJob::pix_t0)];
+  const int lt = t - J.pix_t0;
+  pack_pixel(J.a, median, (lt / J.pix_tx) * 16 + (threadIdx.x >> 4),
+             (lt % J.pix_tx) * 16 + (threadIdx.x & 15), J.out_depth,
+             J.out_plane, J.out_score, J.out_nghbr);
+}
+
+__global__ __launch_bounds__(256) void k_clean_batch(
+    const ClJob* __restrict__ jobs, int njobs, const ClView* __restrict__ views,
+    float thr, int min_views) {
+  const int t = blockIdx.x;
+  const ClJob& J = jobs[find_job(jobs, njobs, t, &ClJob::pix_t0)];
+  const int lt = t - J.pix_t0;
+  clean_pixel(views + J.view0, &J.ref, J.nviews, thr, min_views,
+              (lt / J.pix_tx) * 16 + (threadIdx.x >> 4),
+              (lt % J.pix_tx) * 16 + (threadIdx.x & 15), J.out);
+}
+
 // ---------------------------------------------------------------------------
 // Launch wrappers
 
@@ -762,13 +881,13 @@ hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
   return hipGetLastError();
 }
 
-#define PM_DISPATCH_P(P_, CALL)         \
+#define PM_DISPATCH_P(P_, ...)          \
   switch (P_) {                         \
-    case 3: { constexpr int P = 3; CALL; } break; \
-    case 5: { constexpr int P = 5; CALL; } break; \
-    case 7: { constexpr int P = 7; CALL; } break; \
-    case 9: { constexpr int P = 9; CALL; } break; \
-    case 11: { constexpr int P = 11; CALL; } break; \
+    case 3: { constexpr int P = 3; __VA_ARGS__; } break; \
+    case 5: { constexpr int P = 5; __VA_ARGS__; } break; \
+    case 7: { constexpr int P = 7; __VA_ARGS__; } break; \
+    case 9: { constexpr int P = 9; __VA_ARGS__; } break; \
+    case 11: { constexpr int P = 11; __VA_ARGS__; } break; \
     default: return hipErrorInvalidValue; \
   }
 
@@ -820,6 +939,121 @@ hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
                        float* score, int* nghbr, hipStream_t s) {
   hipLaunchKernelGGL(k_pack, grid16(a.W, a.H), dim3(256), 0, s, a, median,
                      depth, plane, score, nghbr);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+// Resident workgroups of kernel `f` with `threads` lanes on the current
+// device (the persistent grid), cached per (device, kernel) so that launches
+// under graph capture make no device queries after the first eager run.
+template <typename F>
+static hipError_t persistent_grid(F f, int threads, int* grid) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, int> cache;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const auto key = std::make_pair(dev, reinterpret_cast<const void*>(f));
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      *grid = it->second;
+      return hipSuccess;
+    }
+  }
+  int per_cu = 0, cus = 0;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, threads, 0);
+  if (e != hipSuccess) return e;
+  *grid = std::max(1, per_cu) * cus;
+  std::lock_guard<std::mutex> lock(mu);
+  cache[key] = *grid;
+  return hipSuccess;
+}
+
+hipError_t launch_pm_init_batch(const PmJob* jobs, int njobs, int total,
+                                int patch, bool sample, unsigned* queue,
+                                unsigned long long* count, hipStream_t s) {
+  PM_DISPATCH_P(patch, {
+    auto f = sample ? k_pm_init_batch<P, true> : k_pm_init_batch<P, false>;
+    int grid = 0;
+    hipError_t e = persistent_grid(f, 256, &grid);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(f, dim3(std::min(grid, total)), dim3(256), 0, s, jobs,
+                       njobs, total, queue, count);
+  });
+  return hipGetLastError();
+}
+
+template <int P, bool kSample>
+static hipError_t half_batch_tiled(int tile, const PmJob* jobs, int njobs,
+                                   int total, unsigned* queue, int colour,
+                                   int iter, unsigned long long* count,
+                                   hipStream_t s) {
+  int grid = 0;
+  hipError_t e = hipSuccess;
+  switch (tile) {
+#define PM_TILE_CASE(ID, TX, TY)                                              \
+  case ID: {                                                                  \
+    auto f = k_pm_half_batch<P, kSample, TX, TY>;                             \
+    e = persistent_grid(f, TX * TY, &grid);                                   \
+    if (e != hipSuccess) return e;                                            \
+    hipLaunchKernelGGL(f, dim3(std::min(grid, total)), dim3(TX * TY), 0, s,   \
+                       jobs, njobs, total, queue, colour, iter, count);       \
+  } break;
+    PM_TILE_CASE(0, 8, 8)
+    PM_TILE_CASE(1, 16, 16)
+    PM_TILE_CASE(2, 32, 8)
+#undef PM_TILE_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+void tile_shape(int tile, int* tx, int* ty) {
+  static const int shapes[3][2] = {{8, 8}, {16, 16}, {32, 8}};
+  const int k = (tile >= 0 && tile < 3) ? tile : 1;
+  *tx = shapes[k][0];
+  *ty = shapes[k][1];
+}
+
+hipError_t launch_pm_half_batch(const PmJob* jobs, int njobs, int total,
+                                int patch, bool sample, int tile, int colour,
+                                int iter, unsigned* queue,
+                                unsigned long long* count, hipStream_t s) {
+  if (sample) {
+    PM_DISPATCH_P(patch, return (half_batch_tiled<P, true>(
+                             tile, jobs, njobs, total, queue, colour, iter,
+                             count, s)));
+  }
+  PM_DISPATCH_P(patch, return (half_batch_tiled<P, false>(
+                           tile, jobs, njobs, total, queue, colour, iter,
+                           count, s)));
+  return hipSuccess;
+}
+
+hipError_t launch_pack_batch(const PmJob* jobs, int njobs, int total,
+                             int median, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_batch, dim3(total), dim3(256), 0, s, jobs, njobs,
+                     median);
+  return hipGetLastError();
+}
+
+hipError_t launch_clean_batch(const ClJob* jobs, int njobs, int total,
+                              const ClView* views, float thr, int min_views,
+                              hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_clean_batch, dim3(total), dim3(256), 0, s, jobs, njobs,
+                     views, thr, min_views);
   return hipGetLastError();
 }
 

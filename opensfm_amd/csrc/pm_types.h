@@ -43,6 +43,20 @@ struct PmArgs {
   unsigned long long* counter;  // scorings executed (nullptr: not counted)
 };
 
+// One reference view of a batch: its estimator arguments plus where its
+// tiles sit in a launch's flat tile space.  Red/black tiles are TX x TY lanes
+// over (columns of one colour) x rows; init/pack/clean tiles are 16 x 16
+// pixels.
+struct PmJob {
+  PmArgs a;                      // a.counter unused (the batch counts per launch)
+  int half_tx, half_ty, half_t0;  // red/black tiles across, down, first index
+  int pix_tx, pix_ty, pix_t0;     // 16 x 16 pixel tiles across, down, first
+  float* out_depth;               // packed outputs (H*W, H*W*3, H*W, H*W)
+  float* out_plane;
+  float* out_score;
+  int* out_nghbr;
+};
+
 // Cleaner view (DepthmapCleaner::AddView, depthmap.cc:515-522).
 struct ClView {
   const float* depth;
@@ -53,6 +67,15 @@ struct ClView {
 // Inverse pose of the view being cleaned: K_0^-1 and R_0^T.
 struct ClRef {
   double Kinv[9], Rt[9];
+};
+
+// One depthmap of a batched clean: views[view0] is the map being cleaned,
+// views[view0 + 1 .. view0 + nviews - 1] the maps it is checked against.
+struct ClJob {
+  ClRef ref;
+  int view0, nviews;
+  int pix_t0, pix_tx;  // 16 x 16 pixel tiles: first index, tiles across
+  float* out;
 };
 
 }  // namespace pm

@@ -379,3 +379,78 @@ def depth_accuracy(depth, gt, min_depth=None, max_depth=None, rel=0.01):
         valid &= gt <= max_depth
     good = valid & (depth > 0) & (np.abs(depth - gt) < rel * gt)
     return float(good.sum()) / max(1, int(valid.sum()))
+
+
+# --------------------------------------------------------------------------
+# street scenes (batched workloads)
+
+STREET_SIZES = ((640, 480), (1024, 768), (1600, 1200), (2048, 1536))
+
+
+@dataclasses.dataclass
+class BatchScene:
+    views: list            # View, pool of every camera
+    jobs: list             # (reference view index, [neighbour view indices])
+    min_depth: float
+    max_depth: float
+    name: str = ""
+
+
+def street_geometry(rng, length=60.0, street_width=8.0, tex_spacing=0.06):
+    """Ground plane plus facade blocks along both sides of a street that
+    runs along world x: each block is set back from the kerb by a random
+    offset and has a random length and height."""
+    boxes = []
+    for side in (1.0, -1.0):
+        x = -length / 2
+        while x < length / 2:
+            seg = rng.uniform(3.0, 9.0)
+            gap = rng.uniform(0.0, 1.5)
+            y0 = street_width / 2 + rng.uniform(0.0, 2.5)
+            h = rng.uniform(4.0, 12.0)
+            lo = np.array([x, side * y0 if side > 0 else -(y0 + 3.0), 0.0])
+            hi = np.array([x + seg, y0 + 3.0 if side > 0 else -y0, h])
+            boxes.append((lo, hi))
+            x += seg + gap
+    return SceneGeometry(boxes=boxes, ground=True, terrain=None,
+                         tex_spacing=tex_spacing,
+                         tex_seed=int(rng.integers(1, 2 ** 62)))
+
+
+def street_scene(n_views, seed=4000, sizes=STREET_SIZES, neighbours=(4, 12),
+                 min_depth=1.0, max_depth=60.0, name="street"):
+    """Cameras along a street (1 m apart, height 1.6-2.6 m, yaw within 40
+    degrees of the street axis, pitched 3-12 degrees down), each with an
+    image size drawn uniformly from `sizes`.  Every camera is a job whose
+    neighbours are its nearest cameras along the track, their count drawn
+    uniformly from the inclusive range `neighbours`."""
+    rng = np.random.default_rng(seed)
+    length = max(60.0, n_views * 1.0 + 30.0)
+    geom = street_geometry(rng, length=length)
+    xs = -n_views / 2.0 + np.arange(n_views) * 1.0
+    poses, dims = [], []
+    for x in xs:
+        c = np.array([x, rng.uniform(-1.5, 1.5), rng.uniform(1.6, 2.6)])
+        yaw = math.radians(rng.uniform(-40.0, 40.0))
+        pitch = math.radians(rng.uniform(3.0, 12.0))
+        f = np.array([math.cos(yaw) * math.cos(pitch), math.sin(yaw) * math.cos(pitch),
+                      -math.sin(pitch)])
+        poses.append(look_at(c, c + f))
+        dims.append(sizes[int(rng.integers(0, len(sizes)))])
+    jobs = []
+    for k in range(n_views):
+        nn = int(rng.integers(neighbours[0], neighbours[1] + 1))
+        order = np.argsort(np.abs(xs - xs[k]), kind="stable")
+        jobs.append((k, [int(v) for v in order if v != k][:nn]))
+
+    def one(k):
+        w, h = dims[k]
+        K = intrinsics(w, h)
+        img, dep = render(geom, K, poses[k][0], poses[k][1], w, h)
+        return View(K=K, R=poses[k][0], t=poses[k][1], image=img,
+                    mask=np.full_like(img, 255), depth=dep)
+
+    with concurrent.futures.ThreadPoolExecutor(_threads()) as ex:
+        views = list(ex.map(one, range(n_views)))
+    return BatchScene(views=views, jobs=jobs, min_depth=min_depth,
+                      max_depth=max_depth, name=name)

@@ -1,0 +1,147 @@
+"""GPU tests of the batched path (DepthmapBatch, pm_batch_* in the C ABI).
+
+Job k of a batch must equal, bit for bit, a DepthmapEstimator run on the
+same views with set_seed(seed + k), for every tile layout, with and without
+the captured graph; the batched cleaner must equal DepthmapCleaner (and the
+oracle's Clean, depthmap.cc:524-555) on the same depthmaps.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from opensfm_amd import DepthmapBatch, DepthmapCleaner, DepthmapEstimator, scenes
+from opensfm_amd._capi import PM_TILE_16x16, PM_TILE_32x8, PM_TILE_8x8
+
+pytestmark = pytest.mark.gpu
+
+SEED = 77
+
+
+@pytest.fixture(scope="module")
+def street():
+    return scenes.street_scene(6, seed=41, sizes=((160, 120), (200, 150), (240, 180)),
+                               neighbours=(2, 4))
+
+
+def _batch(scene, iters=2, patch=7, tile=PM_TILE_16x16, graph=True):
+    b = DepthmapBatch()
+    for v in scene.views:
+        b.add_view(v.K, v.R, v.t, v.image, v.mask)
+    for ref, nb in scene.jobs:
+        b.add_job(ref, nb, scene.min_depth, scene.max_depth)
+    b.set_patchmatch_iterations(iters)
+    b.set_patch_size(patch)
+    b.set_min_patch_sd(1.0)
+    b.set_seed(SEED)
+    b.set_tile(tile)
+    b.set_graph(graph)
+    return b
+
+
+def _single(scene, k, iters=2, patch=7, sample=False):
+    ref, nb = scene.jobs[k]
+    e = DepthmapEstimator()
+    for v in [ref] + list(nb):
+        V = scene.views[v]
+        e.add_view(V.K, V.R, V.t, V.image, V.mask)
+    e.set_depth_range(scene.min_depth, scene.max_depth, 50)
+    e.set_patchmatch_iterations(iters)
+    e.set_patch_size(patch)
+    e.set_min_patch_sd(1.0)
+    e.set_seed(SEED + k)
+    return e.compute_patch_match_sample() if sample else e.compute_patch_match()
+
+
+def _equal(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("tile", [PM_TILE_8x8, PM_TILE_16x16, PM_TILE_32x8])
+def test_batch_equals_single_view_runs(street, tile):
+    print(f"batch vs single-view estimator, tile {tile}", flush=True)
+    b = _batch(street, tile=tile)
+    b.run()
+    for k in range(len(street.jobs)):
+        got = b.result(k)
+        want = _single(street, k)
+        assert _equal(got, want), f"job {k}"
+        assert (got[0] > 0).mean() > 0.3
+
+
+def test_batch_graph_replays_and_eager_agree(street):
+    print("batch graph replay vs eager", flush=True)
+    e = _batch(street, graph=False)
+    e.run()
+    eager = [e.result(k) for k in range(len(street.jobs))]
+    g = _batch(street, graph=True)
+    for rep in range(4):  # an eager run that captures the graph, then replays
+        g.run()
+        got = [g.result(k) for k in range(len(street.jobs))]
+        for k in range(len(street.jobs)):
+            assert _equal(got[k], eager[k]), f"run {rep} job {k}"
+        # other device and host work between replays
+        e.run()
+        _ = np.linalg.svd(np.random.default_rng(rep).normal(size=(64, 64)))
+    st = g.stats()
+    assert st["launches"] == 4 and st["scorings"] > 0 and st["total_ms"] > 0
+
+
+def test_batch_sample_mode_equals_single_view(street):
+    print("batch sample mode vs single-view estimator", flush=True)
+    b = _batch(street, tile=PM_TILE_32x8)
+    b.run(sample=True)
+    for k in range(len(street.jobs)):
+        assert _equal(b.result(k), _single(street, k, sample=True)), f"job {k}"
+
+
+def test_batch_patch5_and_autotune(street):
+    print("batch autotune over patch {5, 7}", flush=True)
+    b = _batch(street, patch=5)
+    b.run()
+    for k in (0, 3):
+        assert _equal(b.result(k), _single(street, k, patch=5))
+    patch, tile = b.autotune(patch_sizes=(5, 7))
+    assert patch in (5, 7) and tile in (PM_TILE_8x8, PM_TILE_16x16, PM_TILE_32x8)
+    b.run()
+    for k in (1, 5):
+        assert _equal(b.result(k), _single(street, k, patch=patch))
+
+
+def test_batch_clean_equals_cleaner_and_oracle(street):
+    print("batched cleaner vs DepthmapCleaner and oracle", flush=True)
+    b = _batch(street)
+    b.run()
+    b.clean(0.01, 2)
+    depth = {ref: b.result(k)[0] for k, (ref, _) in enumerate(street.jobs)}
+    kept = 0
+    for k, (ref, nb) in enumerate(street.jobs):
+        got = b.clean_result(k)
+        c = DepthmapCleaner()
+        c.set_same_depth_threshold(0.01)
+        c.set_min_consistent_views(2)
+        views = []
+        for v in [ref] + list(nb):
+            V = street.views[v]
+            c.add_view(V.K, V.R, V.t, depth[v])
+            views.append((V.K, V.R, V.t, depth[v]))
+        assert np.array_equal(got, c.clean()), f"job {k}"
+        assert np.array_equal(got, oracle.clean(views, 0.01, 2)), f"job {k}"
+        kept += int((got > 0).sum())
+    assert kept > 0
+    assert b.stats()["clean_ms"] > 0
+
+
+def test_batch_argument_errors(street):
+    b = DepthmapBatch()
+    v = street.views[0]
+    vid = b.add_view(v.K, v.R, v.t, v.image)
+    with pytest.raises(IndexError):
+        b.add_job(vid, [5], 1.0, 10.0)
+    with pytest.raises(ValueError):
+        b.add_job(vid, [], 1.0, 10.0)
+    with pytest.raises(ValueError):
+        b.add_job(vid, [vid], 10.0, 1.0)
+    with pytest.raises(ValueError):
+        b.run()  # no jobs
+    with pytest.raises(ValueError):
+        b.set_tile(7)
