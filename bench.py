@@ -119,6 +119,79 @@ def cpu_baseline(scene, threads, target_s=12.0):
     }
 
 
+def batch_bench(n_views, steps=3, seed=4000):
+    """Batched street workload: n_views reference views of sizes drawn from
+    {640x480, 1024x768, 1600x1200, 2048x1536}, 4-12 neighbours each, 3
+    iterations in one captured hipGraph, tile and patch {5, 7} autotuned,
+    then the cleaner with 2 consistent views.  Compared with one
+    DepthmapEstimator run per view on the same GPU."""
+    from opensfm_amd import DepthmapBatch, DepthmapEstimator, scenes
+    from opensfm_amd._capi import PM_MODE_PATCHMATCH, TILE_NAMES
+    log(f"batched workload: rendering {n_views} street views")
+    t0 = time.perf_counter()
+    sc = scenes.street_scene(n_views, seed=seed)
+    log(f"rendered in {time.perf_counter() - t0:.1f} s")
+    npx = sum(sc.views[r].image.size for r, _ in sc.jobs)
+    b = DepthmapBatch()
+    for v in sc.views:
+        b.add_view(v.K, v.R, v.t, v.image, v.mask)
+    for ref, nb in sc.jobs:
+        b.add_job(ref, nb, sc.min_depth, sc.max_depth)
+    b.set_patchmatch_iterations(3)
+    b.set_min_patch_sd(1.0)
+    b.set_seed(seed)
+    t0 = time.perf_counter()
+    patch, tile = b.autotune(patch_sizes=(5, 7))
+    tune_s = time.perf_counter() - t0
+    log(f"autotune: patch {patch}, tile {TILE_NAMES[tile]} ({tune_s:.1f} s)")
+    b.run()  # eager run that captures the graph
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+    batch_s = (time.perf_counter() - t0) / steps
+    st = b.stats()
+    accs, scores = [], []
+    for k, (ref, _) in enumerate(sc.jobs):
+        d, _, s, _ = b.result(k)
+        accs.append(scenes.depth_accuracy(d, sc.views[ref].depth, sc.min_depth, sc.max_depth))
+        scores.append(float(s[d > 0].mean()) if (d > 0).any() else 0.0)
+    b.clean(0.01, 2)
+    clean_ms = b.stats()["clean_ms"]
+    kept, kept_good = 0, 0
+    for k, (ref, _) in enumerate(sc.jobs):
+        c = b.clean_result(k)
+        gt = sc.views[ref].depth
+        m = c > 0
+        kept += int(m.sum())
+        kept_good += int((m & (np.abs(c - gt) < 0.01 * gt)).sum())
+    # one estimator per view, device-resident runs, same settings
+    per_view_s = 0.0
+    for k, (ref, nb) in enumerate(sc.jobs):
+        e = DepthmapEstimator()
+        for v in [ref] + list(nb):
+            V = sc.views[v]
+            e.add_view(V.K, V.R, V.t, V.image, V.mask)
+        e.set_depth_range(sc.min_depth, sc.max_depth, 50)
+        e.set_patchmatch_iterations(3)
+        e.set_patch_size(patch)
+        e.set_min_patch_sd(1.0)
+        e.set_seed(seed + k)
+        t0 = time.perf_counter()
+        e.run(PM_MODE_PATCHMATCH)
+        per_view_s += time.perf_counter() - t0
+    return {
+        "views": n_views, "mpix": round(npx / 1e6, 2), "iterations": 3,
+        "patch": patch, "tile": TILE_NAMES[tile], "autotune_s": round(tune_s, 2),
+        "mpix_s": round(npx / batch_s / 1e6, 3),
+        "ms_per_run": round(batch_s * 1e3, 2), "device_ms": round(st["total_ms"], 2),
+        "per_view_loop_mpix_s": round(npx / per_view_s / 1e6, 3),
+        "accuracy_within_1pct": round(float(np.mean(accs)), 4),
+        "mean_score": round(float(np.mean(scores)), 4),
+        "clean_ms": round(clean_ms, 3),
+        "clean_kept_within_1pct": round(kept_good / max(1, kept), 4),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +199,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batch-views", type=int, default=16, metavar="N",
+                    help="also run the batched street workload with N reference "
+                         "views (0: skip; 128 is the full batched configuration)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (torch) so that "
                          "profiler byte counters can be scaled against a known size")
@@ -273,6 +349,9 @@ def main():
             "gpu_accuracy": round(float(np.mean(gpu_acc)), 4),
             "gpu_at_least_cpu": bool(np.mean(gpu_acc) >= np.mean(cpu_acc)),
         }
+    if args.batch_views > 0:
+        out["batched"] = batch_bench(args.batch_views)
+        log(f"batched: {out['batched']}")
     print(json.dumps(out), flush=True)
 
 

@@ -42,6 +42,12 @@ extern "C" {
 #define PM_MODE_PATCHMATCH 1       /* ComputePatchMatch, depthmap.cc:202-213 */
 #define PM_MODE_PATCHMATCH_SAMPLE 2 /* ComputePatchMatchSample, :215-227 */
 
+/* sample precision of the PatchMatch scorer (the plane sweep and
+ * pm_estimator_score_hypotheses are always the reference's FP32) */
+#define PM_PRECISION_FP32 0 /* u8 texels, FP32 bilinear */
+#define PM_PRECISION_FP16 1 /* FP16 texels and weights, FP32 dot products */
+#define PM_PRECISION_BF16 2 /* BF16 texels and weights, FP32 dot products */
+
 typedef struct pm_estimator pm_estimator;
 typedef struct pm_cleaner pm_cleaner;
 
@@ -105,6 +111,12 @@ int pm_estimator_set_refinement_steps(pm_estimator* est, int n);
  * is below 0.95.  They reach strongly slanted surfaces that the reference
  * reaches by chaining refinements along its sequential sweep. */
 int pm_estimator_set_exploration(pm_estimator* est, int n);
+/* Sample precision of PatchMatch (PM_PRECISION_*, default FP32; FP16 and
+ * BF16 need patch size 5 or 7).  Image samples are held as FP16/BF16 2x2
+ * quads; each bilinear row blend is one FP16/BF16 dot product accumulated
+ * in FP32, and the NCC sums and plane/depth state stay FP32.  Not in the
+ * reference (all FP32, depthmap.cc:441-478). */
+int pm_estimator_set_precision(pm_estimator* est, int precision);
 /* Run PostProcess (5x5 median, depthmap.cc:489-502) after PatchMatch
  * (default 1, as the reference always does; 0 returns the raw depth). */
 int pm_estimator_set_post_process(pm_estimator* est, int enable);
@@ -132,6 +144,14 @@ int pm_estimator_download(pm_estimator* est, float* depth, float* plane,
 int pm_estimator_score_hypotheses(pm_estimator* est, int n, const int32_t* ij,
                                   const float* planes, const int32_t* views,
                                   float* out);
+
+/* The same hypotheses scored by the PatchMatch scorer: FP32 homography
+ * M_v + b_v (K_0^-T v)^T composed on the device, FMA-contracted NCC, samples
+ * in the estimator's precision.  For measuring the scorer's error against
+ * an FP64 recomputation. */
+int pm_estimator_score_hypotheses_fast(pm_estimator* est, int n,
+                                       const int32_t* ij, const float* planes,
+                                       const int32_t* views, float* out);
 
 int pm_estimator_stats(pm_estimator* est, pm_stats* out);
 
@@ -195,6 +215,7 @@ int pm_batch_set_min_patch_sd(pm_batch* b, float sd);
 int pm_batch_set_seed(pm_batch* b, uint64_t seed);
 int pm_batch_set_exploration(pm_batch* b, int n);
 int pm_batch_set_post_process(pm_batch* b, int enable);
+int pm_batch_set_precision(pm_batch* b, int precision);
 /* Lane layout of the red/black launches (default PM_TILE_16x16).  The maps
  * do not depend on it. */
 int pm_batch_set_tile(pm_batch* b, int tile);

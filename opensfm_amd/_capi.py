@@ -23,6 +23,11 @@ PM_MODE_PLANE_SWEEP = 0
 PM_MODE_PATCHMATCH = 1
 PM_MODE_PATCHMATCH_SAMPLE = 2
 
+PM_PRECISION_FP32 = 0
+PM_PRECISION_FP16 = 1
+PM_PRECISION_BF16 = 2
+PRECISIONS = {"fp32": PM_PRECISION_FP32, "fp16": PM_PRECISION_FP16, "bf16": PM_PRECISION_BF16}
+
 PM_TILE_8x8 = 0
 PM_TILE_16x16 = 1
 PM_TILE_32x8 = 2
@@ -63,11 +68,13 @@ SIGNATURES = {
     "pm_estimator_set_far_probes": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_refinement_steps": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_exploration": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_precision": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_compute": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _ip]),
     "pm_estimator_run": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_download": (C.c_int, [_vp, _fp, _fp, _fp, _ip]),
     "pm_estimator_score_hypotheses": (C.c_int, [_vp, C.c_int, _ip, _fp, _ip, _fp]),
+    "pm_estimator_score_hypotheses_fast": (C.c_int, [_vp, C.c_int, _ip, _fp, _ip, _fp]),
     "pm_estimator_stats": (C.c_int, [_vp, C.POINTER(pm_stats)]),
     "pm_cleaner_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pm_cleaner_destroy": (C.c_int, [_vp]),
@@ -89,6 +96,7 @@ SIGNATURES = {
     "pm_batch_set_exploration": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_tile": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_precision": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_graph": (C.c_int, [_vp, C.c_int]),
     "pm_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _ip, _ip]),
     "pm_batch_run": (C.c_int, [_vp, C.c_int]),
@@ -148,6 +156,15 @@ def device_count() -> int:
     n = C.c_int32(0)
     check(load().pm_device_count(C.byref(n)))
     return n.value
+
+
+def precision_code(p) -> int:
+    """PM_PRECISION_* from a code or one of "fp32", "fp16", "bf16"."""
+    if isinstance(p, str):
+        if p.lower() not in PRECISIONS:
+            raise ValueError(f"unknown precision {p!r}")
+        return PRECISIONS[p.lower()]
+    return int(p)
 
 
 def ptr(a: np.ndarray, ctype):

@@ -94,6 +94,10 @@ class DepthmapBatch:
     def set_post_process(self, enable: bool) -> None:
         _capi.check(self._lib.pm_batch_set_post_process(self._h, int(bool(enable))))
 
+    def set_precision(self, precision) -> None:
+        """PatchMatch sample precision: "fp32" (default), "fp16" or "bf16"."""
+        _capi.check(self._lib.pm_batch_set_precision(self._h, _capi.precision_code(precision)))
+
     def set_tile(self, tile: int) -> None:
         _capi.check(self._lib.pm_batch_set_tile(self._h, int(tile)))
 

@@ -28,6 +28,8 @@ struct PoolView {
   double K[9], R[9], t[3], Kinv[9];
   int width = 0, height = 0;
   DevBuf img, mask, quad;
+  DevBuf quad16;       // 16-bit quads, built on first use
+  int quad16_kind = 0;  // precision they were built for (0: none)
 };
 
 struct BatchJob {
@@ -52,6 +54,7 @@ struct pm_batch {
   int post_process = 1;
   int tile = 1;
   int use_graph = 1;
+  int precision = PM_PRECISION_FP32;
   std::vector<std::unique_ptr<PoolView>> views;
   std::vector<BatchJob> jobs;
   // device state of the prepared configuration
@@ -90,6 +93,8 @@ namespace {
 
 int validate(pm_batch* b) {
   if (b->jobs.empty()) return fail(PM_ERR_INVALID, "batch has no jobs");
+  if (!pm::precision_supported(b->precision, b->patch_size))
+    return fail(PM_ERR_INVALID, "FP16/BF16 samples need patch size 5 or 7");
   const int hpz = (b->patch_size - 1) / 2;
   for (size_t k = 0; k < b->jobs.size(); ++k) {
     const PoolView& r = *b->views[b->jobs[k].ref];
@@ -115,6 +120,21 @@ void prepare(pm_batch* b, int count) {
   }
   int tx, ty;
   pm::tile_shape(b->tile, &tx, &ty);
+  if (b->precision != PM_PRECISION_FP32) {
+    for (int k = 0; k < count; ++k) {
+      std::vector<int> used(b->jobs[k].nb);
+      used.push_back(b->jobs[k].ref);
+      for (int v : used) {
+        PoolView& V = *b->views[v];
+        if (V.quad16_kind == b->precision) continue;
+        V.quad16.ensure((size_t)(V.width + 1) * (V.height + 1) * 8);
+        HIP_CHECK(pm::launch_build_quad16(V.img.as<uint8_t>(), V.quad16.as<uint32_t>(),
+                                          V.width, V.height, b->precision, b->stream));
+        V.quad16_kind = b->precision;
+      }
+    }
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+  }
   std::vector<pm::PmView> vt;
   std::vector<int> view0(count);
   size_t px = 0;
@@ -131,12 +151,14 @@ void prepare(pm_batch* b, int count) {
     vt.emplace_back();
     fill_view(vt.back(), R.quad.as<uint32_t>(), R.width, R.height, R.K, Q, a,
               R.Kinv);
+    vt.back().quad16 = R.quad16.as<uint32_t>();
     for (int v : J.nb) {
       const PoolView& V = *b->views[v];
       relative_pose(R.R, R.t, V.R, V.t, Q, a);
       vt.emplace_back();
       fill_view(vt.back(), V.quad.as<uint32_t>(), V.width, V.height, V.K, Q,
                 a, R.Kinv);
+      vt.back().quad16 = V.quad16.as<uint32_t>();
     }
   }
   b->total_px = px;
@@ -185,6 +207,7 @@ void prepare(pm_batch* b, int count) {
     a.n_far = b->n_far;
     a.n_refine = b->n_refine;
     a.n_explore = b->n_explore;
+    a.precision = b->precision;
     a.counter = nullptr;
     T.half_tx = ceil_div((R.width + 1) / 2, tx);
     T.half_ty = ceil_div(R.height, ty);
@@ -224,15 +247,15 @@ void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   HIP_CHECK(pm::launch_fill_u32(b->state.as<unsigned>(), (int)(4 * b->total_px), 0u, b->stream));
   HIP_CHECK(pm::launch_fill_u32(b->score.as<unsigned>(), (int)b->total_px, 0u, b->stream));
   HIP_CHECK(pm::launch_fill_u32(b->nghbr.as<unsigned>(), (int)b->total_px, 0u, b->stream));
-  HIP_CHECK(pm::launch_pm_init_batch(jt, nj, b->pix_total, b->patch_size, sample,
-                                     q, cnt, b->stream));
+  HIP_CHECK(pm::launch_pm_init_batch(jt, nj, b->pix_total, b->patch_size,
+                                     b->precision, sample, q, cnt, b->stream));
   if (half_ev) HIP_CHECK(hipEventRecord(half_ev[0], b->stream));
   int l = 0;
   for (int it = 0; it < b->iterations; ++it) {
     for (int colour = 0; colour < 2; ++colour) {
       HIP_CHECK(pm::launch_pm_half_batch(jt, nj, b->half_total, b->patch_size,
-                                         sample, b->tile, colour, it, q + 1 + l,
-                                         cnt + 1, b->stream));
+                                         b->precision, sample, b->tile, colour, it,
+                                         q + 1 + l, cnt + 1, b->stream));
       ++l;
       if (half_ev) HIP_CHECK(hipEventRecord(half_ev[l], b->stream));
     }
@@ -480,6 +503,15 @@ int pm_batch_set_tile(pm_batch* b, int tile) {
   if (tile < PM_TILE_8x8 || tile > PM_TILE_32x8)
     return fail(PM_ERR_INVALID, "unknown tile " + std::to_string(tile));
   b->tile = tile;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_precision(pm_batch* b, int precision) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (precision < PM_PRECISION_FP32 || precision > PM_PRECISION_BF16)
+    return fail(PM_ERR_INVALID, "unknown precision " + std::to_string(precision));
+  b->precision = precision;
   b->invalidate();
   return PM_OK;
 }

@@ -30,6 +30,8 @@ struct HostView {
   double K[9], R[9], t[3], Kinv[9], Q[9], a[3];
   int width, height;
   DevBuf img, quad;
+  DevBuf quad16;        // 16-bit quads, built on first use
+  int quad16_kind = 0;  // precision they were built for (0: none)
 };
 
 struct pm_estimator {
@@ -47,6 +49,7 @@ struct pm_estimator {
   int n_far = 6;
   int n_refine = 6;
   int n_explore = 3;
+  int precision = PM_PRECISION_FP32;
   int post_process = 1;
   std::vector<std::unique_ptr<HostView>> views;
   DevBuf mask0;
@@ -127,9 +130,16 @@ pm::PmArgs prepare(pm_estimator* e) {
   const HostView& r = *e->views[0];
   std::vector<pm::PmView> tab(n);
   for (int v = 0; v < n; ++v) {
-    const HostView& V = *e->views[v];
+    HostView& V = *e->views[v];
+    if (e->precision != PM_PRECISION_FP32 && V.quad16_kind != e->precision) {
+      V.quad16.ensure((size_t)(V.width + 1) * (V.height + 1) * 8);
+      HIP_CHECK(pm::launch_build_quad16(V.img.as<uint8_t>(), V.quad16.as<uint32_t>(),
+                                        V.width, V.height, e->precision, e->stream));
+      V.quad16_kind = e->precision;
+    }
     fill_view(tab[v], V.quad.as<uint32_t>(), V.width, V.height, V.K, V.Q, V.a,
               r.Kinv);
+    tab[v].quad16 = V.quad16.as<uint32_t>();
   }
   e->viewtab.ensure(sizeof(pm::PmView) * n);
   HIP_CHECK(hipMemcpyAsync(e->viewtab.p, tab.data(), sizeof(pm::PmView) * n,
@@ -179,6 +189,7 @@ pm::PmArgs prepare(pm_estimator* e) {
   a.n_far = e->n_far;
   a.n_refine = e->n_refine;
   a.n_explore = e->n_explore;
+  a.precision = e->precision;
   a.counter = e->counter.as<unsigned long long>();
   return a;
 }
@@ -189,6 +200,8 @@ int run(pm_estimator* e, int mode) {
   if (mode != PM_MODE_PLANE_SWEEP && mode != PM_MODE_PATCHMATCH &&
       mode != PM_MODE_PATCHMATCH_SAMPLE)
     return fail(PM_ERR_INVALID, "unknown compute mode " + std::to_string(mode));
+  if (mode != PM_MODE_PLANE_SWEEP && !pm::precision_supported(e->precision, e->patch_size))
+    return fail(PM_ERR_INVALID, "FP16/BF16 samples need patch size 5 or 7");
   DeviceGuard dg(e->device);
   e->have_result = false;
   pm::PmArgs a = prepare(e);
@@ -305,6 +318,81 @@ int download(pm_estimator* e, float* depth, float* plane, float* score,
   HIP_CHECK(hipStreamSynchronize(e->stream));
   return PM_OK;
 }
+
+// ComputePlaneImageScore of fixed hypotheses: the exact FP32 scorer with
+// host-composed homographies, or (fast) the PatchMatch scorer in the
+// estimator's sample precision.
+int score_hypotheses(pm_estimator* e, int n, const int32_t* ij,
+                     const float* planes, const int32_t* views, float* out,
+                     bool fast) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (n < 0) return fail(PM_ERR_INVALID, "negative hypothesis count");
+  if (n == 0) return PM_OK;
+  if (!ij || !planes || !views || !out) return fail(PM_ERR_INVALID, "null buffer");
+  if (e->views.size() < 2)
+    return fail(PM_ERR_INVALID, "need a reference view and at least one neighbour view");
+  const HostView& r = *e->views[0];
+  const int hpz = (e->patch_size - 1) / 2;
+  for (int k = 0; k < n; ++k) {
+    const int i = ij[2 * k], j = ij[2 * k + 1];
+    if (i < hpz || i >= r.height - hpz || j < hpz || j >= r.width - hpz)
+      return fail(PM_ERR_RANGE, "hypothesis " + std::to_string(k) +
+                                    ": pixel closer than (patch-1)/2 to the border");
+    if (views[k] < 1 || views[k] >= (int)e->views.size())
+      return fail(PM_ERR_RANGE, "hypothesis " + std::to_string(k) + ": view index out of range");
+  }
+  if (fast && !pm::precision_supported(e->precision, e->patch_size))
+    return fail(PM_ERR_INVALID, "FP16/BF16 samples need patch size 5 or 7");
+  return guarded([&] {
+    DeviceGuard dg(e->device);
+    pm::PmArgs a = prepare(e);
+    // group by view: one launch per view keeps the view index uniform
+    const int nv = (int)e->views.size();
+    std::vector<int> order(n), start(nv + 1, 0);
+    for (int k = 0; k < n; ++k) start[views[k] + 1]++;
+    for (int v = 0; v < nv; ++v) start[v + 1] += start[v];
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int k = 0; k < n; ++k) order[fill[views[k]]++] = k;
+    std::vector<int32_t> s_ij(2 * (size_t)n);
+    // FP64 composition on the host, FP32 homography per hypothesis
+    // (PlaneInducedHomographyBaked, depthmap.cc:96-102)
+    // exact: the FP32 homography per hypothesis; fast: the plane itself
+    const int per = fast ? 3 : 9;
+    std::vector<float> s_H(per * (size_t)n), s_out(n);
+    for (int m = 0; m < n; ++m) {
+      const int k = order[m];
+      s_ij[2 * m] = ij[2 * k];
+      s_ij[2 * m + 1] = ij[2 * k + 1];
+      const HostView& V = *e->views[views[k]];
+      if (fast)
+        memcpy(&s_H[3 * (size_t)m], planes + 3 * (size_t)k, 3 * sizeof(float));
+      else
+        homography_baked(r.Kinv, V.Q, V.a, V.K, planes + 3 * (size_t)k, &s_H[9 * (size_t)m]);
+    }
+    DevBuf d_ij, d_H, d_out;
+    d_ij.ensure(sizeof(int32_t) * 2 * n);
+    d_H.ensure(sizeof(float) * per * n);
+    d_out.ensure(sizeof(float) * n);
+    HIP_CHECK(hipMemcpyAsync(d_ij.p, s_ij.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, e->stream));
+    HIP_CHECK(hipMemcpyAsync(d_H.p, s_H.data(), sizeof(float) * per * n, hipMemcpyHostToDevice, e->stream));
+    for (int v = 1; v < nv; ++v) {
+      const int cnt = start[v + 1] - start[v];
+      if (cnt == 0) continue;
+      const int* dij = d_ij.as<int>() + 2 * start[v];
+      const float* dh = d_H.as<float>() + per * (size_t)start[v];
+      float* dout = d_out.as<float>() + start[v];
+      if (fast)
+        HIP_CHECK(pm::launch_score_hyp_fast(a, v, cnt, dij, dh, dout, e->stream));
+      else
+        HIP_CHECK(pm::launch_score_hyp(a, v, cnt, dij, dh, dout, e->stream));
+    }
+    HIP_CHECK(hipMemcpyAsync(s_out.data(), d_out.p, sizeof(float) * n, hipMemcpyDeviceToHost, e->stream));
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    for (int m = 0; m < n; ++m) out[order[m]] = s_out[m];
+    return PM_OK;
+  });
+}
+
 
 }  // namespace
 
@@ -444,6 +532,14 @@ int pm_estimator_set_exploration(pm_estimator* e, int n) {
   return PM_OK;
 }
 
+int pm_estimator_set_precision(pm_estimator* e, int precision) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (precision < PM_PRECISION_FP32 || precision > PM_PRECISION_BF16)
+    return fail(PM_ERR_INVALID, "unknown precision " + std::to_string(precision));
+  e->precision = precision;
+  return PM_OK;
+}
+
 int pm_estimator_set_far_probes(pm_estimator* e, int n) {
   if (!e) return fail(PM_ERR_INVALID, "null estimator");
   if (n < 0 || n > 64) return fail(PM_ERR_INVALID, "far probes must be in [0, 64]");
@@ -481,61 +577,13 @@ int pm_estimator_compute(pm_estimator* e, int mode, float* depth, float* plane,
 int pm_estimator_score_hypotheses(pm_estimator* e, int n, const int32_t* ij,
                                   const float* planes, const int32_t* views,
                                   float* out) {
-  if (!e) return fail(PM_ERR_INVALID, "null estimator");
-  if (n < 0) return fail(PM_ERR_INVALID, "negative hypothesis count");
-  if (n == 0) return PM_OK;
-  if (!ij || !planes || !views || !out) return fail(PM_ERR_INVALID, "null buffer");
-  if (e->views.size() < 2)
-    return fail(PM_ERR_INVALID, "need a reference view and at least one neighbour view");
-  const HostView& r = *e->views[0];
-  const int hpz = (e->patch_size - 1) / 2;
-  for (int k = 0; k < n; ++k) {
-    const int i = ij[2 * k], j = ij[2 * k + 1];
-    if (i < hpz || i >= r.height - hpz || j < hpz || j >= r.width - hpz)
-      return fail(PM_ERR_RANGE, "hypothesis " + std::to_string(k) +
-                                    ": pixel closer than (patch-1)/2 to the border");
-    if (views[k] < 1 || views[k] >= (int)e->views.size())
-      return fail(PM_ERR_RANGE, "hypothesis " + std::to_string(k) + ": view index out of range");
-  }
-  return guarded([&] {
-    DeviceGuard dg(e->device);
-    pm::PmArgs a = prepare(e);
-    // group by view: one launch per view keeps the view index uniform
-    const int nv = (int)e->views.size();
-    std::vector<int> order(n), start(nv + 1, 0);
-    for (int k = 0; k < n; ++k) start[views[k] + 1]++;
-    for (int v = 0; v < nv; ++v) start[v + 1] += start[v];
-    std::vector<int> fill(start.begin(), start.end() - 1);
-    for (int k = 0; k < n; ++k) order[fill[views[k]]++] = k;
-    std::vector<int32_t> s_ij(2 * (size_t)n);
-    // FP64 composition on the host, FP32 homography per hypothesis
-    // (PlaneInducedHomographyBaked, depthmap.cc:96-102)
-    std::vector<float> s_H(9 * (size_t)n), s_out(n);
-    for (int m = 0; m < n; ++m) {
-      const int k = order[m];
-      s_ij[2 * m] = ij[2 * k];
-      s_ij[2 * m + 1] = ij[2 * k + 1];
-      const HostView& V = *e->views[views[k]];
-      homography_baked(r.Kinv, V.Q, V.a, V.K, planes + 3 * (size_t)k, &s_H[9 * (size_t)m]);
-    }
-    DevBuf d_ij, d_H, d_out;
-    d_ij.ensure(sizeof(int32_t) * 2 * n);
-    d_H.ensure(sizeof(float) * 9 * n);
-    d_out.ensure(sizeof(float) * n);
-    HIP_CHECK(hipMemcpyAsync(d_ij.p, s_ij.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, e->stream));
-    HIP_CHECK(hipMemcpyAsync(d_H.p, s_H.data(), sizeof(float) * 9 * n, hipMemcpyHostToDevice, e->stream));
-    for (int v = 1; v < nv; ++v) {
-      const int cnt = start[v + 1] - start[v];
-      if (cnt == 0) continue;
-      HIP_CHECK(pm::launch_score_hyp(a, v, cnt, d_ij.as<int>() + 2 * start[v],
-                                     d_H.as<float>() + 9 * (size_t)start[v],
-                                     d_out.as<float>() + start[v], e->stream));
-    }
-    HIP_CHECK(hipMemcpyAsync(s_out.data(), d_out.p, sizeof(float) * n, hipMemcpyDeviceToHost, e->stream));
-    HIP_CHECK(hipStreamSynchronize(e->stream));
-    for (int m = 0; m < n; ++m) out[order[m]] = s_out[m];
-    return PM_OK;
-  });
+  return score_hypotheses(e, n, ij, planes, views, out, false);
+}
+
+int pm_estimator_score_hypotheses_fast(pm_estimator* e, int n, const int32_t* ij,
+                                       const float* planes, const int32_t* views,
+                                       float* out) {
+  return score_hypotheses(e, n, ij, planes, views, out, true);
 }
 
 int pm_estimator_stats(pm_estimator* e, pm_stats* out) {

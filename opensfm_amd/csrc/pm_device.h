@@ -218,11 +218,76 @@ __device__ __forceinline__ uint32_t fetch_quad(const QuadTex& t, float flx,
   return __builtin_amdgcn_raw_buffer_load_b32(t.rsrc, rec << 2, 0, 0);
 }
 
-// Global-memory texel source for the scorers.
+// Texture of 8-byte quads (FP16 or BF16 values) of a wave-uniform view.
+__device__ __forceinline__ QuadTex make_tex16(const uint32_t* quad16, int W,
+                                              int H) {
+  QuadTex t;
+  quad16 = uniform(quad16);
+  W = uniform(W);
+  H = uniform(H);
+  t.pitch = W + 1;
+  const int records = t.pitch * (H + 1) - (t.pitch + 1);
+  t.rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(quad16 + 2 * (t.pitch + 1)), (short)0, records * 8, 0x00020000);
+  t.xmax = (float)(W - 1);
+  t.ymax = (float)(H - 1);
+  return t;
+}
+
+__device__ __forceinline__ uint2 fetch_quad16(const QuadTex& t, float flx,
+                                              float fly) {
+  const int ix = (int)__builtin_amdgcn_fmed3f(flx, -1.0f, t.xmax);
+  const int iy = (int)__builtin_amdgcn_fmed3f(fly, -1.0f, t.ymax);
+  const int rec = __mul24(iy, t.pitch) + ix;
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(t.rsrc, rec << 3, 0, 0);
+  return make_uint2(v[0], v[1]);
+}
+
+// Texel sources for the fast scorer: the texel of floor coordinates, and
+// its bilinear blend at fractions (fx, fy).
+//
+// FP32: u8 quad, blended in FP32 with FMAs.
 struct GlobalFetch {
   QuadTex tex;
   __device__ __forceinline__ uint32_t operator()(float flx, float fly) const {
     return fetch_quad(tex, flx, fly);
+  }
+  static __device__ __forceinline__ float bilerp(uint32_t t, float fx, float fy) {
+    const float a = ub0(t), b = ub1(t), c = ub2(t), d = ub3(t);
+    const float im0 = __builtin_fmaf(fx, b - a, a);
+    const float im1 = __builtin_fmaf(fx, d - c, c);
+    return __builtin_fmaf(fy, im1 - im0, im0);
+  }
+};
+
+typedef _Float16 pm_half2 __attribute__((ext_vector_type(2)));
+typedef __bf16 pm_bf162 __attribute__((ext_vector_type(2)));
+
+// FP16 / BF16: the quad holds (a, b) and (c, d) as 16-bit pairs; each row
+// is blended by one dot product with the weights (1 - fx, fx) rounded to
+// the same 16-bit format and accumulated in FP32 (v_dot2_f32_f16 /
+// v_dot2_f32_bf16), the column blend is FP32.
+template <bool kBF16>
+struct Global16Fetch {
+  QuadTex tex;
+  __device__ __forceinline__ uint2 operator()(float flx, float fly) const {
+    return fetch_quad16(tex, flx, fly);
+  }
+  static __device__ __forceinline__ float bilerp(uint2 t, float fx, float fy) {
+    float im0, im1;
+    if constexpr (kBF16) {
+      pm_bf162 w;
+      w[0] = (__bf16)(1.0f - fx);
+      w[1] = (__bf16)fx;
+      im0 = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(pm_bf162, t.x), w, 0.0f, false);
+      im1 = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(pm_bf162, t.y), w, 0.0f, false);
+    } else {
+      const pm_half2 w =
+          __builtin_bit_cast(pm_half2, __builtin_amdgcn_cvt_pkrtz(1.0f - fx, fx));
+      im0 = __builtin_amdgcn_fdot2(__builtin_bit_cast(pm_half2, t.x), w, 0.0f, false);
+      im1 = __builtin_amdgcn_fdot2(__builtin_bit_cast(pm_half2, t.y), w, 0.0f, false);
+    }
+    return __builtin_fmaf(fy, im1 - im0, im0);
   }
 };
 
@@ -328,7 +393,8 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
 // ---------------------------------------------------------------------------
 // Fast scorer for PatchMatch: the same local-affine bilateral NCC, with FMAs
 // and one-ulp reciprocals.  Takes the pixel's warp and a texel source
-// (GlobalFetch, or any functor returning the texel of floor coordinates).
+// (GlobalFetch or Global16Fetch: the texel of floor coordinates and its
+// bilinear blend); the NCC sums are FP32 for every texel format.
 // A non-finite coordinate yields a NaN score,
 // which never wins a strict comparison.
 
@@ -353,12 +419,8 @@ __device__ __forceinline__ float score_fast_affine(const RefPatch<P>& rp,
       const float x2 = __builtin_fmaf(dfdx_x, (float)dx, rx);
       const float y2 = __builtin_fmaf(dfdx_y, (float)dx, ry);
       const float flx = floorf(x2), fly = floorf(y2);
-      const uint32_t t = fetch_texel(flx, fly);
-      const float fx = x2 - flx, fy = y2 - fly;
-      const float a = ub0(t), b = ub1(t), c = ub2(t), d = ub3(t);
-      const float im0 = __builtin_fmaf(fx, b - a, a);
-      const float im1 = __builtin_fmaf(fx, d - c, c);
-      const float y = __builtin_fmaf(fy, im1 - im0, im0);
+      const auto t = fetch_texel(flx, fly);
+      const float y = Fetch::bilerp(t, x2 - flx, y2 - fly);
       const float wy = rp.w[k] * y;
       sumy += wy;
       sumyy = __builtin_fmaf(wy, y, sumyy);

@@ -9,6 +9,16 @@ namespace pm {
 
 hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
                              hipStream_t s);
+// 8-byte quads of FP16 (precision 1) or BF16 (precision 2) values.
+hipError_t launch_build_quad16(const uint8_t* img, uint32_t* quad16, int W,
+                               int H, int precision, hipStream_t s);
+// Whether PatchMatch kernels exist for this (sample precision, patch size).
+bool precision_supported(int precision, int patch);
+// The PatchMatch scorer (device homography, sample precision a.precision)
+// for planes[3k..3k+2] at pixels ij[2k], ij[2k+1] against one view.
+hipError_t launch_score_hyp_fast(const PmArgs& a, int view, int n,
+                                 const int* ij, const float* planes, float* out,
+                                 hipStream_t s);
 hipError_t launch_sweep(const PmArgs& a, const float* Htab, const float* planes,
                         int nplanes, hipStream_t s);
 hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
@@ -23,11 +33,12 @@ hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
 hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s);
 void tile_shape(int tile, int* tx, int* ty);
 hipError_t launch_pm_init_batch(const PmJob* jobs, int njobs, int total,
-                                int patch, bool sample, unsigned* queue,
-                                unsigned long long* count, hipStream_t s);
+                                int patch, int precision, bool sample,
+                                unsigned* queue, unsigned long long* count,
+                                hipStream_t s);
 hipError_t launch_pm_half_batch(const PmJob* jobs, int njobs, int total,
-                                int patch, bool sample, int tile, int colour,
-                                int iter, unsigned* queue,
+                                int patch, int precision, bool sample, int tile,
+                                int colour, int iter, unsigned* queue,
                                 unsigned long long* count, hipStream_t s);
 hipError_t launch_pack_batch(const PmJob* jobs, int njobs, int total,
                              int median, hipStream_t s);

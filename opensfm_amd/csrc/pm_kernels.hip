@@ -71,6 +71,46 @@ __global__ void k_build_quad(const uint8_t* __restrict__ img,
   quad[(size_t)ty * (W + 1) + tx] = v;
 }
 
+// The same quads as 16-bit values: uint2 {(a, b), (c, d)} with a, b the top
+// row and c, d the bottom row of the 2x2 neighbourhood, FP16 or BF16 (u8
+// values are exact in both).
+template <bool kBF16>
+__global__ void k_build_quad16(const uint8_t* __restrict__ img,
+                               uint2* __restrict__ quad, int W, int H) {
+  const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ty = blockIdx.y * blockDim.y + threadIdx.y;
+  if (tx > W || ty > H) return;
+  const int ix = tx - 1, iy = ty - 1;
+  float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (ix >= 0 && iy >= 0 && ix <= W - 2 && iy <= H - 2) {
+    const uint8_t* r0 = img + (size_t)iy * W + ix;
+    const uint8_t* r1 = r0 + W;
+    v[0] = r0[0];
+    v[1] = r0[1];
+    v[2] = r1[0];
+    v[3] = r1[1];
+  }
+  uint2 q;
+  if constexpr (kBF16) {
+    pm_bf162 top, bot;
+    top[0] = (__bf16)v[0];
+    top[1] = (__bf16)v[1];
+    bot[0] = (__bf16)v[2];
+    bot[1] = (__bf16)v[3];
+    q.x = __builtin_bit_cast(uint32_t, top);
+    q.y = __builtin_bit_cast(uint32_t, bot);
+  } else {
+    pm_half2 top, bot;
+    top[0] = (_Float16)v[0];
+    top[1] = (_Float16)v[1];
+    bot[0] = (_Float16)v[2];
+    bot[1] = (_Float16)v[3];
+    q.x = __builtin_bit_cast(uint32_t, top);
+    q.y = __builtin_bit_cast(uint32_t, bot);
+  }
+  quad[(size_t)ty * (W + 1) + tx] = q;
+}
+
 // ---------------------------------------------------------------------------
 // Plane sweep = ComputeBruteForce (depthmap.cc:184-200), exact FP32 order.
 // Htab[(d * nviews + v) * 9] is the FP32 homography of sweep plane d into
@@ -163,7 +203,7 @@ __global__ __launch_bounds__(256) void k_score_hyp(PmArgs a, int view, int n,
 }
 
 // ---------------------------------------------------------------------------
-// PatchMatch (fast FP32 path).
+// PatchMatch (fast path).
 
 struct PixelRay {
   float x, y, z;  // K_0^-1 (j, i, 1)
@@ -243,11 +283,28 @@ __device__ __forceinline__ bool warp_to_view(const PmView& V,
   return w != 0.0f && __builtin_isfinite(s);
 }
 
+// Fast NCC of one view under a warp, with the view's texture in the given
+// sample precision (0 FP32 u8 quads, 1 FP16, 2 BF16).
+template <int P, int Prec>
+__device__ __forceinline__ float score_view_fast(const RefPatch<P>& rp,
+                                                 const Warp& wp,
+                                                 const PmView& V) {
+  if constexpr (Prec == 0) {
+    GlobalFetch g;
+    g.tex = make_tex(V.quad, V.width, V.height);
+    return score_fast_affine<P>(rp, wp, g);
+  } else {
+    Global16Fetch<Prec == 2> g;
+    g.tex = make_tex16(V.quad16, V.width, V.height);
+    return score_fast_affine<P>(rp, wp, g);
+  }
+}
+
 // Score a plane against every neighbour view from global memory: max NCC
 // with the lowest view index winning ties (ComputePlaneScore,
 // depthmap.cc:408-419).  Returns the score; *nb gets the view and *depth the
 // plane's depth at the pixel.
-template <int P>
+template <int P, int Prec>
 __device__ __forceinline__ float score_plane_all(const PmArgs& a,
                                                  const RefPatch<P>& rp,
                                                  const PixelRay& r, int i,
@@ -263,11 +320,7 @@ __device__ __forceinline__ float score_plane_all(const PmArgs& a,
     const PmView& V = a.views[v];
     Warp wp;
     float s = -1.0f;
-    if (warp_to_view(V, p, fj, fi, &wp)) {
-      GlobalFetch g;
-      g.tex = make_tex(V.quad, V.width, V.height);
-      s = score_fast_affine<P>(rp, wp, g);
-    }
+    if (warp_to_view(V, p, fj, fi, &wp)) s = score_view_fast<P, Prec>(rp, wp, V);
     if (s > best) {
       best = s;
       bn = v;
@@ -283,7 +336,7 @@ __device__ __forceinline__ float score_plane_all(const PmArgs& a,
 // ask for, so every texture access uses a wave-uniform view.  Lanes with
 // pend == false skip.  Returns the score (-1 if not scored) and the plane's
 // depth at the pixel in *depth.
-template <int P>
+template <int P, int Prec>
 __device__ __forceinline__ float score_plane_view(const PmArgs& a,
                                                   const RefPatch<P>& rp,
                                                   const PixelRay& r, int i,
@@ -301,15 +354,35 @@ __device__ __forceinline__ float score_plane_view(const PmArgs& a,
     if (pend && view == vv) {
       const PmView& V = a.views[vv];
       Warp wp;
-      if (warp_to_view(V, p, fj, fi, &wp)) {
-        GlobalFetch g;
-        g.tex = make_tex(V.quad, V.width, V.height);
-        s = score_fast_affine<P>(rp, wp, g);
-      }
+      if (warp_to_view(V, p, fj, fi, &wp)) s = score_view_fast<P, Prec>(rp, wp, V);
       pend = false;
     }
   }
   return s;
+}
+
+// The PatchMatch scorer for fixed (pixel, plane) hypotheses against one
+// view: the device homography M_v + b_v g^T and the fast NCC in the
+// sample precision Prec.  -1 where the warp is degenerate or the patch
+// untextured, like the scorer inside PatchMatch.
+template <int P, int Prec>
+__global__ __launch_bounds__(256) void k_score_hyp_fast(
+    PmArgs a, int view, int n, const int* __restrict__ ij,
+    const float* __restrict__ planes, float* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = ij[2 * k], j = ij[2 * k + 1];
+  RefPatch<P> rp;
+  float s = -1.0f;
+  if (load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) {
+    const PixelRay r = pixel_ray(a, i, j);
+    const PlaneAtPixel p =
+        plane_at_pixel(a, r, planes[3 * k], planes[3 * k + 1], planes[3 * k + 2]);
+    Warp wp;
+    const PmView& V = a.views[view];
+    if (warp_to_view(V, p, (float)j, (float)i, &wp)) s = score_view_fast<P, Prec>(rp, wp, V);
+  }
+  out[k] = s;
 }
 
 __device__ __forceinline__ bool is_interior(const PmArgs& a, int i, int j) {
@@ -319,7 +392,7 @@ __device__ __forceinline__ bool is_interior(const PmArgs& a, int i, int j) {
 // Random initialisation + ignore mask (RandomInitialization and
 // ComputeIgnoreMask, depthmap.cc:237-281).  Masked and low-variance pixels
 // keep the zero state (the reference scores them and then zeroes them).
-template <int P, bool kSample>
+template <int P, bool kSample, int Prec>
 __device__ __forceinline__ unsigned init_pixel(const PmArgs& a, int tile_x,
                                                int tile_y) {
   constexpr int hpz = (P - 1) / 2;
@@ -362,10 +435,10 @@ __device__ __forceinline__ unsigned init_pixel(const PmArgs& a, int tile_x,
   if constexpr (kSample) {
     // one random neighbour view (uni_(rng_), depthmap.cc:247-249)
     nb = 1 + (int)(r.w % (uint32_t)(a.nviews - 1));
-    s = score_plane_view<P>(a, rp, ray, i, j, v[0], v[1], v[2], nb, textured, &dtmp);
+    s = score_plane_view<P, Prec>(a, rp, ray, i, j, v[0], v[1], v[2], nb, textured, &dtmp);
     count = textured ? 1 : 0;
   } else if (textured) {
-    s = score_plane_all<P>(a, rp, ray, i, j, v[0], v[1], v[2], &nb, &dtmp);
+    s = score_plane_all<P, Prec>(a, rp, ray, i, j, v[0], v[1], v[2], &nb, &dtmp);
     count = a.nviews - 1;
   }
   reinterpret_cast<float4*>(a.state)[idx] = make_float4(v[0], v[1], v[2], depth);
@@ -374,10 +447,10 @@ __device__ __forceinline__ unsigned init_pixel(const PmArgs& a, int tile_x,
   return count;
 }
 
-template <int P, bool kSample>
+template <int P, bool kSample, int Prec>
 __global__ __launch_bounds__(256) void k_pm_init(PmArgs a) {
   const Tile t = row_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
-  const unsigned n = t.ok ? init_pixel<P, kSample>(a, t.x, t.y) : 0u;
+  const unsigned n = t.ok ? init_pixel<P, kSample, Prec>(a, t.x, t.y) : 0u;
   if (a.counter) wave_count(a.counter, n);
 }
 
@@ -405,7 +478,7 @@ constexpr float kExploreBelow = 0.95f;
 // Tile (tile_x, tile_y) of TX x TY lanes covers columns [2 TX tile_x,
 // 2 TX (tile_x + 1)) and rows [TY tile_y, TY (tile_y + 1)); each lane takes
 // the pixel of the given colour in its pair of columns.
-template <int P, bool kSample, int TX, int TY>
+template <int P, bool kSample, int TX, int TY, int Prec>
 __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
                                                int iter, int tile_x,
                                                int tile_y) {
@@ -591,7 +664,7 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
     }
     if constexpr (kSample) {
       float dep;
-      const float s = score_plane_view<P>(a, rp, ray, i, j, v0, v1, v2, view, go, &dep);
+      const float s = score_plane_view<P, Prec>(a, rp, ray, i, j, v0, v1, v2, view, go, &dep);
       if (go) {
         tries += 1;
         if (s > best_s) {
@@ -603,7 +676,7 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
     } else if (go) {
       int nb;
       float dep;
-      const float s = score_plane_all<P>(a, rp, ray, i, j, v0, v1, v2, &nb, &dep);
+      const float s = score_plane_all<P, Prec>(a, rp, ray, i, j, v0, v1, v2, &nb, &dep);
       tries += (unsigned)(a.nviews - 1);
       if (s > best_s) {
         best_s = s;
@@ -618,12 +691,12 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   return tries;
 }
 
-template <int P, bool kSample>
+template <int P, bool kSample, int Prec>
 __global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour,
                                                  int iter) {
   const Tile t = row_tile(((a.W + 1) / 2 + 15) >> 4, (a.H + 15) >> 4);
   const unsigned n =
-      t.ok ? half_pixel<P, kSample, 16, 16>(a, colour, iter, t.x, t.y) : 0u;
+      t.ok ? half_pixel<P, kSample, 16, 16, Prec>(a, colour, iter, t.x, t.y) : 0u;
   if (a.counter) wave_count(a.counter, n);
 }
 
@@ -812,7 +885,7 @@ __device__ __forceinline__ int next_tile(unsigned* __restrict__ queue) {
   return t;
 }
 
-template <int P, bool kSample>
+template <int P, bool kSample, int Prec>
 __global__ __launch_bounds__(256) void k_pm_init_batch(
     const PmJob* __restrict__ jobs, int njobs, int total,
     unsigned* __restrict__ queue, unsigned long long* __restrict__ count) {
@@ -822,12 +895,12 @@ __global__ __launch_bounds__(256) void k_pm_init_batch(
     if (t >= total) break;
     const PmJob& J = jobs[find_job(jobs, njobs, t, &PmJob::pix_t0)];
     const int lt = t - J.pix_t0;
-    n += init_pixel<P, kSample>(J.a, lt % J.pix_tx, lt / J.pix_tx);
+    n += init_pixel<P, kSample, Prec>(J.a, lt % J.pix_tx, lt / J.pix_tx);
   }
   if (count) wave_count(count, n);
 }
 
-template <int P, bool kSample, int TX, int TY>
+template <int P, bool kSample, int TX, int TY, int Prec>
 __global__ __launch_bounds__(256) void k_pm_half_batch(
     const PmJob* __restrict__ jobs, int njobs, int total,
     unsigned* __restrict__ queue, int colour, int iter,
@@ -838,8 +911,8 @@ __global__ __launch_bounds__(256) void k_pm_half_batch(
     if (t >= total) break;
     const PmJob& J = jobs[find_job(jobs, njobs, t, &PmJob::half_t0)];
     const int lt = t - J.half_t0;
-    n += half_pixel<P, kSample, TX, TY>(J.a, colour, iter, lt % J.half_tx,
-                                        lt / J.half_tx);
+    n += half_pixel<P, kSample, TX, TY, Prec>(J.a, colour, iter,
+                                              lt % J.half_tx, lt / J.half_tx);
   }
   if (count) wave_count(count, n);
 }
@@ -871,6 +944,20 @@ __global__ __launch_bounds__(256) void k_clean_batch(
 
 static inline dim3 grid16(int w, int h) {
   return dim3((w + 15) / 16, (h + 15) / 16);
+}
+
+hipError_t launch_build_quad16(const uint8_t* img, uint32_t* quad16, int W,
+                               int H, int precision, hipStream_t s) {
+  dim3 blk(16, 16);
+  dim3 grd((W + 1 + 15) / 16, (H + 1 + 15) / 16);
+  uint2* q = reinterpret_cast<uint2*>(quad16);
+  if (precision == 2)
+    hipLaunchKernelGGL(k_build_quad16<true>, grd, blk, 0, s, img, q, W, H);
+  else if (precision == 1)
+    hipLaunchKernelGGL(k_build_quad16<false>, grd, blk, 0, s, img, q, W, H);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
@@ -907,31 +994,64 @@ hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
   return hipGetLastError();
 }
 
-hipError_t launch_pm_init(const PmArgs& a, bool sample, hipStream_t s) {
-  const int P = 2 * a.hpz + 1;
-  if (sample) {
-    PM_DISPATCH_P(P, hipLaunchKernelGGL((k_pm_init<P, true>),
-                                        tile_grid((a.W + 15) / 16, (a.H + 15) / 16),
-                                        dim3(256), 0, s, a));
-    return hipGetLastError();
+// Reduced-precision samples are built for patch sizes 5 and 7 only.
+#define PM_DISPATCH_P57(P_, ...)        \
+  switch (P_) {                         \
+    case 5: { constexpr int P = 5; __VA_ARGS__; } break; \
+    case 7: { constexpr int P = 7; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue; \
   }
-  PM_DISPATCH_P(P, hipLaunchKernelGGL((k_pm_init<P, false>), tile_grid((a.W + 15) / 16, (a.H + 15) / 16), dim3(256),
-                                      0, s, a));
+
+#define PM_DISPATCH_PREC(PREC_, P_, ...)                                 \
+  switch (PREC_) {                                                       \
+    case 0: { constexpr int Prec = 0; PM_DISPATCH_P(P_, __VA_ARGS__); } break;   \
+    case 1: { constexpr int Prec = 1; PM_DISPATCH_P57(P_, __VA_ARGS__); } break; \
+    case 2: { constexpr int Prec = 2; PM_DISPATCH_P57(P_, __VA_ARGS__); } break; \
+    default: return hipErrorInvalidValue;                                \
+  }
+
+bool precision_supported(int precision, int patch) {
+  return precision == 0 || ((precision == 1 || precision == 2) &&
+                            (patch == 5 || patch == 7));
+}
+
+hipError_t launch_pm_init(const PmArgs& a, bool sample, hipStream_t s) {
+  const int P_ = 2 * a.hpz + 1;
+  const dim3 grid = tile_grid((a.W + 15) / 16, (a.H + 15) / 16);
+  if (sample) {
+    PM_DISPATCH_PREC(a.precision, P_,
+                     hipLaunchKernelGGL((k_pm_init<P, true, Prec>), grid, dim3(256), 0, s, a));
+  } else {
+    PM_DISPATCH_PREC(a.precision, P_,
+                     hipLaunchKernelGGL((k_pm_init<P, false, Prec>), grid, dim3(256), 0, s, a));
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
                           hipStream_t s) {
-  const int P = 2 * a.hpz + 1;
+  const int P_ = 2 * a.hpz + 1;
+  const dim3 grid = tile_grid(((a.W + 1) / 2 + 15) / 16, (a.H + 15) / 16);
   if (sample) {
-    PM_DISPATCH_P(P, hipLaunchKernelGGL((k_pm_half<P, true>),
-                                        tile_grid(((a.W + 1) / 2 + 15) / 16, (a.H + 15) / 16),
-                                        dim3(256), 0, s, a, colour, iter));
-    return hipGetLastError();
+    PM_DISPATCH_PREC(a.precision, P_,
+                     hipLaunchKernelGGL((k_pm_half<P, true, Prec>), grid, dim3(256), 0, s,
+                                        a, colour, iter));
+  } else {
+    PM_DISPATCH_PREC(a.precision, P_,
+                     hipLaunchKernelGGL((k_pm_half<P, false, Prec>), grid, dim3(256), 0, s,
+                                        a, colour, iter));
   }
-  PM_DISPATCH_P(P, hipLaunchKernelGGL((k_pm_half<P, false>),
-                                      tile_grid(((a.W + 1) / 2 + 15) / 16, (a.H + 15) / 16), dim3(256), 0,
-                                      s, a, colour, iter));
+  return hipGetLastError();
+}
+
+hipError_t launch_score_hyp_fast(const PmArgs& a, int view, int n,
+                                 const int* ij, const float* planes, float* out,
+                                 hipStream_t s) {
+  const int P_ = 2 * a.hpz + 1;
+  if (n <= 0) return hipSuccess;
+  PM_DISPATCH_PREC(a.precision, P_,
+                   hipLaunchKernelGGL((k_score_hyp_fast<P, Prec>), dim3((n + 255) / 256),
+                                      dim3(256), 0, s, a, view, n, ij, planes, out));
   return hipGetLastError();
 }
 
@@ -978,21 +1098,34 @@ static hipError_t persistent_grid(F f, int threads, int* grid) {
   return hipSuccess;
 }
 
-hipError_t launch_pm_init_batch(const PmJob* jobs, int njobs, int total,
-                                int patch, bool sample, unsigned* queue,
-                                unsigned long long* count, hipStream_t s) {
-  PM_DISPATCH_P(patch, {
-    auto f = sample ? k_pm_init_batch<P, true> : k_pm_init_batch<P, false>;
-    int grid = 0;
-    hipError_t e = persistent_grid(f, 256, &grid);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(f, dim3(std::min(grid, total)), dim3(256), 0, s, jobs,
-                       njobs, total, queue, count);
-  });
+template <int P, bool kSample, int Prec>
+static hipError_t init_batch(const PmJob* jobs, int njobs, int total,
+                             unsigned* queue, unsigned long long* count,
+                             hipStream_t s) {
+  auto f = k_pm_init_batch<P, kSample, Prec>;
+  int grid = 0;
+  hipError_t e = persistent_grid(f, 256, &grid);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(f, dim3(std::min(grid, total)), dim3(256), 0, s, jobs,
+                     njobs, total, queue, count);
   return hipGetLastError();
 }
 
-template <int P, bool kSample>
+hipError_t launch_pm_init_batch(const PmJob* jobs, int njobs, int total,
+                                int patch, int precision, bool sample,
+                                unsigned* queue, unsigned long long* count,
+                                hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  if (sample) {
+    PM_DISPATCH_PREC(precision, patch,
+                     return (init_batch<P, true, Prec>(jobs, njobs, total, queue, count, s)));
+  }
+  PM_DISPATCH_PREC(precision, patch,
+                   return (init_batch<P, false, Prec>(jobs, njobs, total, queue, count, s)));
+  return hipSuccess;
+}
+
+template <int P, bool kSample, int Prec>
 static hipError_t half_batch_tiled(int tile, const PmJob* jobs, int njobs,
                                    int total, unsigned* queue, int colour,
                                    int iter, unsigned long long* count,
@@ -1002,7 +1135,7 @@ static hipError_t half_batch_tiled(int tile, const PmJob* jobs, int njobs,
   switch (tile) {
 #define PM_TILE_CASE(ID, TX, TY)                                              \
   case ID: {                                                                  \
-    auto f = k_pm_half_batch<P, kSample, TX, TY>;                             \
+    auto f = k_pm_half_batch<P, kSample, TX, TY, Prec>;                       \
     e = persistent_grid(f, TX * TY, &grid);                                   \
     if (e != hipSuccess) return e;                                            \
     hipLaunchKernelGGL(f, dim3(std::min(grid, total)), dim3(TX * TY), 0, s,   \
@@ -1026,17 +1159,18 @@ void tile_shape(int tile, int* tx, int* ty) {
 }
 
 hipError_t launch_pm_half_batch(const PmJob* jobs, int njobs, int total,
-                                int patch, bool sample, int tile, int colour,
-                                int iter, unsigned* queue,
+                                int patch, int precision, bool sample, int tile,
+                                int colour, int iter, unsigned* queue,
                                 unsigned long long* count, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
   if (sample) {
-    PM_DISPATCH_P(patch, return (half_batch_tiled<P, true>(
-                             tile, jobs, njobs, total, queue, colour, iter,
-                             count, s)));
+    PM_DISPATCH_PREC(precision, patch,
+                     return (half_batch_tiled<P, true, Prec>(tile, jobs, njobs, total, queue,
+                                                             colour, iter, count, s)));
   }
-  PM_DISPATCH_P(patch, return (half_batch_tiled<P, false>(
-                           tile, jobs, njobs, total, queue, colour, iter,
-                           count, s)));
+  PM_DISPATCH_PREC(precision, patch,
+                   return (half_batch_tiled<P, false, Prec>(tile, jobs, njobs, total, queue,
+                                                            colour, iter, count, s)));
   return hipSuccess;
 }
 

@@ -14,6 +14,8 @@ constexpr int kMaxViews = 64;  // reference view + up to 63 neighbours
 // M_v = K_v Q_v K_0^-1 and b_v = K_v a_v composed in FP64 on the host.
 struct PmView {
   const uint32_t* quad;  // zero-bordered 2x2 texel quads, pitch width + 1
+  const uint32_t* quad16;  // the same quads as 4 FP16 or BF16 values (8 B
+                           // per texel), for reduced-precision PatchMatch
   int width, height;
   float M[9];
   float b[3];
@@ -39,7 +41,8 @@ struct PmArgs {
   uint32_t seed_lo, seed_hi;
   int n_far;           // far propagation probes per direction
   int n_refine;        // random refinements per pixel update (6-step schedule)
-  int n_explore;       // wide-cone normal samples per pixel update (0: none)
+  int n_explore;       // exploration samples per pixel update (0: none)
+  int precision;       // PatchMatch bilinear samples: 0 FP32, 1 FP16, 2 BF16
   unsigned long long* counter;  // scorings executed (nullptr: not counted)
 };
 

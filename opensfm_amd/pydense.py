@@ -9,8 +9,10 @@ include/opensfm_amd.h; the calls release the GIL (ctypes does) like the
 reference's bindings do (depthmap_bind.h:57-60).
 
 Additive API: ``compute_plane_sweep`` (alias of ``compute_brute_force``),
-``set_seed``, ``set_far_probes``, ``score_hypotheses``, ``stats`` and the
-``device`` constructor argument.
+``set_seed``, ``set_far_probes``, ``set_refinement_steps``,
+``set_exploration``, ``set_precision``, ``set_post_process``,
+``score_hypotheses``, ``run``/``download``, ``stats`` and the ``device``
+constructor argument.
 """
 from __future__ import annotations
 
@@ -91,6 +93,10 @@ class DepthmapEstimator:
     def set_exploration(self, n: int) -> None:
         _capi.check(self._lib.pm_estimator_set_exploration(self._h, int(n)))
 
+    def set_precision(self, precision) -> None:
+        """PatchMatch sample precision: "fp32" (default), "fp16" or "bf16"."""
+        _capi.check(self._lib.pm_estimator_set_precision(self._h, _capi.precision_code(precision)))
+
     def set_post_process(self, enable: bool) -> None:
         _capi.check(self._lib.pm_estimator_set_post_process(self._h, int(bool(enable))))
 
@@ -131,17 +137,19 @@ class DepthmapEstimator:
             _capi.ptr(score, C.c_float), _capi.ptr(nghbr, C.c_int32)))
         return [depth, plane, score, nghbr]
 
-    def score_hypotheses(self, ij, planes, views) -> np.ndarray:
+    def score_hypotheses(self, ij, planes, views, fast: bool = False) -> np.ndarray:
         """ComputePlaneImageScore (depthmap.cc:441-478) of fixed hypotheses:
-        ij (n, 2) = (row, col), planes (n, 3), views (n,) in 1..N-1."""
+        ij (n, 2) = (row, col), planes (n, 3), views (n,) in 1..N-1.  With
+        fast=True, the PatchMatch scorer in the set sample precision."""
         ij = np.ascontiguousarray(np.asarray(ij, np.int32).reshape(-1, 2))
         n = ij.shape[0]
         planes = np.ascontiguousarray(np.asarray(planes, np.float32).reshape(n, 3))
         views = np.ascontiguousarray(np.asarray(views, np.int32).reshape(n))
         out = np.zeros(n, np.float32)
-        _capi.check(self._lib.pm_estimator_score_hypotheses(
-            self._h, n, _capi.ptr(ij, C.c_int32), _capi.ptr(planes, C.c_float),
-            _capi.ptr(views, C.c_int32), _capi.ptr(out, C.c_float)))
+        fn = (self._lib.pm_estimator_score_hypotheses_fast if fast
+              else self._lib.pm_estimator_score_hypotheses)
+        _capi.check(fn(self._h, n, _capi.ptr(ij, C.c_int32), _capi.ptr(planes, C.c_float),
+                       _capi.ptr(views, C.c_int32), _capi.ptr(out, C.c_float)))
         return out
 
     def stats(self) -> dict:
