@@ -40,6 +40,23 @@ WORKLOAD = dict(width=1600, height=1200, neighbours=8, boxes=20,
                 min_patch_sd=1.0, iterations=5, seed=2000)
 
 
+PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           "pm_half_p7_1600x1200_pmc.md")
+
+
+def pmc_traffic():
+    """HBM bytes per k_pm_half<7> launch on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy
+    kernel of known size); None if the summary is absent."""
+    try:
+        for line in open(PMC_SUMMARY):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -318,7 +335,9 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": pmc_traffic(),
+            "traffic_source": "rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch, "
+                              "profiles/pm_half_p7_1600x1200_pmc.md",
             "avg_launch_ms": round(avg_launch_s * 1e3, 4),
             "scorings_per_launch": int(dom_scorings / max(1, dom_launches)),
         },

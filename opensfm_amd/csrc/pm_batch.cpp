@@ -172,7 +172,8 @@ void prepare(pm_batch* b, int count) {
   b->out_plane.ensure(px * 3 * sizeof(float));
   b->out_score.ensure(px * sizeof(float));
   b->out_nghbr.ensure(px * sizeof(int));
-  b->queue.ensure((2 * (size_t)std::max(b->iterations, 1) + 1) * sizeof(unsigned));
+  // one init queue, then 8 band queues per red/black launch
+  b->queue.ensure((1 + 16 * (size_t)std::max(b->iterations, 1)) * sizeof(unsigned));
   b->counter.ensure(2 * sizeof(unsigned long long));
 
   std::vector<pm::PmJob> jt(count);
@@ -239,7 +240,7 @@ void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   const pm::PmJob* jt = b->jobtab.as<pm::PmJob>();
   unsigned* q = b->queue.as<unsigned>();
   unsigned long long* cnt = b->counter.as<unsigned long long>();
-  const int nq = 2 * b->iterations + 1;
+  const int nq = 1 + 16 * b->iterations;
   HIP_CHECK(pm::launch_fill_u32(q, nq, 0u, b->stream));
   HIP_CHECK(pm::launch_fill_u32(reinterpret_cast<unsigned*>(cnt), 4, 0u, b->stream));
   // every pixel starts at the zero state (border, masked and low-variance
@@ -255,7 +256,7 @@ void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
     for (int colour = 0; colour < 2; ++colour) {
       HIP_CHECK(pm::launch_pm_half_batch(jt, nj, b->half_total, b->patch_size,
                                          b->precision, sample, b->tile, colour, it,
-                                         q + 1 + l, cnt + 1, b->stream));
+                                         q + 1 + 8 * l, cnt + 1, b->stream));
       ++l;
       if (half_ev) HIP_CHECK(hipEventRecord(half_ev[l], b->stream));
     }

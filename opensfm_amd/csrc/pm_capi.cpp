@@ -54,7 +54,7 @@ struct pm_estimator {
   std::vector<std::unique_ptr<HostView>> views;
   DevBuf mask0;
   // per-compute device buffers
-  DevBuf viewtab, wlut, state, score, nghbr, counter;
+  DevBuf viewtab, wlut, state, score, nghbr, counter, queues;
   DevBuf out_depth, out_plane, out_score, out_nghbr;
   DevBuf sweep_H, sweep_planes;
   bool have_result = false;
@@ -257,10 +257,14 @@ int run(pm_estimator* e, int mode) {
       HIP_CHECK(hipEventCreate(&ev));
       e->half_ev.push_back(ev);
     }
+    // 8 band queues per red/black launch, cleared once per compute
+    e->queues.ensure((size_t)8 * std::max(nl, 1) * sizeof(unsigned));
+    HIP_CHECK(pm::launch_fill_u32(e->queues.as<unsigned>(), 8 * nl, 0u, e->stream));
     HIP_CHECK(hipEventRecord(e->half_ev[0], e->stream));
     for (int it = 0; it < e->iterations; ++it) {
       for (int colour = 0; colour < 2; ++colour) {
-        HIP_CHECK(pm::launch_pm_half(ah, sample, colour, it, e->stream));
+        HIP_CHECK(pm::launch_pm_half(ah, sample, colour, it,
+                                     e->queues.as<unsigned>() + 8 * launches, e->stream));
         HIP_CHECK(hipEventRecord(e->half_ev[++launches], e->stream));
       }
     }

@@ -24,8 +24,9 @@ hipError_t launch_sweep(const PmArgs& a, const float* Htab, const float* planes,
 hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
                             const float* Hs, float* out, hipStream_t s);
 hipError_t launch_pm_init(const PmArgs& a, bool sample, hipStream_t s);
+// queues: 8 zeroed counters (the band queues of one launch).
 hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
-                          hipStream_t s);
+                          unsigned* queues, hipStream_t s);
 hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
                        float* score, int* nghbr, hipStream_t s);
 // Batched launches over the jobs of a PmJob table (see pm_types.h).  `tile`
@@ -36,6 +37,7 @@ hipError_t launch_pm_init_batch(const PmJob* jobs, int njobs, int total,
                                 int patch, int precision, bool sample,
                                 unsigned* queue, unsigned long long* count,
                                 hipStream_t s);
+// launch_pm_half_batch: queue = 8 zeroed band counters.
 hipError_t launch_pm_half_batch(const PmJob* jobs, int njobs, int total,
                                 int patch, int precision, bool sample, int tile,
                                 int colour, int iter, unsigned* queue,

@@ -28,12 +28,13 @@ __device__ __forceinline__ void wave_count(unsigned long long* c, unsigned v) {
 
 // Tile order.  The MI355X deals the workgroups of a launch round-robin to its
 // 8 XCDs (each with its own L2), so consecutive row-major tiles land on
-// different XCDs.  Renumbering so that each XCD walks one contiguous band of
-// the image was measured 4 % slower on the 1600x1200 workload: the per-pixel
+// different XCDs.  A static renumbering that gives each XCD one contiguous
+// band was measured 4 % slower on the 1600x1200 workload: the per-pixel
 // work varies across the image (masked pixels, skipped duplicate
-// candidates) and contiguous bands leave XCDs unevenly loaded, while the
-// round-robin order samples the whole image evenly.  Tiles therefore stay in
-// plain row-major order on a 1-D grid.
+// candidates), so fixed bands leave XCDs unevenly loaded.  The one-shot
+// kernels (init, sweep) therefore use plain row-major order on a 1-D grid;
+// the red/black kernel, which dominates, bands dynamically with band
+// switching at the end (k_pm_half).
 struct Tile {
   int x, y;
   bool ok;
@@ -691,15 +692,6 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   return tries;
 }
 
-template <int P, bool kSample, int Prec>
-__global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour,
-                                                 int iter) {
-  const Tile t = row_tile(((a.W + 1) / 2 + 15) >> 4, (a.H + 15) >> 4);
-  const unsigned n =
-      t.ok ? half_pixel<P, kSample, 16, 16, Prec>(a, colour, iter, t.x, t.y) : 0u;
-  if (a.counter) wave_count(a.counter, n);
-}
-
 // ---------------------------------------------------------------------------
 // Output packing with the optional PostProcess (depthmap.cc:489-502):
 // 5x5 median with replicate border, then depth = 0 where d == 0 or
@@ -900,19 +892,57 @@ __global__ __launch_bounds__(256) void k_pm_init_batch(
   if (count) wave_count(count, n);
 }
 
+// Single-view red/black half-iteration, persistent and XCD-banded.  The
+// tile rows are split into 8 bands; workgroup b starts on band b % 8 -- the
+// XCD the hardware deals it to -- and pulls that band's tiles in raster
+// order from the band's queue, then moves on to the other bands' queues
+// until all are dry.  Each XCD thus works through one compact region at a
+// time, so its L2 holds the neighbour-view footprint of that region, and
+// the band switching at the end balances the XCDs' loads.  queues[0..7]
+// start at zero.
+template <int P, bool kSample, int Prec>
+__global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour, int iter,
+                                                 unsigned* __restrict__ queues) {
+  const int tiles_x = ((a.W + 1) / 2 + 15) >> 4;
+  const int ntiles = tiles_x * ((a.H + 15) >> 4);
+  const int home = (int)(blockIdx.x & 7u);
+  unsigned n = 0;
+  for (int s = 0; s < 8; ++s) {
+    const int band = (home + s) & 7;
+    const int t0 = (int)((long long)band * ntiles / 8);
+    const int t1 = (int)((long long)(band + 1) * ntiles / 8);
+    for (;;) {
+      const int t = t0 + next_tile(queues + band);
+      if (t >= t1) break;
+      n += half_pixel<P, kSample, 16, 16, Prec>(a, colour, iter, t % tiles_x,
+                                                t / tiles_x);
+    }
+  }
+  if (a.counter) wave_count(a.counter, n);
+}
+
+// Batched red/black half-iteration: the flat tile space of all jobs split
+// into 8 bands, pulled XCD by XCD as in k_pm_half.  queues[0..7] start at
+// zero.
 template <int P, bool kSample, int TX, int TY, int Prec>
 __global__ __launch_bounds__(256) void k_pm_half_batch(
     const PmJob* __restrict__ jobs, int njobs, int total,
-    unsigned* __restrict__ queue, int colour, int iter,
+    unsigned* __restrict__ queues, int colour, int iter,
     unsigned long long* __restrict__ count) {
+  const int home = (int)(blockIdx.x & 7u);
   unsigned n = 0;
-  for (;;) {
-    const int t = next_tile(queue);
-    if (t >= total) break;
-    const PmJob& J = jobs[find_job(jobs, njobs, t, &PmJob::half_t0)];
-    const int lt = t - J.half_t0;
-    n += half_pixel<P, kSample, TX, TY, Prec>(J.a, colour, iter,
-                                              lt % J.half_tx, lt / J.half_tx);
+  for (int s = 0; s < 8; ++s) {
+    const int band = (home + s) & 7;
+    const int t0 = (int)((long long)band * total / 8);
+    const int t1 = (int)((long long)(band + 1) * total / 8);
+    for (;;) {
+      const int t = t0 + next_tile(queues + band);
+      if (t >= t1) break;
+      const PmJob& J = jobs[find_job(jobs, njobs, t, &PmJob::half_t0)];
+      const int lt = t - J.half_t0;
+      n += half_pixel<P, kSample, TX, TY, Prec>(J.a, colour, iter,
+                                                lt % J.half_tx, lt / J.half_tx);
+    }
   }
   if (count) wave_count(count, n);
 }
@@ -1028,20 +1058,30 @@ hipError_t launch_pm_init(const PmArgs& a, bool sample, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int P, bool kSample, int Prec>
+static hipError_t half_banded(const PmArgs& a, int colour, int iter,
+                              unsigned* queues, hipStream_t s) {
+  auto f = k_pm_half<P, kSample, Prec>;
+  int grid = 0;
+  hipError_t e = persistent_grid(f, 256, &grid);
+  if (e != hipSuccess) return e;
+  const int ntiles = (((a.W + 1) / 2 + 15) / 16) * ((a.H + 15) / 16);
+  // whole multiples of 8 keep workgroup b on XCD b % 8
+  grid = std::max(8, std::min(grid, (ntiles + 7) / 8 * 8));
+  hipLaunchKernelGGL(f, dim3(grid), dim3(256), 0, s, a, colour, iter, queues);
+  return hipGetLastError();
+}
+
 hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
-                          hipStream_t s) {
+                          unsigned* queues, hipStream_t s) {
   const int P_ = 2 * a.hpz + 1;
-  const dim3 grid = tile_grid(((a.W + 1) / 2 + 15) / 16, (a.H + 15) / 16);
   if (sample) {
     PM_DISPATCH_PREC(a.precision, P_,
-                     hipLaunchKernelGGL((k_pm_half<P, true, Prec>), grid, dim3(256), 0, s,
-                                        a, colour, iter));
-  } else {
-    PM_DISPATCH_PREC(a.precision, P_,
-                     hipLaunchKernelGGL((k_pm_half<P, false, Prec>), grid, dim3(256), 0, s,
-                                        a, colour, iter));
+                     return (half_banded<P, true, Prec>(a, colour, iter, queues, s)));
   }
-  return hipGetLastError();
+  PM_DISPATCH_PREC(a.precision, P_,
+                   return (half_banded<P, false, Prec>(a, colour, iter, queues, s)));
+  return hipSuccess;
 }
 
 hipError_t launch_score_hyp_fast(const PmArgs& a, int view, int n,
@@ -1138,7 +1178,8 @@ static hipError_t half_batch_tiled(int tile, const PmJob* jobs, int njobs,
     auto f = k_pm_half_batch<P, kSample, TX, TY, Prec>;                       \
     e = persistent_grid(f, TX * TY, &grid);                                   \
     if (e != hipSuccess) return e;                                            \
-    hipLaunchKernelGGL(f, dim3(std::min(grid, total)), dim3(TX * TY), 0, s,   \
+    grid = std::max(8, std::min(grid, (total + 7) / 8 * 8));                  \
+    hipLaunchKernelGGL(f, dim3(grid), dim3(TX * TY), 0, s,                    \
                        jobs, njobs, total, queue, colour, iter, count);       \
   } break;
     PM_TILE_CASE(0, 8, 8)
