@@ -246,6 +246,69 @@ int pm_batch_stats(pm_batch* b, pm_stats* out);
 /* Device time of the last pm_batch_clean, milliseconds. */
 int pm_batch_clean_ms(pm_batch* b, float* ms);
 
+/* ---- Initialiser network kernels ------------------------------------- */
+
+/* BF16 MFMA kernels of the learned PatchMatch initialiser (a U-Net over the
+ * quarter-resolution reference image and a coarse plane-sweep cost
+ * volume), forward and backward.  No counterpart in the reference.  All
+ * tensor pointers are device pointers; `stream` is a hipStream_t (NULL: the
+ * default stream).  Activations are NHWC BF16 with a channel stride (xs,
+ * ys, as) and offset (yo, ao), so skip connections share concatenation
+ * buffers; gradients are FP32; weights are [cout][kp] BF16 with
+ * k = tap * cin + ci, tap = 3 (dy + 1) + (dx + 1), zero padded to
+ * kp = pm_nn_conv_kpad(cin).  Channel counts, strides and offsets of
+ * convolution inputs are multiples of 8. */
+int pm_nn_conv_kpad(int cin);
+/* 3x3 convolution, zero padding 1: y = conv(x, w) + bias (BF16 or FP32,
+ * FP32 optionally accumulated), act = leaky_relu_0.1(y) in BF16 (either
+ * may be NULL).  x is BF16, or FP32 (x_f32) rounded to BF16 on load -- the
+ * input gradient is this call on the output gradient with
+ * pm_nn_weight_flip weights. */
+int pm_nn_conv3x3(const void* x, int x_f32, int npix, int H, int W, int cin,
+                  int xs, const void* w, int cout, int kp, const float* bias,
+                  void* y, int y_f32, int ys, int yo, int accumulate, void* act,
+                  int as, int ao, void* stream);
+/* dw[cout][kp] += d(conv)/dw for output gradient dz (FP32 [npix][dzs]);
+ * db[cout] += sum over pixels (db may be NULL). */
+int pm_nn_conv3x3_wgrad(const void* x, int npix, int H, int W, int cin, int xs,
+                        const float* dz, int cout, int dzs, float* dw, int kp,
+                        float* db, void* stream);
+/* wt[ci][tap * cout + co] = w[co][(8 - tap) * cin + ci], kpt =
+ * pm_nn_conv_kpad(cout): the weights of the input-gradient convolution. */
+int pm_nn_weight_flip(const void* w, int cout, int cin, int kp, void* wt,
+                      int kpt, void* stream);
+/* dz = da * leaky_relu_0.1'(.), the slope taken from the activation a. */
+int pm_nn_lrelu_backward(const float* da, int das, int dao, const void* a,
+                         int as, int ao, float* dz, int npix, int c,
+                         void* stream);
+/* 2x2 max pool (even H, W) into contiguous y; idx records the winner. */
+int pm_nn_maxpool2(const void* x, int xs, int xo, int N, int H, int W, int c,
+                   void* y, uint8_t* idx, void* stream);
+/* dx[winner] += dy. */
+int pm_nn_maxpool2_backward(const float* dy, const uint8_t* idx, int N, int H,
+                            int W, int c, float* dx, int dxs, int dxo,
+                            void* stream);
+/* Nearest 2x upsampling of contiguous x ([N][h][w][c]) into y. */
+int pm_nn_upsample2(const void* x, int N, int h, int w, int c, void* y, int ys,
+                    int yo, void* stream);
+/* dx (contiguous, low resolution) = sum of the 4 children of dy. */
+int pm_nn_upsample2_backward(const float* dy, int dys, int dyo, int N, int h,
+                             int w, int c, float* dx, void* stream);
+/* Initialiser loss of out (FP32 [npix][os], channels t, sx, sy, c) against
+ * target (FP32 [npix][4]: t*, sx*, sy*, valid): Huber(0.02) on t and
+ * w_normal * Huber(0.5) on the slopes over valid pixels (mean over
+ * n_valid), w_conf * BCE(sigmoid(c), valid) (mean over npix).  Writes
+ * dout and adds the three terms to sums[0..2]. */
+int pm_nn_loss(const float* out, int os, const float* target, int npix,
+               int n_valid, float w_normal, float w_conf, float* dout,
+               float* sums, void* stream);
+/* Adam step `step` (1-based, bias-corrected) on FP32 w, refreshing the BF16
+ * copy w16 (may be NULL). */
+int pm_nn_adam(float* w, void* w16, float* m, float* v, const float* g, int n,
+               float lr, float beta1, float beta2, float eps, int step,
+               void* stream);
+int pm_nn_to_bf16(const float* x, void* y, int n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

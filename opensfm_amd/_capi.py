@@ -105,6 +105,28 @@ SIGNATURES = {
     "pm_batch_download_clean": (C.c_int, [_vp, C.c_int, _fp]),
     "pm_batch_stats": (C.c_int, [_vp, C.POINTER(pm_stats)]),
     "pm_batch_clean_ms": (C.c_int, [_vp, _fp]),
+    "pm_nn_conv_kpad": (C.c_int, [C.c_int]),
+    "pm_nn_conv3x3": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int,
+                                C.c_int, _vp, C.c_int, C.c_int, _vp]),
+    "pm_nn_conv3x3_wgrad": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp,
+                                      C.c_int, C.c_int, _vp, C.c_int, _vp, _vp]),
+    "pm_nn_weight_flip": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp]),
+    "pm_nn_lrelu_backward": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
+                                       C.c_int, C.c_int, _vp]),
+    "pm_nn_maxpool2": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 _vp, _vp, _vp]),
+    "pm_nn_maxpool2_backward": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp,
+                                          C.c_int, C.c_int, _vp]),
+    "pm_nn_upsample2": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
+                                  C.c_int, _vp]),
+    "pm_nn_upsample2_backward": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_int, _vp, _vp]),
+    "pm_nn_loss": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_float, C.c_float, _vp,
+                             _vp, _vp]),
+    "pm_nn_adam": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_float, C.c_float,
+                             C.c_float, C.c_float, C.c_int, _vp]),
+    "pm_nn_to_bf16": (C.c_int, [_vp, _vp, C.c_int, _vp]),
 }
 
 _lock = threading.Lock()
