@@ -153,6 +153,41 @@ int pm_estimator_score_hypotheses_fast(pm_estimator* est, int n,
                                        const int32_t* ij, const float* planes,
                                        const int32_t* views, float* out);
 
+/* Learned initialisation (not in the reference; RandomInitialization,
+ * depthmap.cc:237-256, is what it replaces where it is confident).
+ *
+ * pm_estimator_initializer_input writes the initialiser network's input for
+ * the current views into device memory `out` (BF16, out_h x out_w x 40,
+ * NHWC; out_w >= W / scale, out_h >= H / scale): channel 0 the reference
+ * image box-averaged by `scale`, (x - 128) / 64; channels 1..nplanes the
+ * plane-sweep cost volume at that scale -- max-over-views NCC of nplanes
+ * fronto-parallel planes uniform in inverse depth over the depth range,
+ * scored with FP16 samples (-1 where untextured); other channels and the
+ * padding 0.  Needs patch size 5 or 7.  Synchronous.
+ *
+ * pm_estimator_set_init_map gives PatchMatch's initialisation a coarse map
+ * (device memory, h x w cells of `stride` floats: t, sx, sy, c) at 1/scale:
+ * each pixel blends the map bilinearly, and where the confidence logit
+ * c >= 0 starts from depth exp(log_min + t (log_max - log_min)) and normal
+ * (sx, sy, -1) instead of a random plane.  The map must stay valid until
+ * the next compute; NULL restores random initialisation. */
+int pm_estimator_initializer_input(pm_estimator* est, int scale, int nplanes,
+                                   void* out, int out_w, int out_h);
+int pm_estimator_set_init_map(pm_estimator* est, const float* map, int w,
+                              int h, int stride, int scale);
+/* Depth hypotheses tried at a seeded, confident pixel (1..15),
+ * all with the map's normal: the map's depth, then scaled by exp(+-0.03)
+ * and exp(+-0.06) (hypotheses 2..5), then the best so far scaled by
+ * exp(+-0.01), exp(+-0.02), ... (the rest); the best-scoring one starts
+ * PatchMatch.  Default 1 (no search). */
+int pm_estimator_set_init_search(pm_estimator* est, int n);
+/* Per-pixel Gaussian spread around the map's plane at confident pixels
+ * (default 0.05 in log depth, 0.3 in slope; 0, 0: the map's plane itself).
+ * Distinct nearby hypotheses at neighbouring pixels keep PatchMatch's
+ * propagation a search. */
+int pm_estimator_set_init_spread(pm_estimator* est, float sigma_log_depth,
+                                 float sigma_slope);
+
 int pm_estimator_stats(pm_estimator* est, pm_stats* out);
 
 /* ---- DepthmapCleaner -------------------------------------------------- */

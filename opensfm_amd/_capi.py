@@ -75,6 +75,10 @@ SIGNATURES = {
     "pm_estimator_download": (C.c_int, [_vp, _fp, _fp, _fp, _ip]),
     "pm_estimator_score_hypotheses": (C.c_int, [_vp, C.c_int, _ip, _fp, _ip, _fp]),
     "pm_estimator_score_hypotheses_fast": (C.c_int, [_vp, C.c_int, _ip, _fp, _ip, _fp]),
+    "pm_estimator_initializer_input": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int]),
+    "pm_estimator_set_init_map": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "pm_estimator_set_init_search": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_init_spread": (C.c_int, [_vp, C.c_float, C.c_float]),
     "pm_estimator_stats": (C.c_int, [_vp, C.POINTER(pm_stats)]),
     "pm_cleaner_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pm_cleaner_destroy": (C.c_int, [_vp]),

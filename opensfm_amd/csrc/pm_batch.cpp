@@ -209,6 +209,7 @@ void prepare(pm_batch* b, int count) {
     a.n_refine = b->n_refine;
     a.n_explore = b->n_explore;
     a.precision = b->precision;
+    a.init_search = 1;
     a.counter = nullptr;
     T.half_tx = ceil_div((R.width + 1) / 2, tx);
     T.half_ty = ceil_div(R.height, ty);

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <memory>
 #include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -62,6 +63,14 @@ struct pm_estimator {
   int wlut_patch = -1;
 
   std::vector<hipEvent_t> half_ev;  // around the red/black launches
+  // learned initialisation map (caller's device memory) and the coarse
+  // copies of the views its input is computed on
+  const float* init_map = nullptr;
+  int init_w = 0, init_h = 0, init_stride = 0, init_scale = 1, init_search = 1;
+  float init_sigma = 0.05f, init_sigma_slope = 0.3f;
+  int coarse_scale = 0;
+  std::vector<std::unique_ptr<HostView>> coarse;
+  DevBuf coarse_viewtab;
 
   ~pm_estimator() {
     for (auto& e : ev)
@@ -190,6 +199,14 @@ pm::PmArgs prepare(pm_estimator* e) {
   a.n_refine = e->n_refine;
   a.n_explore = e->n_explore;
   a.precision = e->precision;
+  a.init_map = e->init_map;
+  a.init_w = e->init_w;
+  a.init_h = e->init_h;
+  a.init_stride = e->init_stride;
+  a.init_scale = e->init_scale;
+  a.init_search = e->init_search;
+  a.init_sigma = e->init_sigma;
+  a.init_sigma_slope = e->init_sigma_slope;
   a.counter = e->counter.as<unsigned long long>();
   return a;
 }
@@ -321,6 +338,47 @@ int download(pm_estimator* e, float* depth, float* plane, float* score,
                            hipMemcpyDeviceToHost, e->stream));
   HIP_CHECK(hipStreamSynchronize(e->stream));
   return PM_OK;
+}
+
+// Coarse copies of the views at 1/scale (box mean of the image; K scaled
+// about pixel centres, poses unchanged), with FP16 quads for the
+// initialiser's cost volume.  Built once per scale and
+// extended when views are added.
+void build_coarse(pm_estimator* e, int scale) {
+  if (e->coarse_scale != scale) {
+    e->coarse.clear();
+    e->coarse_scale = scale;
+  }
+  for (size_t v = e->coarse.size(); v < e->views.size(); ++v) {
+    const HostView& V = *e->views[v];
+    std::unique_ptr<HostView> c(new HostView());
+    memcpy(c->K, V.K, sizeof(c->K));
+    memcpy(c->R, V.R, sizeof(c->R));
+    memcpy(c->t, V.t, sizeof(c->t));
+    memcpy(c->Q, V.Q, sizeof(c->Q));
+    memcpy(c->a, V.a, sizeof(c->a));
+    // x_coarse = (x + 0.5) / scale - 0.5
+    const double inv = 1.0 / scale;
+    for (int k = 0; k < 3; ++k) {
+      c->K[k] = V.K[k] * inv;
+      c->K[3 + k] = V.K[3 + k] * inv;
+    }
+    c->K[2] += (inv - 1.0) * 0.5 * V.K[8];
+    c->K[5] += (inv - 1.0) * 0.5 * V.K[8];
+    if (!inv3(c->K, c->Kinv)) throw std::runtime_error("coarse K is singular");
+    c->width = V.width / scale;
+    c->height = V.height / scale;
+    const size_t np = (size_t)c->width * c->height;
+    c->img.ensure(np);
+    HIP_CHECK(pm::launch_downsample_u8(V.img.as<uint8_t>(), V.width, V.height, scale, 0,
+                                       c->img.as<uint8_t>(), c->width, c->height, e->stream));
+    c->quad16.ensure((size_t)(c->width + 1) * (c->height + 1) * 8);
+    HIP_CHECK(pm::launch_build_quad16(c->img.as<uint8_t>(), c->quad16.as<uint32_t>(), c->width,
+                                      c->height, PM_PRECISION_FP16, e->stream));
+    c->quad16_kind = PM_PRECISION_FP16;
+    e->coarse.push_back(std::move(c));
+  }
+  HIP_CHECK(hipStreamSynchronize(e->stream));
 }
 
 // ComputePlaneImageScore of fixed hypotheses: the exact FP32 scorer with
@@ -588,6 +646,99 @@ int pm_estimator_score_hypotheses_fast(pm_estimator* e, int n, const int32_t* ij
                                        const float* planes, const int32_t* views,
                                        float* out) {
   return score_hypotheses(e, n, ij, planes, views, out, true);
+}
+
+int pm_estimator_initializer_input(pm_estimator* e, int scale, int nplanes,
+                                   void* out, int out_w, int out_h) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  int rc = validate_for_compute(e);
+  if (rc) return rc;
+  if (!out) return fail(PM_ERR_INVALID, "null output");
+  if (scale < 1 || scale > 16) return fail(PM_ERR_INVALID, "scale must be in [1, 16]");
+  if (nplanes < 2 || nplanes > 39) return fail(PM_ERR_INVALID, "nplanes must be in [2, 39]");
+  if (e->patch_size != 5 && e->patch_size != 7)
+    return fail(PM_ERR_INVALID, "the initialiser input needs patch size 5 or 7");
+  const HostView& r = *e->views[0];
+  const int wl = r.width / scale, hl = r.height / scale;
+  if (wl < e->patch_size || hl < e->patch_size)
+    return fail(PM_ERR_INVALID, "reference view too small for this scale");
+  if (out_w < wl || out_h < hl)
+    return fail(PM_ERR_INVALID, "output grid smaller than the coarse view (" +
+                                    std::to_string(wl) + "x" + std::to_string(hl) + ")");
+  return guarded([&] {
+    DeviceGuard dg(e->device);
+    build_coarse(e, scale);
+    const int n = (int)e->views.size();
+    const HostView& c0 = *e->coarse[0];
+    std::vector<pm::PmView> tab(n);
+    for (int v = 0; v < n; ++v) {
+      const HostView& V = *e->coarse[v];
+      fill_view(tab[v], nullptr, V.width, V.height, V.K, V.Q, V.a, c0.Kinv);
+      tab[v].quad16 = V.quad16.as<uint32_t>();
+    }
+    e->coarse_viewtab.ensure(sizeof(pm::PmView) * n);
+    HIP_CHECK(hipMemcpyAsync(e->coarse_viewtab.p, tab.data(), sizeof(pm::PmView) * n,
+                             hipMemcpyHostToDevice, e->stream));
+    const int hpz = (e->patch_size - 1) / 2;
+    if (e->wlut_patch != e->patch_size) {
+      std::vector<float> t = bilateral_table(hpz);
+      e->wlut.ensure(t.size() * sizeof(float));
+      HIP_CHECK(hipMemcpyAsync(e->wlut.p, t.data(), t.size() * sizeof(float),
+                               hipMemcpyHostToDevice, e->stream));
+      HIP_CHECK(hipStreamSynchronize(e->stream));
+      e->wlut_patch = e->patch_size;
+    }
+    pm::PmArgs a;
+    memset(&a, 0, sizeof(a));
+    a.W = c0.width;
+    a.H = c0.height;
+    a.nviews = n;
+    a.hpz = hpz;
+    a.views = e->coarse_viewtab.as<pm::PmView>();
+    a.ref_img = c0.img.as<uint8_t>();
+    a.ref_mask = nullptr;
+    a.wlut = e->wlut.as<float>();
+    for (int k = 0; k < 9; ++k) {
+      a.Kinv0f[k] = (float)c0.Kinv[k];
+      a.Kinv0[k] = c0.Kinv[k];
+    }
+    a.precision = PM_PRECISION_FP16;
+    const float inv_min = (float)(1.0 / e->min_depth);
+    const float inv_step = (float)((1.0 / e->max_depth - 1.0 / e->min_depth) / (nplanes - 1));
+    HIP_CHECK(pm::launch_init_input(a, nplanes, inv_min, inv_step, out, out_w, out_h, e->stream));
+    HIP_CHECK(hipStreamSynchronize(e->stream));
+    return PM_OK;
+  });
+}
+
+int pm_estimator_set_init_spread(pm_estimator* e, float sigma_log_depth,
+                                 float sigma_slope) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (!(sigma_log_depth >= 0) || !(sigma_slope >= 0) || sigma_log_depth > 1 || sigma_slope > 10)
+    return fail(PM_ERR_INVALID, "spreads must be in [0, 1] and [0, 10]");
+  e->init_sigma = sigma_log_depth;
+  e->init_sigma_slope = sigma_slope;
+  return PM_OK;
+}
+
+int pm_estimator_set_init_search(pm_estimator* e, int n) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (n < 1 || n > 15) return fail(PM_ERR_INVALID, "init search must be in [1, 15]");
+  e->init_search = n;
+  return PM_OK;
+}
+
+int pm_estimator_set_init_map(pm_estimator* e, const float* map, int w, int h,
+                              int stride, int scale) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (map && (w < 1 || h < 1 || stride < 4 || scale < 1))
+    return fail(PM_ERR_INVALID, "init map needs w, h >= 1, stride >= 4, scale >= 1");
+  e->init_map = map;
+  e->init_w = map ? w : 0;
+  e->init_h = map ? h : 0;
+  e->init_stride = map ? stride : 0;
+  e->init_scale = map ? scale : 1;
+  return PM_OK;
 }
 
 int pm_estimator_stats(pm_estimator* e, pm_stats* out) {

@@ -27,6 +27,16 @@ hipError_t launch_pm_init(const PmArgs& a, bool sample, hipStream_t s);
 // queues: 8 zeroed counters (the band queues of one launch).
 hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
                           unsigned* queues, hipStream_t s);
+// Box downsampling (mode 0 mean, 1 min) by an integer factor.
+hipError_t launch_downsample_u8(const uint8_t* in, int W, int H, int sc,
+                                int mode, uint8_t* out, int Wl, int Hl,
+                                hipStream_t s);
+// Learned-initialiser input (image + nplanes-plane NCC volume, BF16 NHWC,
+// 40 channels) over an ow x oh grid for the (coarse) views of `a`; patch
+// sizes 5 and 7.
+hipError_t launch_init_input(const PmArgs& a, int nplanes, float inv_min,
+                             float inv_step, void* out, int ow, int oh,
+                             hipStream_t s);
 hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
                        float* score, int* nghbr, hipStream_t s);
 // Batched launches over the jobs of a PmJob table (see pm_types.h).  `tile`

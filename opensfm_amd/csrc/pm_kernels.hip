@@ -421,9 +421,50 @@ __device__ __forceinline__ unsigned init_pixel(const PmArgs& a, int tile_x,
 
   const u4 r = philox4x32_10(a.seed_lo ^ (uint32_t)idx, a.seed_hi,
                              u4{0xFFFFFFFFu, 0u, 0u, 0u});
-  const float depth = expf(a.log_min + (a.log_max - a.log_min) * u01(r.x));
-  const float n0 = -1.0f + 2.0f * u01(r.y);
-  const float n1 = -1.0f + 2.0f * u01(r.z);
+  float depth = expf(a.log_min + (a.log_max - a.log_min) * u01(r.x));
+  float n0 = -1.0f + 2.0f * u01(r.y);
+  float n1 = -1.0f + 2.0f * u01(r.z);
+  bool seeded = false;
+  if (a.init_map) {
+    // Learned initialisation (not in the reference): the map's
+    // (t, sx, sy, c) blended bilinearly at this pixel's position in the
+    // coarse grid; confident pixels (logit c >= 0) start from depth
+    // exp(log_min + t (log_max - log_min)) and normal (sx, sy, -1), the
+    // others from the random plane above.
+    const float inv = 1.0f / (float)a.init_scale;
+    const float gx = fminf(fmaxf(((float)j + 0.5f) * inv - 0.5f, 0.0f), (float)(a.init_w - 1));
+    const float gy = fminf(fmaxf(((float)i + 0.5f) * inv - 0.5f, 0.0f), (float)(a.init_h - 1));
+    const int x0 = min((int)gx, a.init_w - 2 < 0 ? 0 : a.init_w - 2);
+    const int y0 = min((int)gy, a.init_h - 2 < 0 ? 0 : a.init_h - 2);
+    const int x1 = min(x0 + 1, a.init_w - 1), y1 = min(y0 + 1, a.init_h - 1);
+    const float fx = gx - (float)x0, fy = gy - (float)y0;
+    float m[4];
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) {
+      const float* q = a.init_map + ch;
+      const size_t st = (size_t)a.init_stride;
+      const float v00 = q[((size_t)y0 * a.init_w + x0) * st];
+      const float v01 = q[((size_t)y0 * a.init_w + x1) * st];
+      const float v10 = q[((size_t)y1 * a.init_w + x0) * st];
+      const float v11 = q[((size_t)y1 * a.init_w + x1) * st];
+      const float top = v00 + fx * (v01 - v00), bot = v10 + fx * (v11 - v10);
+      m[ch] = top + fy * (bot - top);
+    }
+    if (m[3] >= 0.0f) {
+      // the map's plane, perturbed per pixel by Gaussians of spread
+      // init_sigma (log depth) and init_sigma_slope, so that neighbouring
+      // pixels hold distinct nearby hypotheses for propagation to choose
+      // among (0: the map's plane itself)
+      float g0, g1, g2, g3;
+      box_muller(r.x, r.y, &g0, &g1);
+      box_muller(r.z, r.w, &g2, &g3);
+      const float t = fminf(fmaxf(m[0], 0.0f), 1.0f);
+      depth = expf(a.log_min + (a.log_max - a.log_min) * t + a.init_sigma * g0);
+      n0 = fminf(fmaxf(m[1] + a.init_sigma_slope * g1, -50.0f), 50.0f);
+      n1 = fminf(fmaxf(m[2] + a.init_sigma_slope * g2, -50.0f), 50.0f);
+      seeded = true;
+    }
+  }
   const PixelRay ray = pixel_ray(a, i, j);
   float v[3];
   plane_from_depth_normal(ray, depth, n0, n1, -1.0f, v);
@@ -441,6 +482,36 @@ __device__ __forceinline__ unsigned init_pixel(const PmArgs& a, int tile_x,
   } else if (textured) {
     s = score_plane_all<P, Prec>(a, rp, ray, i, j, v[0], v[1], v[2], &nb, &dtmp);
     count = a.nviews - 1;
+    // A seeded pixel also searches depth along the seed's normal, coarse
+    // then fine: the seed's depth scaled by exp(+-0.03), exp(+-0.06)
+    // (candidates 2..5 of init_search), then the best so far scaled by
+    // exp(+-0.01), exp(+-0.02), ... (the rest).  The network's depth is
+    // accurate to a few percent; the first refinement steps reach 2 %.
+    if (seeded) {
+      const int n_coarse = min(a.init_search, 5);
+      float best_depth = depth, centre = depth;
+      for (int k = 1; k < a.init_search; ++k) {
+        const bool fine = k >= n_coarse;
+        if (k == n_coarse) centre = best_depth;  // fine stage around the winner
+        const int m = fine ? k - n_coarse : k - 1;
+        const float mag = (fine ? 0.01f : 0.03f) * (float)(m / 2 + 1);
+        const float dk = centre * expf((m & 1) ? -mag : mag);
+        float vk[3];
+        plane_from_depth_normal(ray, dk, n0, n1, -1.0f, vk);
+        int nbk;
+        const float sk = score_plane_all<P, Prec>(a, rp, ray, i, j, vk[0], vk[1], vk[2], &nbk, &dtmp);
+        count += a.nviews - 1;
+        if (sk > s) {
+          s = sk;
+          nb = nbk;
+          v[0] = vk[0];
+          v[1] = vk[1];
+          v[2] = vk[2];
+          best_depth = dk;
+        }
+      }
+      depth = best_depth;
+    }
   }
   reinterpret_cast<float4*>(a.state)[idx] = make_float4(v[0], v[1], v[2], depth);
   a.score[idx] = s;
@@ -453,6 +524,66 @@ __global__ __launch_bounds__(256) void k_pm_init(PmArgs a) {
   const Tile t = row_tile((a.W + 15) >> 4, (a.H + 15) >> 4);
   const unsigned n = t.ok ? init_pixel<P, kSample, Prec>(a, t.x, t.y) : 0u;
   if (a.counter) wave_count(a.counter, n);
+}
+
+// ---------------------------------------------------------------------------
+// Inputs of the learned initialiser (not in the reference).
+
+// Box-filtered (mode 0: mean, rounded; mode 1: min) downsampling of a u8
+// image by an integer factor: out[i][j] over in[s i .. s i + s - 1][s j ..].
+__global__ void k_downsample_u8(const uint8_t* __restrict__ in, int W, int H,
+                                int sc, int mode, uint8_t* __restrict__ out,
+                                int Wl, int Hl) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= Hl || j >= Wl) return;
+  int acc = mode ? 255 : 0;
+  for (int u = 0; u < sc; ++u)
+    for (int v = 0; v < sc; ++v) {
+      const int val = in[(size_t)(i * sc + u) * W + (j * sc + v)];
+      acc = mode ? min(acc, val) : acc + val;
+    }
+  out[(size_t)i * Wl + j] =
+      (uint8_t)(mode ? acc : (acc + (sc * sc) / 2) / (sc * sc));
+}
+
+// The network input at one coarse pixel, NHWC BF16 with 40 channels:
+// channel 0 the image (x - 128) / 64, channels 1..nplanes the max-over-views
+// NCC of fronto-parallel planes at depths uniform in inverse depth over
+// [min, max] (the plane sweep of ComputeBruteForce, depthmap.cc:184-200,
+// scored by the PatchMatch scorer with the given sample precision; -1 for
+// untextured or border pixels), the rest 0.  Pixels of the padded grid
+// beyond the image are all 0.
+template <int P, int Prec>
+__global__ __launch_bounds__(256) void k_init_input(PmArgs a, int nplanes,
+                                                    float inv_min,
+                                                    float inv_step,
+                                                    __bf16* __restrict__ out,
+                                                    int ow, int oh) {
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i >= oh || j >= ow) return;
+  constexpr int C = 40;
+  __bf16* o = out + ((size_t)i * ow + j) * C;
+  const bool inside = i < a.H && j < a.W;
+  const bool interior = inside && is_interior(a, i, j);
+  RefPatch<P> rp;
+  const bool textured =
+      interior && load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut);
+  o[0] = (__bf16)(inside ? ((float)a.ref_img[(size_t)i * a.W + j] - 128.0f) / 64.0f : 0.0f);
+  const PixelRay ray = pixel_ray(a, i, j);
+  for (int d = 0; d < nplanes; ++d) {
+    float sc = inside ? -1.0f : 0.0f;
+    if (textured) {
+      const float depth = 1.0f / (inv_min + (float)d * inv_step);
+      int nb;
+      float dep;
+      sc = score_plane_all<P, Prec>(a, rp, ray, i, j, 0.0f, 0.0f, -1.0f / depth, &nb, &dep);
+      sc = sc == sc ? sc : -1.0f;
+    }
+    o[1 + d] = (__bf16)sc;
+  }
+  for (int ch = 1 + nplanes; ch < C; ++ch) o[ch] = (__bf16)0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1082,6 +1213,26 @@ hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
   PM_DISPATCH_PREC(a.precision, P_,
                    return (half_banded<P, false, Prec>(a, colour, iter, queues, s)));
   return hipSuccess;
+}
+
+hipError_t launch_downsample_u8(const uint8_t* in, int W, int H, int sc,
+                                int mode, uint8_t* out, int Wl, int Hl,
+                                hipStream_t s) {
+  dim3 blk(16, 16);
+  dim3 grd((Wl + 15) / 16, (Hl + 15) / 16);
+  hipLaunchKernelGGL(k_downsample_u8, grd, blk, 0, s, in, W, H, sc, mode, out, Wl, Hl);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_input(const PmArgs& a, int nplanes, float inv_min,
+                             float inv_step, void* out, int ow, int oh,
+                             hipStream_t s) {
+  const int P_ = 2 * a.hpz + 1;
+  dim3 grd((ow + 15) / 16, (oh + 15) / 16);
+  __bf16* o = static_cast<__bf16*>(out);
+  PM_DISPATCH_P57(P_, hipLaunchKernelGGL((k_init_input<P, 1>), grd, dim3(256), 0, s, a,
+                                         nplanes, inv_min, inv_step, o, ow, oh));
+  return hipGetLastError();
 }
 
 hipError_t launch_score_hyp_fast(const PmArgs& a, int view, int n,

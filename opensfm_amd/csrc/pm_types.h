@@ -43,6 +43,13 @@ struct PmArgs {
   int n_refine;        // random refinements per pixel update (6-step schedule)
   int n_explore;       // exploration samples per pixel update (0: none)
   int precision;       // PatchMatch bilinear samples: 0 FP32, 1 FP16, 2 BF16
+  // Learned initialisation map (nullptr: random initialisation only):
+  // init_h x init_w cells of init_stride floats (t, sx, sy, c logit, ...),
+  // cell (gx, gy) centred on pixel ((gx + 0.5) init_scale - 0.5, ...).
+  const float* init_map;
+  int init_w, init_h, init_stride, init_scale;
+  int init_search;     // depth hypotheses tried at a seeded pixel (>= 1)
+  float init_sigma, init_sigma_slope;  // per-pixel spread around the map
   unsigned long long* counter;  // scorings executed (nullptr: not counted)
 };
 

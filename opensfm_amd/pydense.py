@@ -152,6 +152,30 @@ class DepthmapEstimator:
                        _capi.ptr(views, C.c_int32), _capi.ptr(out, C.c_float)))
         return out
 
+    def initializer_input(self, out_ptr: int, out_w: int, out_h: int, scale: int = 4,
+                          nplanes: int = 32) -> None:
+        """Writes the learned initialiser's input (BF16 NHWC, 40 channels) to
+        device memory at out_ptr."""
+        _capi.check(self._lib.pm_estimator_initializer_input(
+            self._h, int(scale), int(nplanes), C.c_void_p(out_ptr), int(out_w), int(out_h)))
+
+    def set_init_map(self, map_ptr, w: int = 0, h: int = 0, stride: int = 8,
+                     scale: int = 4) -> None:
+        """Seeds PatchMatch's initialisation from a coarse (t, sx, sy, c) map
+        in device memory (None: random initialisation)."""
+        _capi.check(self._lib.pm_estimator_set_init_map(
+            self._h, C.c_void_p(map_ptr) if map_ptr else None, int(w), int(h), int(stride),
+            int(scale)))
+
+    def set_init_search(self, n: int) -> None:
+        """Depth hypotheses tried around the init map's depth (1..15)."""
+        _capi.check(self._lib.pm_estimator_set_init_search(self._h, int(n)))
+
+    def set_init_spread(self, sigma_log_depth: float, sigma_slope: float) -> None:
+        """Per-pixel spread of the seeded planes around the init map's."""
+        _capi.check(self._lib.pm_estimator_set_init_spread(self._h, float(sigma_log_depth),
+                                                           float(sigma_slope)))
+
     def stats(self) -> dict:
         s = _capi.pm_stats()
         _capi.check(self._lib.pm_estimator_stats(self._h, C.byref(s)))

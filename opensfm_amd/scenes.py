@@ -236,6 +236,7 @@ class SceneGeometry:
     boxes: list                  # list of (lo[3], hi[3])
     ground: bool = True
     terrain: tuple | None = None  # (amplitude, period) sinusoidal heightfield
+    heightfield: object = None    # or any object with amp, height(x, y), grad(x, y)
     tex_spacing: float = 0.15
     tex_seed: int = 0
     light: tuple = (0.4, 0.3, 0.866)
@@ -296,7 +297,11 @@ def render(geom: SceneGeometry, K, R, t, width, height):
             best_t[c] = th[better]
             best_n[:, c] = nh[:, better]
 
-    if geom.terrain is not None:
+    if geom.heightfield is not None:
+        hf = geom.heightfield
+        take(*_hit_heightfield(o, d, hf.height, hf.grad, -hf.amp, hf.amp,
+                               steps=getattr(hf, "steps", 48), bisect=getattr(hf, "bisect", 12)))
+    elif geom.terrain is not None:
         amp = geom.terrain[0]
         take(*_hit_heightfield(o, d, geom.height, geom.grad, -amp, amp))
     elif geom.ground:
@@ -454,3 +459,104 @@ def street_scene(n_views, seed=4000, sizes=STREET_SIZES, neighbours=(4, 12),
         views = list(ex.map(one, range(n_views)))
     return BatchScene(views=views, jobs=jobs, min_depth=min_depth,
                       max_depth=max_depth, name=name)
+
+
+# --------------------------------------------------------------------------
+# nadir aerial views over fractal-noise terrain
+
+
+def _value_noise2(x, y, seed):
+    """2-D uniform lattice noise in [0, 1), smoothstep-bilinear."""
+    ix, iy = np.floor(x), np.floor(y)
+    fx, fy = x - ix, y - iy
+    fx = fx * fx * (3 - 2 * fx)
+    fy = fy * fy * (3 - 2 * fy)
+    ix, iy = ix.astype(np.int64), iy.astype(np.int64)
+    z = np.zeros_like(ix)
+    with np.errstate(over="ignore"):
+        v00 = _hash3(ix, iy, z, seed)
+        v10 = _hash3(ix + 1, iy, z, seed)
+        v01 = _hash3(ix, iy + 1, z, seed)
+        v11 = _hash3(ix + 1, iy + 1, z, seed)
+    top = v00 + fx * (v10 - v00)
+    bot = v01 + fx * (v11 - v01)
+    return (top + fy * (bot - top)).astype(x.dtype)
+
+
+class FractalTerrain:
+    """Heightfield amp * sum_o 0.5^o (2 noise(2^o p / period) - 1) / norm
+    (within [-amp, amp]), tabulated on a `res`-metre grid over
+    [-extent, extent]^2 and read bilinearly (constant beyond the grid)."""
+
+    steps, bisect = 12, 10  # marching for near-vertical (nadir) rays
+
+    def __init__(self, seed, amp=6.0, period=24.0, octaves=5, extent=80.0, res=0.1):
+        self.amp, self.extent, self.res = float(amp), float(extent), float(res)
+        n = int(round(2 * extent / res)) + 1
+        g = np.linspace(-extent, extent, n)
+        X, Y = np.meshgrid(g, g)
+        h = np.zeros_like(X)
+        norm = sum(0.5 ** o for o in range(octaves))
+        for o in range(octaves):
+            f = (2.0 ** o) / period
+            h += 0.5 ** o * (2.0 * _value_noise2(X * f, Y * f, int(seed) + o) - 1.0)
+        self.grid = (h * (amp / norm)).astype(np.float32)
+        self.n = n
+
+    def _cell(self, x, y):
+        gx = np.clip((np.asarray(x, np.float64) + self.extent) / self.res, 0, self.n - 1.001)
+        gy = np.clip((np.asarray(y, np.float64) + self.extent) / self.res, 0, self.n - 1.001)
+        ix, iy = gx.astype(np.int64), gy.astype(np.int64)
+        return ix, iy, gx - ix, gy - iy
+
+    def height(self, x, y):
+        ix, iy, fx, fy = self._cell(x, y)
+        g = self.grid
+        top = g[iy, ix] + fx * (g[iy, ix + 1] - g[iy, ix])
+        bot = g[iy + 1, ix] + fx * (g[iy + 1, ix + 1] - g[iy + 1, ix])
+        return (top + fy * (bot - top)).astype(np.asarray(x).dtype)
+
+    def grad(self, x, y):
+        ix, iy, fx, fy = self._cell(x, y)
+        g = self.grid
+        dx = ((1 - fy) * (g[iy, ix + 1] - g[iy, ix]) + fy * (g[iy + 1, ix + 1] - g[iy + 1, ix]))
+        dy = ((1 - fx) * (g[iy + 1, ix] - g[iy, ix]) + fx * (g[iy + 1, ix + 1] - g[iy, ix + 1]))
+        return dx / self.res, dy / self.res
+
+
+def nadir_scene(rows, cols, width, height, seed=5000, neighbours=10, altitude=40.0,
+                spacing=6.0, amp=6.0, min_depth=20.0, max_depth=60.0, name="nadir"):
+    """rows x cols cameras looking straight down from `altitude` over fractal
+    terrain, on a grid of `spacing` metres; every camera is a job whose
+    neighbours are its `neighbours` nearest cameras."""
+    rng = np.random.default_rng(seed)
+    K = intrinsics(width, height)
+    geom = SceneGeometry(boxes=[], ground=False,
+                         heightfield=FractalTerrain(
+                             int(rng.integers(1, 2 ** 40)), amp=amp,
+                             extent=max(rows, cols) * spacing / 2 + 0.75 * altitude + 5.0),
+                         tex_spacing=4.0 * altitude / K[0, 0],
+                         tex_seed=int(rng.integers(1, 2 ** 62)))
+    centres = []
+    for r in range(rows):
+        for c in range(cols):
+            centres.append(np.array([(c - (cols - 1) / 2) * spacing + rng.uniform(-1, 1),
+                                     (r - (rows - 1) / 2) * spacing + rng.uniform(-1, 1),
+                                     altitude + rng.uniform(-2, 2)]))
+    poses = [look_at(c, c - np.array([0.0, 0.0, 1.0]), up=(0.0, 1.0, 0.0)) for c in centres]
+    n = len(poses)
+    jobs = []
+    for k in range(n):
+        dist = [np.linalg.norm(centres[k][:2] - centres[v][:2]) for v in range(n)]
+        order = [int(v) for v in np.argsort(dist, kind="stable") if v != k]
+        jobs.append((k, order[:neighbours]))
+
+    def one(k):
+        img, dep = render(geom, K, poses[k][0], poses[k][1], width, height)
+        return View(K=K.copy(), R=poses[k][0], t=poses[k][1], image=img,
+                    mask=np.full_like(img, 255), depth=dep)
+
+    with concurrent.futures.ThreadPoolExecutor(_threads()) as ex:
+        views = list(ex.map(one, range(n)))
+    return BatchScene(views=views, jobs=jobs, min_depth=min_depth, max_depth=max_depth,
+                      name=name)
