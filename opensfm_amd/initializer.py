@@ -118,14 +118,14 @@ def scene_views(scene, ref, nb):
              scene.views[v].mask) for v in [ref] + list(nb)]
 
 
-def training_scenes(n, width, height, seed):
+def training_scenes(n, width, height, seed, nadir_fraction=1 / 3):
     """Arc scenes of boxes on terrain or a ground plane (the single-view
     family) and nadir fractal-terrain scenes, as (views, min, max) jobs."""
     rng = np.random.default_rng(seed)
     jobs = []
     for k in range(n):
         s = int(rng.integers(1, 2 ** 31))
-        if k % 3 == 2:
+        if rng.random() < nadir_fraction:
             sc = scenes.nadir_scene(2, 3, width, height, seed=s, neighbours=5,
                                     altitude=float(rng.uniform(30, 50)))
             ref, nb = sc.jobs[int(rng.integers(0, len(sc.jobs)))]
@@ -185,7 +185,7 @@ def train(init, xs, ts, grid, steps=1500, batch=8, lr=2e-3, w_normal=0.05, w_con
 
 
 MODES = {"plain_5": (False, 5), "plain_2": (False, 2), "plain_1": (False, 1),
-         "learned_0": (True, 0), "learned_1": (True, 1), "learned_2": (True, 2)}
+         "learned_1": (True, 1), "learned_2": (True, 2)}
 
 
 def evaluate(init, jobs, modes=MODES):
@@ -234,10 +234,11 @@ def main(argv=None):
     ap.add_argument("--scenes", type=int, default=200)
     ap.add_argument("--eval-scenes", type=int, default=12)
     ap.add_argument("--size", type=int, nargs=2, default=(320, 256))
-    ap.add_argument("--eval-size", type=int, nargs=2, default=(960, 768))
+    ap.add_argument("--eval-size", type=int, nargs=2, default=(1000, 752))
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--seed", type=int, default=123)
+    ap.add_argument("--nadir-fraction", type=float, default=0.67)
     args = ap.parse_args(argv)
 
     def log(m):
@@ -247,7 +248,7 @@ def main(argv=None):
     init = LearnedInitializer(seed=args.seed)
     if args.train:
         log(f"rendering {args.scenes} training scenes of {W}x{H}")
-        jobs = training_scenes(args.scenes, W, H, args.seed)
+        jobs = training_scenes(args.scenes, W, H, args.seed, args.nadir_fraction)
         log("PatchMatch targets")
         xs, ts, grid = build_samples(init, jobs)
         log(f"training {init.net.n_params} parameters for {args.steps} steps")
@@ -261,9 +262,13 @@ def main(argv=None):
         init = LearnedInitializer(weights=args.out)
         hist = None
     EW, EH = args.eval_size
-    log(f"rendering {args.eval_scenes} evaluation scenes of {EW}x{EH}")
-    ev_jobs = training_scenes(args.eval_scenes, EW, EH, args.seed + 999)
-    res = evaluate(init, ev_jobs)
+    log(f"rendering {args.eval_scenes} nadir and {args.eval_scenes} arc evaluation views "
+        f"of {EW}x{EH}")
+    sc = scenes.nadir_scene(4, 4, EW, EH, seed=args.seed + 777, neighbours=10)
+    nadir = [(scene_views(sc, r, nb), sc.min_depth, sc.max_depth, sc.views[r])
+             for r, nb in sc.jobs[:args.eval_scenes]]
+    arc = training_scenes(args.eval_scenes, EW, EH, args.seed + 999, nadir_fraction=0.0)
+    res = {"nadir": evaluate(init, nadir), "arc": evaluate(init, arc)}
     print(json.dumps({"evaluation": res, "params": init.net.n_params,
                       "loss_history": hist}), flush=True)
 
