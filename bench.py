@@ -209,6 +209,27 @@ def batch_bench(n_views, steps=3, seed=4000):
     }
 
 
+def learned_bench(width=2000, height=1504, views=4, seed=5000):
+    """Learned initialiser vs plain PatchMatch on nadir views of fractal
+    terrain (a 4 x 4 camera grid, 10 neighbours each; the first `views`
+    reference views timed): network input + network + seeded PatchMatch
+    against PatchMatch from random planes, same GPU, same settings."""
+    from opensfm_amd import initializer as I, scenes
+    log(f"learned initialiser: rendering 16 nadir views of {width}x{height}")
+    sc = scenes.nadir_scene(4, 4, width, height, seed=seed, neighbours=10)
+    jobs = [(I.scene_views(sc, r, nb), sc.min_depth, sc.max_depth, sc.views[r])
+            for r, nb in sc.jobs[:views]]
+    init = I.LearnedInitializer(weights=I.WEIGHTS)
+    res = I.evaluate(init, jobs)
+    out = {"workload": f"{views} nadir views {width}x{height}, 10 neighbours, patch 7",
+           "params": init.net.n_params}
+    out.update(res)
+    base = res["plain_5"]["mpix_s"]
+    out["learned_1_over_plain_5"] = round(res["learned_1"]["mpix_s"] / base, 2)
+    out["learned_2_over_plain_5"] = round(res["learned_2"]["mpix_s"] / base, 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +240,8 @@ def main():
     ap.add_argument("--batch-views", type=int, default=16, metavar="N",
                     help="also run the batched street workload with N reference "
                          "views (0: skip; 128 is the full batched configuration)")
+    ap.add_argument("--learned-views", type=int, default=4, metavar="N",
+                    help="also time the learned initialiser on N nadir views (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (torch) so that "
                          "profiler byte counters can be scaled against a known size")
@@ -371,6 +394,9 @@ def main():
     if args.batch_views > 0:
         out["batched"] = batch_bench(args.batch_views)
         log(f"batched: {out['batched']}")
+    if args.learned_views > 0:
+        out["learned_init"] = learned_bench(views=args.learned_views)
+        log(f"learned initialiser: {out['learned_init']}")
     print(json.dumps(out), flush=True)
 
 

@@ -148,6 +148,14 @@ def load():
     with _lock:
         if _lib is not None:
             return _lib
+        # PyTorch (which the learned initialiser uses for device memory)
+        # ships its own HIP runtime; loading it first makes the engine bind
+        # to that same runtime instead of bringing up a second one in the
+        # process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         path = library_path()
         if not os.path.exists(path):
             _build.build()

@@ -200,8 +200,9 @@ def evaluate(init, jobs, modes=MODES):
         for mode, (learned, iters) in modes.items():
             e = make_estimator(views, dmin, dmax, seed=7000 + k)
             e.set_patchmatch_iterations(iters)
-            if learned:  # coarse views built outside the timed region, like uploads
-                init.network_input(e, W, H)
+            if learned:  # coarse views and the network warmed outside the timed region
+                init.attach(e, W, H)
+                init.detach(e)
             e.run(1)  # warm
             torch.cuda.synchronize()
             t0 = time.perf_counter()
