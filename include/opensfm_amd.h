@@ -210,6 +210,43 @@ int pm_cleaner_clean(pm_cleaner* cl, float* clean_depth);
 /* HIP-event time of the last clean kernel, milliseconds. */
 int pm_cleaner_last_ms(pm_cleaner* cl, float* ms);
 
+/* ---- GPUDepthmapCleaner ----------------------------------------------- */
+
+/* The live pipeline's cleaner (opensfm/src/lib/dense/cleaner.h:16-63,
+ * cleaner.cc:56-179, kernel acmmp_clean_depthmap in opencl_kernels.h:
+ * 2155-2273): a pixel of view ref_idx keeps its depth when at least
+ * min(min_consistent_views, 1 + n) views (itself included) agree -- the
+ * neighbour's bilinear depth at the projection within the threshold of the
+ * projected depth, and the point back-projected from that depth landing
+ * within 2 px (L1) and the threshold in depth in the reference view -- and
+ * fewer than max_carved_views neighbours see more than (1 + carving
+ * threshold) times further along the ray.  FP32 throughout.  At most 16
+ * neighbours are used; a neighbour of another size is resized to the
+ * reference's (cv::resize INTER_LINEAR) with its K unchanged, as the
+ * reference does.  Defaults: threshold 0.01, 3 views, carving 0.2, 1
+ * carved view. */
+typedef struct pm_gpu_cleaner pm_gpu_cleaner;
+int pm_gpu_cleaner_create(int device, pm_gpu_cleaner** out);
+int pm_gpu_cleaner_destroy(pm_gpu_cleaner* c);
+int pm_gpu_cleaner_set_same_depth_threshold(pm_gpu_cleaner* c, float t);
+int pm_gpu_cleaner_set_min_consistent_views(pm_gpu_cleaner* c, int n);
+/* negative values clamp to 0 (cleaner.cc:30-36) */
+int pm_gpu_cleaner_set_carving_threshold(pm_gpu_cleaner* c, float t);
+int pm_gpu_cleaner_set_max_carved_views(pm_gpu_cleaner* c, int n);
+/* GPUDepthmapCleaner::AddView: *index receives the view's index. */
+int pm_gpu_cleaner_add_view(pm_gpu_cleaner* c, const double* K,
+                            const double* R, const double* t,
+                            const float* depth, int width, int height,
+                            int* index);
+/* GPUDepthmapCleaner::Clean into clean_depth (view ref_idx's H*W f32);
+ * indices out of range give PM_ERR_RANGE. */
+int pm_gpu_cleaner_clean(pm_gpu_cleaner* c, int ref_idx,
+                         const int32_t* neighbor_ids, int n,
+                         float* clean_depth);
+/* GPUDepthmapCleaner::Clear */
+int pm_gpu_cleaner_clear(pm_gpu_cleaner* c);
+int pm_gpu_cleaner_last_ms(pm_gpu_cleaner* c, float* ms);
+
 /* ---- Batched estimation ---------------------------------------------- */
 
 /* Many reference views ("jobs") of one pool of views, run together: each

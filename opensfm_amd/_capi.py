@@ -87,6 +87,16 @@ SIGNATURES = {
     "pm_cleaner_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _fp, C.c_int, C.c_int]),
     "pm_cleaner_clean": (C.c_int, [_vp, _fp]),
     "pm_cleaner_last_ms": (C.c_int, [_vp, _fp]),
+    "pm_gpu_cleaner_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "pm_gpu_cleaner_destroy": (C.c_int, [_vp]),
+    "pm_gpu_cleaner_set_same_depth_threshold": (C.c_int, [_vp, C.c_float]),
+    "pm_gpu_cleaner_set_min_consistent_views": (C.c_int, [_vp, C.c_int]),
+    "pm_gpu_cleaner_set_carving_threshold": (C.c_int, [_vp, C.c_float]),
+    "pm_gpu_cleaner_set_max_carved_views": (C.c_int, [_vp, C.c_int]),
+    "pm_gpu_cleaner_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _fp, C.c_int, C.c_int, _ip]),
+    "pm_gpu_cleaner_clean": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _fp]),
+    "pm_gpu_cleaner_clear": (C.c_int, [_vp]),
+    "pm_gpu_cleaner_last_ms": (C.c_int, [_vp, _fp]),
     "pm_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pm_batch_destroy": (C.c_int, [_vp]),
     "pm_batch_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _u8p, _u8p, C.c_int, C.c_int,

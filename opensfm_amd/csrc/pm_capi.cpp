@@ -458,6 +458,34 @@ int score_hypotheses(pm_estimator* e, int n, const int32_t* ij,
 
 }  // namespace
 
+// ===========================================================================
+// GPUDepthmapCleaner (opensfm/src/lib/dense/cleaner.h:16-63)
+
+struct GcView {
+  double K[9], R[9], t[3];
+  int width, height;
+  DevBuf depth;
+};
+
+struct pm_gpu_cleaner {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  // defaults of cleaner.h:47-50
+  float same_depth_threshold = 0.01f;
+  int min_consistent_views = 3;
+  float carving_threshold = 0.2f;
+  int max_carved_views = 1;
+  std::vector<std::unique_ptr<GcView>> views;
+  DevBuf resized, ptrs, cams, out;
+  float last_ms = 0;
+  ~pm_gpu_cleaner() {
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
 extern "C" {
 
 const char* pm_last_error(void) { return g_last_error.c_str(); }
@@ -882,6 +910,167 @@ int pm_cleaner_clean(pm_cleaner* c, float* clean_depth) {
 }
 
 int pm_cleaner_last_ms(pm_cleaner* c, float* ms) {
+  if (!c || !ms) return fail(PM_ERR_INVALID, "null argument");
+  *ms = c->last_ms;
+  return PM_OK;
+}
+
+
+int pm_gpu_cleaner_create(int device, pm_gpu_cleaner** out) {
+  if (!out) return fail(PM_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  int rc = check_device(device);
+  if (rc) return rc;
+  return guarded([&] {
+    std::unique_ptr<pm_gpu_cleaner> c(new pm_gpu_cleaner());
+    c->device = device;
+    DeviceGuard dg(device);
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
+    *out = c.release();
+    return PM_OK;
+  });
+}
+
+int pm_gpu_cleaner_destroy(pm_gpu_cleaner* c) {
+  if (!c) return PM_OK;
+  return guarded([&] {
+    DeviceGuard dg(c->device);
+    delete c;
+    return PM_OK;
+  });
+}
+
+int pm_gpu_cleaner_set_same_depth_threshold(pm_gpu_cleaner* c, float t) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  c->same_depth_threshold = t;
+  return PM_OK;
+}
+
+int pm_gpu_cleaner_set_min_consistent_views(pm_gpu_cleaner* c, int n) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  c->min_consistent_views = n;
+  return PM_OK;
+}
+
+int pm_gpu_cleaner_set_carving_threshold(pm_gpu_cleaner* c, float t) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  c->carving_threshold = t > 0.0f ? t : 0.0f;  // cleaner.cc:30-32
+  return PM_OK;
+}
+
+int pm_gpu_cleaner_set_max_carved_views(pm_gpu_cleaner* c, int n) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  c->max_carved_views = n > 0 ? n : 0;  // cleaner.cc:34-36
+  return PM_OK;
+}
+
+int pm_gpu_cleaner_add_view(pm_gpu_cleaner* c, const double* K, const double* R,
+                            const double* t, const float* depth, int width,
+                            int height, int* index) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  if (!K || !R || !t || !depth) return fail(PM_ERR_INVALID, "null K, R, t or depth");
+  if (width < 2 || height < 2) return fail(PM_ERR_INVALID, "depth map must be at least 2x2");
+  return guarded([&] {
+    std::unique_ptr<GcView> v(new GcView());
+    memcpy(v->K, K, sizeof(v->K));
+    memcpy(v->R, R, sizeof(v->R));
+    memcpy(v->t, t, sizeof(v->t));
+    v->width = width;
+    v->height = height;
+    DeviceGuard dg(c->device);
+    const size_t bytes = (size_t)width * height * sizeof(float);
+    v->depth.ensure(bytes);
+    HIP_CHECK(hipMemcpyAsync(v->depth.p, depth, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (index) *index = (int)c->views.size();
+    c->views.push_back(std::move(v));
+    return PM_OK;
+  });
+}
+
+int pm_gpu_cleaner_clean(pm_gpu_cleaner* c, int ref_idx, const int32_t* neighbor_ids,
+                         int n, float* clean_depth) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  if (!clean_depth) return fail(PM_ERR_INVALID, "null output");
+  if (n < 0 || (n > 0 && !neighbor_ids)) return fail(PM_ERR_INVALID, "neighbour list");
+  const int nv = (int)c->views.size();
+  // cleaner.cc:68-81
+  if (ref_idx < 0 || ref_idx >= nv)
+    return fail(PM_ERR_RANGE, "Clean: ref_idx " + std::to_string(ref_idx) +
+                                  " out of range [0, " + std::to_string(nv) + ")");
+  for (int i = 0; i < n; ++i)
+    if (neighbor_ids[i] < 0 || neighbor_ids[i] >= nv)
+      return fail(PM_ERR_RANGE, "Clean: neighbor_ids[" + std::to_string(i) + "] = " +
+                                    std::to_string(neighbor_ids[i]) + " out of range [0, " +
+                                    std::to_string(nv) + ")");
+  return guarded([&] {
+    DeviceGuard dg(c->device);
+    const GcView& R = *c->views[ref_idx];
+    const int w = R.width, h = R.height;
+    const size_t np = (size_t)w * h;
+    const int nsrc = std::min(n, 16);  // MAX_CLEAN_SOURCES, opencl_kernels.h:2103
+    c->resized.ensure(np * sizeof(float) * std::max(1, nsrc));
+    std::vector<const float*> ptrs(std::max(1, nsrc), nullptr);
+    std::vector<pm::GcCamera> cams(1 + nsrc);
+    auto cam = [](const GcView& V, pm::GcCamera& g) {
+      for (int k = 0; k < 9; ++k) {
+        g.K[k] = (float)V.K[k];
+        g.R[k] = (float)V.R[k];
+      }
+      for (int k = 0; k < 3; ++k) g.t[k] = (float)V.t[k];
+    };
+    cam(R, cams[0]);
+    for (int v = 0; v < nsrc; ++v) {
+      const GcView& S = *c->views[neighbor_ids[v]];
+      float* dst = c->resized.as<float>() + (size_t)v * np;
+      if (S.width == w && S.height == h) {
+        HIP_CHECK(hipMemcpyAsync(dst, S.depth.p, np * sizeof(float), hipMemcpyDeviceToDevice,
+                                 c->stream));
+      } else {
+        HIP_CHECK(pm::launch_resize_linear(S.depth.as<float>(), S.width, S.height, dst, w, h,
+                                           c->stream));
+      }
+      ptrs[v] = dst;
+      cam(S, cams[1 + v]);
+    }
+    c->ptrs.ensure(ptrs.size() * sizeof(float*));
+    c->cams.ensure(cams.size() * sizeof(pm::GcCamera));
+    c->out.ensure(np * sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(c->ptrs.p, ptrs.data(), ptrs.size() * sizeof(float*),
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->cams.p, cams.data(), cams.size() * sizeof(pm::GcCamera),
+                             hipMemcpyHostToDevice, c->stream));
+    const int eff_min = std::min(c->min_consistent_views, 1 + n);  // cleaner.cc:152
+    HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
+    HIP_CHECK(pm::launch_gpu_clean(R.depth.as<float>(), c->ptrs.as<const float*>(),
+                                   c->cams.as<pm::GcCamera>(), w, h, nsrc,
+                                   c->same_depth_threshold, eff_min, c->carving_threshold,
+                                   c->max_carved_views, c->out.as<float>(), c->stream));
+    HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
+    HIP_CHECK(hipMemcpyAsync(clean_depth, c->out.p, np * sizeof(float), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms, c->ev[0], c->ev[1]));
+    return PM_OK;
+  });
+}
+
+int pm_gpu_cleaner_clear(pm_gpu_cleaner* c) {
+  if (!c) return fail(PM_ERR_INVALID, "null cleaner");
+  return guarded([&] {
+    DeviceGuard dg(c->device);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->views.clear();
+    c->resized.release();
+    c->ptrs.release();
+    c->cams.release();
+    c->out.release();
+    return PM_OK;
+  });
+}
+
+int pm_gpu_cleaner_last_ms(pm_gpu_cleaner* c, float* ms) {
   if (!c || !ms) return fail(PM_ERR_INVALID, "null argument");
   *ms = c->last_ms;
   return PM_OK;

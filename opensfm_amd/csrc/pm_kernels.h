@@ -57,6 +57,14 @@ hipError_t launch_pack_batch(const PmJob* jobs, int njobs, int total,
 hipError_t launch_clean_batch(const ClJob* jobs, int njobs, int total,
                               const ClView* views, float thr, int min_views,
                               hipStream_t s);
+// GPUDepthmapCleaner: cv::resize(INTER_LINEAR) of a float map, and the
+// carving/backward-check clean of ref_depth against nsrc resized maps.
+hipError_t launch_resize_linear(const float* src, int sw, int sh, float* dst,
+                                int w, int h, hipStream_t s);
+hipError_t launch_gpu_clean(const float* ref_depth, const float* const* src,
+                            const GcCamera* cams, int w, int h, int nsrc,
+                            float thr, int min_views, float carving,
+                            int max_carved, float* out, hipStream_t s);
 hipError_t launch_clean(const ClView* views, const ClRef* ref, int n, int W,
                         int H, float thr, int min_views, float* out,
                         hipStream_t s);

@@ -1101,6 +1101,114 @@ __global__ __launch_bounds__(256) void k_clean_batch(
 }
 
 // ---------------------------------------------------------------------------
+// GPUDepthmapCleaner::Clean (cleaner.cc:56-179): forward depth check with
+// space carving and a backward reprojection check, FP32 throughout, in the
+// operation order of its kernel (opencl_kernels.h:2155-2273).  Neighbour
+// depths come resized to the reference size (cv::resize INTER_LINEAR,
+// cleaner.cc:114-117) while their K stay as given, as the reference does.
+
+// cv::resize(INTER_LINEAR) of a float map: source coordinate
+// (x + 0.5) * sw / w - 0.5, floor and fraction, clamped at the borders.
+__global__ void k_resize_linear(const float* __restrict__ src, int sw, int sh,
+                                float* __restrict__ dst, int w, int h) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= w || y >= h) return;
+  auto axis = [](int d, int n_dst, int n_src, int* i0, int* i1, float* a) {
+    const float f = (float)(((double)d + 0.5) * ((double)n_src / n_dst) - 0.5);
+    int i = (int)floorf(f);
+    float fr = f - (float)i;
+    if (i < 0) {
+      fr = 0.0f;
+      i = 0;
+    }
+    if (i >= n_src - 1) {
+      fr = 0.0f;
+      i = n_src - 1;
+    }
+    *i0 = i;
+    *i1 = min(i + 1, n_src - 1);
+    *a = fr;
+  };
+  int x0, x1, y0, y1;
+  float ax, ay;
+  axis(x, w, sw, &x0, &x1, &ax);
+  axis(y, h, sh, &y0, &y1, &ay);
+  const float* r0 = src + (size_t)y0 * sw;
+  const float* r1 = src + (size_t)y1 * sw;
+  const float top = r0[x0] * (1.0f - ax) + r0[x1] * ax;
+  const float bot = r1[x0] * (1.0f - ax) + r1[x1] * ax;
+  dst[(size_t)y * w + x] = top * (1.0f - ay) + bot * ay;
+}
+
+__global__ __launch_bounds__(256) void k_gpu_clean(
+    const float* __restrict__ ref_depth, const float* const* __restrict__ src,
+    const GcCamera* __restrict__ cams, int w, int h, int nsrc,
+    float thr, int min_views, float carving, int max_carved,
+    float* __restrict__ out) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= w || y >= h) return;
+  const size_t idx = (size_t)y * w + x;
+  const float d = ref_depth[idx];
+  if (!(d > 0.0f)) {
+    out[idx] = 0.0f;
+    return;
+  }
+  const GcCamera& rc = cams[0];
+  const float fx = rc.K[0], cx = rc.K[2], fy = rc.K[4], cy = rc.K[5];
+  const float px = (float)x, py = (float)y;
+  const float pcx = __fdiv_rn(px - cx, fx) * d - rc.t[0];
+  const float pcy = __fdiv_rn(py - cy, fy) * d - rc.t[1];
+  const float pcz = d - rc.t[2];
+  const float wx = rc.R[0] * pcx + rc.R[3] * pcy + rc.R[6] * pcz;
+  const float wy = rc.R[1] * pcx + rc.R[4] * pcy + rc.R[7] * pcz;
+  const float wz = rc.R[2] * pcx + rc.R[5] * pcy + rc.R[8] * pcz;
+  int consistent = 1, carved = 0;
+  for (int v = 0; v < nsrc; ++v) {
+    const GcCamera& sc = cams[v + 1];
+    const float sx = sc.R[0] * wx + sc.R[1] * wy + sc.R[2] * wz + sc.t[0];
+    const float sy = sc.R[3] * wx + sc.R[4] * wy + sc.R[5] * wz + sc.t[1];
+    const float sz = sc.R[6] * wx + sc.R[7] * wy + sc.R[8] * wz + sc.t[2];
+    if (!(sz > 1e-6f)) continue;
+    const float su = __fdiv_rn(sc.K[0] * sx, sz) + sc.K[2];
+    const float sv = __fdiv_rn(sc.K[4] * sy, sz) + sc.K[5];
+    if (!(su >= 0.0f) || !(sv >= 0.0f) || su >= (float)(w - 1) || sv >= (float)(h - 1)) continue;
+    // bilinear depth of the (resized) neighbour map at (su, sv)
+    const int x0 = (int)su, y0 = (int)sv;
+    const float a = su - (float)x0, b = sv - (float)y0;
+    const int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
+    const float* m = src[v];
+    const float r0 = m[(size_t)y0 * w + x0] * (1.0f - a) + m[(size_t)y0 * w + x1] * a;
+    const float r1 = m[(size_t)y1 * w + x0] * (1.0f - a) + m[(size_t)y1 * w + x1] * a;
+    const float sd = r0 * (1.0f - b) + r1 * b;
+    if (!(sd > 0.0f)) continue;
+    if (sd > sz * (1.0f + carving)) {  // the neighbour sees through the point
+      carved++;
+      continue;
+    }
+    if (!(fabsf(sd - sz) < sz * thr)) continue;
+    // back from the neighbour's depth into the reference view
+    const float qx = __fdiv_rn(su - sc.K[2], sc.K[0]) * sd - sc.t[0];
+    const float qy = __fdiv_rn(sv - sc.K[5], sc.K[4]) * sd - sc.t[1];
+    const float qz = sd - sc.t[2];
+    const float w2x = sc.R[0] * qx + sc.R[3] * qy + sc.R[6] * qz;
+    const float w2y = sc.R[1] * qx + sc.R[4] * qy + sc.R[7] * qz;
+    const float w2z = sc.R[2] * qx + sc.R[5] * qy + sc.R[8] * qz;
+    const float rx = rc.R[0] * w2x + rc.R[1] * w2y + rc.R[2] * w2z + rc.t[0];
+    const float ry = rc.R[3] * w2x + rc.R[4] * w2y + rc.R[5] * w2z + rc.t[1];
+    const float rz = rc.R[6] * w2x + rc.R[7] * w2y + rc.R[8] * w2z + rc.t[2];
+    if (!(rz > 1e-6f)) continue;
+    const float ru = __fdiv_rn(fx * rx, rz) + cx;
+    const float rv = __fdiv_rn(fy * ry, rz) + cy;
+    if (fabsf(ru - px) + fabsf(rv - py) > 2.0f) continue;
+    if (!(fabsf(rz - d) < d * thr)) continue;
+    consistent++;
+  }
+  out[idx] = (consistent >= min_views && carved < max_carved) ? d : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
 // Launch wrappers
 
 static inline dim3 grid16(int w, int h) {
@@ -1380,6 +1488,23 @@ hipError_t launch_clean_batch(const ClJob* jobs, int njobs, int total,
   if (total <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_clean_batch, dim3(total), dim3(256), 0, s, jobs, njobs,
                      views, thr, min_views);
+  return hipGetLastError();
+}
+
+hipError_t launch_resize_linear(const float* src, int sw, int sh, float* dst,
+                                int w, int h, hipStream_t s) {
+  dim3 blk(16, 16);
+  dim3 grd((w + 15) / 16, (h + 15) / 16);
+  hipLaunchKernelGGL(k_resize_linear, grd, blk, 0, s, src, sw, sh, dst, w, h);
+  return hipGetLastError();
+}
+
+hipError_t launch_gpu_clean(const float* ref_depth, const float* const* src,
+                            const GcCamera* cams, int w, int h, int nsrc,
+                            float thr, int min_views, float carving,
+                            int max_carved, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gpu_clean, grid16(w, h), dim3(256), 0, s, ref_depth, src, cams, w, h,
+                     nsrc, thr, min_views, carving, max_carved, out);
   return hipGetLastError();
 }
 

@@ -79,6 +79,12 @@ struct ClRef {
   double Kinv[9], Rt[9];
 };
 
+// Camera of the carving cleaner (GPUDepthmapCleaner): FP32 row-major K, R
+// and t, as the reference hands them to its kernel (opencl_utils.h:230-252).
+struct GcCamera {
+  float K[9], R[9], t[3];
+};
+
 // One depthmap of a batched clean: views[view0] is the map being cleaned,
 // views[view0 + 1 .. view0 + nviews - 1] the maps it is checked against.
 struct ClJob {

@@ -230,3 +230,91 @@ class DepthmapCleaner:
         ms = C.c_float(0)
         _capi.check(self._lib.pm_cleaner_last_ms(self._h, C.byref(ms)))
         return ms.value
+
+
+class GPUDepthmapCleaner:
+    """The live pipeline's cleaner with space carving and a backward
+    reprojection check (GPUDepthmapCleaner, cleaner.h:16-63; Python surface
+    pydense.pyi:129-148)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _capi.load()
+        self._device = int(device)
+        self._create()
+        self._shapes = []
+
+    def _create(self):
+        h = C.c_void_p()
+        _capi.check(self._lib.pm_gpu_cleaner_create(self._device, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.pm_gpu_cleaner_destroy(h)
+            self._h = None
+
+    def set_same_depth_threshold(self, t: float) -> None:
+        _capi.check(self._lib.pm_gpu_cleaner_set_same_depth_threshold(self._h, float(t)))
+
+    def set_min_consistent_views(self, n: int) -> None:
+        _capi.check(self._lib.pm_gpu_cleaner_set_min_consistent_views(self._h, int(n)))
+
+    def set_carving_threshold(self, t: float) -> None:
+        _capi.check(self._lib.pm_gpu_cleaner_set_carving_threshold(self._h, float(t)))
+
+    def set_max_carved_views(self, n: int) -> None:
+        _capi.check(self._lib.pm_gpu_cleaner_set_max_carved_views(self._h, int(n)))
+
+    def set_device(self, idx: int) -> None:
+        """Selects the device; only before the first add_view (the views
+        live in that device's memory)."""
+        if self._shapes:
+            raise RuntimeError("set_device must come before add_view")
+        self._lib.pm_gpu_cleaner_destroy(self._h)
+        self._device = int(idx)
+        self._create()
+
+    def add_view(self, K, R, t, depth) -> None:
+        K = _mat(K, (3, 3), "K")
+        R = _mat(R, (3, 3), "R")
+        t = _mat(t, (3,), "t")
+        depth = np.asarray(depth)
+        if depth.ndim != 2:
+            raise ValueError("depth must be a 2-D array")
+        depth = np.ascontiguousarray(depth, dtype=np.float32)
+        h, w = depth.shape
+        idx = C.c_int32(-1)
+        _capi.check(self._lib.pm_gpu_cleaner_add_view(
+            self._h, _capi.ptr(K, C.c_double), _capi.ptr(R, C.c_double),
+            _capi.ptr(t, C.c_double), _capi.ptr(depth, C.c_float), w, h, C.byref(idx)))
+        self._shapes.append((h, w))
+
+    def clean(self, ref_idx: int, neighbor_ids) -> np.ndarray:
+        nb = np.ascontiguousarray(np.asarray(neighbor_ids, np.int32).reshape(-1))
+        if not (0 <= int(ref_idx) < len(self._shapes)):
+            shape = (1, 1)
+        else:
+            shape = self._shapes[int(ref_idx)]
+        out = np.zeros(shape, np.float32)
+        _capi.check(self._lib.pm_gpu_cleaner_clean(
+            self._h, int(ref_idx), _capi.ptr(nb, C.c_int32), int(nb.size),
+            _capi.ptr(out, C.c_float)))
+        return out
+
+    def clear(self) -> None:
+        _capi.check(self._lib.pm_gpu_cleaner_clear(self._h))
+        self._shapes = []
+
+    def last_ms(self) -> float:
+        ms = C.c_float(0)
+        _capi.check(self._lib.pm_gpu_cleaner_last_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    @staticmethod
+    def is_available() -> bool:
+        try:
+            return _capi.device_count() > 0
+        except RuntimeError:
+            return False
+

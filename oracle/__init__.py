@@ -256,3 +256,98 @@ class OracleEstimator:
         return self._L.orc_score_f64(self._h, i, j,
                                      np.ascontiguousarray(plane, dtype=np.float64),
                                      other)
+
+
+# ---------------------------------------------------------------------------
+# GPUDepthmapCleaner (opensfm/src/lib/dense/src/cleaner.cc:56-179 and its
+# kernel acmmp_clean_depthmap, opencl_kernels.h:2155-2273), restated in
+# numpy float32 with the operation order written there.
+
+def resize_linear_f32(depth, w, h):
+    """cv::resize(INTER_LINEAR) of a float32 map to (h, w): source
+    coordinate (x + 0.5) * sw / w - 0.5, floor and fraction, clamped at
+    the borders (cleaner.cc:114-117)."""
+    src = np.asarray(depth, np.float32)
+    sh, sw = src.shape
+    if (sh, sw) == (h, w):
+        return src.copy()
+
+    def axis(n_dst, n_src):
+        scale = np.float64(n_src) / n_dst
+        f = ((np.arange(n_dst) + 0.5) * scale - 0.5).astype(np.float32)
+        i = np.floor(f).astype(np.int64)
+        a = (f - i.astype(np.float32)).astype(np.float32)
+        low = i < 0
+        a[low], i[low] = 0, 0
+        high = i >= n_src - 1
+        a[high], i[high] = 0, n_src - 1
+        return i, np.minimum(i + 1, n_src - 1), a
+
+    x0, x1, ax = axis(w, sw)
+    y0, y1, ay = axis(h, sh)
+    one = np.float32(1)
+    r0 = src[y0][:, x0] * (one - ax) + src[y0][:, x1] * ax
+    r1 = src[y1][:, x0] * (one - ax) + src[y1][:, x1] * ax
+    return (r0 * (one - ay)[:, None] + r1 * ay[:, None]).astype(np.float32)
+
+
+def gpu_clean(views, ref_idx, neighbor_ids, same_depth_threshold=0.01,
+              min_consistent_views=3, carving_threshold=0.2, max_carved_views=1):
+    """views: list of (K, R, t, depth).  Returns the cleaned depth of
+    views[ref_idx] checked against views[neighbor_ids] (at most 16)."""
+    f = np.float32
+    Kr, Rr, tr, dr = views[ref_idx]
+    dr = np.asarray(dr, f)
+    h, w = dr.shape
+    Kr, Rr, tr = (np.asarray(a, np.float64).astype(f) for a in (Kr, Rr, tr))
+    nb = list(neighbor_ids)[:16]
+    num_views = 1 + len(neighbor_ids)
+    eff_min = min(min_consistent_views, num_views)
+    carving = f(max(0.0, carving_threshold))
+    thr = f(same_depth_threshold)
+    yy, xx = np.meshgrid(np.arange(h, dtype=f), np.arange(w, dtype=f), indexing="ij")
+    d = dr
+    cam = [(xx - Kr[0, 2]) / Kr[0, 0] * d, (yy - Kr[1, 2]) / Kr[1, 1] * d, d]
+    pc = [cam[k] - tr[k] for k in range(3)]
+    wp = [Rr[0, k] * pc[0] + Rr[1, k] * pc[1] + Rr[2, k] * pc[2] for k in range(3)]
+    consistent = np.ones((h, w), np.int32)
+    carved = np.zeros((h, w), np.int32)
+    for v in nb:
+        Ks, Rs, ts, ds = views[v]
+        Ks, Rs, ts = (np.asarray(a, np.float64).astype(f) for a in (Ks, Rs, ts))
+        src = resize_linear_f32(ds, w, h)
+        s = [Rs[k, 0] * wp[0] + Rs[k, 1] * wp[1] + Rs[k, 2] * wp[2] + ts[k] for k in range(3)]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            su = Ks[0, 0] * s[0] / s[2] + Ks[0, 2]
+            sv = Ks[1, 1] * s[1] / s[2] + Ks[1, 2]
+        ok = (d > 0) & (s[2] > f(1e-6))
+        ok &= ~((su < 0) | (sv < 0) | (su >= f(w - 1)) | (sv >= f(h - 1)))
+        ok &= np.isfinite(su) & np.isfinite(sv)
+        sus, svs = np.where(ok, su, 0).astype(f), np.where(ok, sv, 0).astype(f)
+        x0 = np.floor(sus).astype(np.int64)
+        y0 = np.floor(svs).astype(np.int64)
+        a = sus - x0.astype(f)
+        b = svs - y0.astype(f)
+        x1, y1 = np.minimum(x0 + 1, w - 1), np.minimum(y0 + 1, h - 1)
+        one = f(1)
+        r0 = src[y0, x0] * (one - a) + src[y0, x1] * a
+        r1 = src[y1, x0] * (one - a) + src[y1, x1] * a
+        sd = (r0 * (one - b) + r1 * b).astype(f)
+        ok &= sd > 0
+        carve = ok & (sd > s[2] * (one + carving))
+        carved += carve
+        ok &= ~carve
+        ok &= np.abs(sd - s[2]) < s[2] * thr
+        sc = [(sus - Ks[0, 2]) / Ks[0, 0] * sd, (svs - Ks[1, 2]) / Ks[1, 1] * sd, sd]
+        spc = [sc[k] - ts[k] for k in range(3)]
+        w2 = [Rs[0, k] * spc[0] + Rs[1, k] * spc[1] + Rs[2, k] * spc[2] for k in range(3)]
+        r = [Rr[k, 0] * w2[0] + Rr[k, 1] * w2[1] + Rr[k, 2] * w2[2] + tr[k] for k in range(3)]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ru = Kr[0, 0] * r[0] / r[2] + Kr[0, 2]
+            rv = Kr[1, 1] * r[1] / r[2] + Kr[1, 2]
+        ok &= r[2] > f(1e-6)
+        ok &= ~(np.abs(ru - xx) + np.abs(rv - yy) > f(2))
+        ok &= np.abs(r[2] - d) < d * thr
+        consistent += ok
+    keep = (d > 0) & (consistent >= eff_min) & (carved < max_carved_views)
+    return np.where(keep, d, f(0)).astype(f)
