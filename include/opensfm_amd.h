@@ -247,6 +247,35 @@ int pm_gpu_cleaner_clean(pm_gpu_cleaner* c, int ref_idx,
 int pm_gpu_cleaner_clear(pm_gpu_cleaner* c);
 int pm_gpu_cleaner_last_ms(pm_gpu_cleaner* c, float* ms);
 
+/* ---- DepthmapPruner (depthmap.h:131-153, depthmap.cc:557-637) --------- */
+
+/* Merges view 0's depthmap into points, dropping each pixel that another
+ * view sees, at the rounded projection, with a depth beyond (1 - threshold)
+ * times the point's and a larger -n_z / depth * f (a finer sample of the
+ * surface).  Kept pixels come out in row-major order with the reference's
+ * types: FP64 back-projection and projection, cv::normalize of the plane
+ * (FP64 norm), FP32 area and normal rotation R0^T.  Default threshold
+ * 0.01.  Only view 0's colour and label are read (the merge outputs them). */
+typedef struct pm_pruner pm_pruner;
+int pm_pruner_create(int device, pm_pruner** out);
+int pm_pruner_destroy(pm_pruner* c);
+/* DepthmapPruner::SetSameDepthThreshold (depthmap.cc:559-561) */
+int pm_pruner_set_same_depth_threshold(pm_pruner* c, float t);
+/* DepthmapPruner::AddView (depthmap.cc:563-575): K, R row-major 3x3, t 3;
+ * depth H*W f32, plane H*W*3 f32, color H*W*3 u8, label H*W u8. */
+int pm_pruner_add_view(pm_pruner* c, const double* K, const double* R,
+                       const double* t, const float* depth,
+                       const float* plane, const uint8_t* color,
+                       const uint8_t* label, int width, int height);
+/* DepthmapPruner::Prune (depthmap.cc:577-636): runs the merge; *count
+ * receives the number of kept points, which stay on the device until
+ * pm_pruner_download copies count*3 floats of points and of normals,
+ * count*3 bytes of colours and count bytes of labels. */
+int pm_pruner_prune(pm_pruner* c, int64_t* count);
+int pm_pruner_download(pm_pruner* c, float* points, float* normals,
+                       uint8_t* colors, uint8_t* labels);
+int pm_pruner_last_ms(pm_pruner* c, float* ms);
+
 /* ---- Batched estimation ---------------------------------------------- */
 
 /* Many reference views ("jobs") of one pool of views, run together: each

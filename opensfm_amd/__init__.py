@@ -6,6 +6,8 @@ by hand-written HIP kernels for gfx950 behind the C ABI in
 include/opensfm_amd.h.
 """
 from .batch import DepthmapBatch  # noqa: F401
-from .pydense import DepthmapCleaner, DepthmapEstimator, GPUDepthmapCleaner  # noqa: F401
+from .pydense import (  # noqa: F401
+    DepthmapCleaner, DepthmapEstimator, DepthmapPruner, GPUDepthmapCleaner)
 
-__all__ = ["DepthmapEstimator", "DepthmapCleaner", "GPUDepthmapCleaner", "DepthmapBatch"]
+__all__ = ["DepthmapEstimator", "DepthmapCleaner", "GPUDepthmapCleaner", "DepthmapPruner",
+           "DepthmapBatch"]

@@ -97,6 +97,14 @@ SIGNATURES = {
     "pm_gpu_cleaner_clean": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _fp]),
     "pm_gpu_cleaner_clear": (C.c_int, [_vp]),
     "pm_gpu_cleaner_last_ms": (C.c_int, [_vp, _fp]),
+    "pm_pruner_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "pm_pruner_destroy": (C.c_int, [_vp]),
+    "pm_pruner_set_same_depth_threshold": (C.c_int, [_vp, C.c_float]),
+    "pm_pruner_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _fp, _fp, _u8p, _u8p, C.c_int,
+                                     C.c_int]),
+    "pm_pruner_prune": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "pm_pruner_download": (C.c_int, [_vp, _fp, _fp, _u8p, _u8p]),
+    "pm_pruner_last_ms": (C.c_int, [_vp, _fp]),
     "pm_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pm_batch_destroy": (C.c_int, [_vp]),
     "pm_batch_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _u8p, _u8p, C.c_int, C.c_int,

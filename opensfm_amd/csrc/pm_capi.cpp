@@ -1077,3 +1077,197 @@ int pm_gpu_cleaner_last_ms(pm_gpu_cleaner* c, float* ms) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// DepthmapPruner (depthmap.h:131-153, depthmap.cc:557-637)
+
+struct PrHostView {
+  double K[9], R[9], t[3];
+  int width, height;
+  DevBuf depth, plane;
+};
+
+struct pm_pruner {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  float same_depth_threshold = 0.01f;  // depthmap.cc:557
+  std::vector<std::unique_ptr<PrHostView>> views;
+  DevBuf color, label;  // of view 0, the only one the merge reads
+  DevBuf table, counts, offsets, points, normals, out_color, out_label;
+  int64_t count = -1;  // kept points of the last Prune, -1 before one
+  float last_ms = 0;
+  ~pm_pruner() {
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+int pm_pruner_create(int device, pm_pruner** out) {
+  if (!out) return fail(PM_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  int rc = check_device(device);
+  if (rc) return rc;
+  return guarded([&] {
+    std::unique_ptr<pm_pruner> c(new pm_pruner());
+    c->device = device;
+    DeviceGuard dg(device);
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIP_CHECK(hipEventCreate(&e));
+    *out = c.release();
+    return PM_OK;
+  });
+}
+
+int pm_pruner_destroy(pm_pruner* c) {
+  if (!c) return PM_OK;
+  return guarded([&] {
+    DeviceGuard dg(c->device);
+    delete c;
+    return PM_OK;
+  });
+}
+
+int pm_pruner_set_same_depth_threshold(pm_pruner* c, float t) {
+  if (!c) return fail(PM_ERR_INVALID, "null pruner");
+  c->same_depth_threshold = t;
+  return PM_OK;
+}
+
+int pm_pruner_add_view(pm_pruner* c, const double* K, const double* R, const double* t,
+                       const float* depth, const float* plane, const uint8_t* color,
+                       const uint8_t* label, int width, int height) {
+  if (!c) return fail(PM_ERR_INVALID, "null pruner");
+  if (!K || !R || !t || !depth || !plane || !color || !label)
+    return fail(PM_ERR_INVALID, "null K, R, t, depth, plane, color or label");
+  if (width < 1 || height < 1) return fail(PM_ERR_INVALID, "empty depth map");
+  if ((int64_t)width * height > (int64_t)INT32_MAX)
+    return fail(PM_ERR_INVALID, "depth map larger than 2^31 pixels");
+  if (c->views.empty()) {
+    double Kinv[9];
+    if (!inv3(K, Kinv)) return fail(PM_ERR_INVALID, "singular K of the first view");
+  }
+  return guarded([&] {
+    std::unique_ptr<PrHostView> v(new PrHostView());
+    memcpy(v->K, K, sizeof(v->K));
+    memcpy(v->R, R, sizeof(v->R));
+    memcpy(v->t, t, sizeof(v->t));
+    v->width = width;
+    v->height = height;
+    DeviceGuard dg(c->device);
+    const size_t np = (size_t)width * height;
+    v->depth.ensure(np * sizeof(float));
+    v->plane.ensure(np * 3 * sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(v->depth.p, depth, np * sizeof(float), hipMemcpyHostToDevice,
+                             c->stream));
+    HIP_CHECK(hipMemcpyAsync(v->plane.p, plane, np * 3 * sizeof(float), hipMemcpyHostToDevice,
+                             c->stream));
+    if (c->views.empty()) {
+      c->color.ensure(np * 3);
+      c->label.ensure(np);
+      HIP_CHECK(hipMemcpyAsync(c->color.p, color, np * 3, hipMemcpyHostToDevice, c->stream));
+      HIP_CHECK(hipMemcpyAsync(c->label.p, label, np, hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->views.push_back(std::move(v));
+    c->count = -1;
+    return PM_OK;
+  });
+}
+
+int pm_pruner_prune(pm_pruner* c, int64_t* count) {
+  if (!c || !count) return fail(PM_ERR_INVALID, "null argument");
+  if (c->views.empty()) return fail(PM_ERR_INVALID, "Prune needs at least one view");
+  return guarded([&] {
+    DeviceGuard dg(c->device);
+    const int n = (int)c->views.size();
+    const PrHostView& V0 = *c->views[0];
+    const int npix = V0.width * V0.height;
+    const int nblk = (npix + 255) / 256;
+    std::vector<pm::PrView> table(n);
+    for (int v = 0; v < n; ++v) {
+      const PrHostView& H = *c->views[v];
+      pm::PrView& T = table[v];
+      T.depth = H.depth.as<float>();
+      T.plane = H.plane.as<float>();
+      T.width = H.width;
+      T.height = H.height;
+      memcpy(T.K, H.K, sizeof(T.K));
+      memcpy(T.R, H.R, sizeof(T.R));
+      memcpy(T.t, H.t, sizeof(T.t));
+    }
+    pm::PrRef ref;
+    inv3(V0.K, ref.Kinv);
+    mat3_transpose(V0.R, ref.Rt);
+    for (int k = 0; k < 9; ++k) ref.Rinv[k] = (float)ref.Rt[k];
+    c->table.ensure(n * sizeof(pm::PrView));
+    c->counts.ensure(nblk * sizeof(int));
+    c->offsets.ensure(nblk * sizeof(long long));
+    HIP_CHECK(hipMemcpyAsync(c->table.p, table.data(), n * sizeof(pm::PrView),
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipEventRecord(c->ev[0], c->stream));
+    HIP_CHECK(pm::launch_prune_count(c->table.as<pm::PrView>(), n, ref, c->same_depth_threshold,
+                                     npix, c->counts.as<int>(), c->stream));
+    // The block offsets are an exclusive scan of at most ~2^23 counts, done
+    // on the host: the total sizes the output buffers anyway.
+    std::vector<int> counts(nblk);
+    HIP_CHECK(hipMemcpyAsync(counts.data(), c->counts.p, nblk * sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    std::vector<long long> offsets(nblk);
+    long long total = 0;
+    for (int b = 0; b < nblk; ++b) {
+      offsets[b] = total;
+      total += counts[b];
+    }
+    const size_t cap = (size_t)std::max<long long>(total, 1);
+    c->points.ensure(cap * 3 * sizeof(float));
+    c->normals.ensure(cap * 3 * sizeof(float));
+    c->out_color.ensure(cap * 3);
+    c->out_label.ensure(cap);
+    HIP_CHECK(hipMemcpyAsync(c->offsets.p, offsets.data(), nblk * sizeof(long long),
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(pm::launch_prune_write(
+        c->table.as<pm::PrView>(), n, ref, c->same_depth_threshold, npix,
+        c->offsets.as<long long>(), c->color.as<uint8_t>(), c->label.as<uint8_t>(),
+        c->points.as<float>(), c->normals.as<float>(), c->out_color.as<uint8_t>(),
+        c->out_label.as<uint8_t>(), c->stream));
+    HIP_CHECK(hipEventRecord(c->ev[1], c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipEventElapsedTime(&c->last_ms, c->ev[0], c->ev[1]));
+    c->count = total;
+    *count = total;
+    return PM_OK;
+  });
+}
+
+int pm_pruner_download(pm_pruner* c, float* points, float* normals, uint8_t* colors,
+                       uint8_t* labels) {
+  if (!c) return fail(PM_ERR_INVALID, "null pruner");
+  if (c->count < 0) return fail(PM_ERR_INVALID, "download before prune");
+  if (c->count == 0) return PM_OK;
+  if (!points || !normals || !colors || !labels) return fail(PM_ERR_INVALID, "null output");
+  return guarded([&] {
+    DeviceGuard dg(c->device);
+    const size_t m = (size_t)c->count;
+    HIP_CHECK(hipMemcpyAsync(points, c->points.p, m * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIP_CHECK(hipMemcpyAsync(normals, c->normals.p, m * 3 * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(colors, c->out_color.p, m * 3, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(labels, c->out_label.p, m, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return PM_OK;
+  });
+}
+
+int pm_pruner_last_ms(pm_pruner* c, float* ms) {
+  if (!c || !ms) return fail(PM_ERR_INVALID, "null argument");
+  *ms = c->last_ms;
+  return PM_OK;
+}
+
+}  // extern "C"

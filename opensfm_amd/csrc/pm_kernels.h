@@ -65,6 +65,15 @@ hipError_t launch_gpu_clean(const float* ref_depth, const float* const* src,
                             const GcCamera* cams, int w, int h, int nsrc,
                             float thr, int min_views, float carving,
                             int max_carved, float* out, hipStream_t s);
+// DepthmapPruner: kept-pixel counts per 256-pixel block, then the kept
+// pixels written at the blocks' offsets (row-major order).
+hipError_t launch_prune_count(const PrView* views, int n, const PrRef& ref,
+                              float thr, int npix, int* counts, hipStream_t s);
+hipError_t launch_prune_write(const PrView* views, int n, const PrRef& ref,
+                              float thr, int npix, const long long* offsets,
+                              const uint8_t* color, const uint8_t* label,
+                              float* points, float* normals, uint8_t* out_color,
+                              uint8_t* out_label, hipStream_t s);
 hipError_t launch_clean(const ClView* views, const ClRef* ref, int n, int W,
                         int H, float thr, int min_views, float* out,
                         hipStream_t s);

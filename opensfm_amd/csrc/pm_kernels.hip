@@ -1209,6 +1209,141 @@ __global__ __launch_bounds__(256) void k_gpu_clean(
 }
 
 // ---------------------------------------------------------------------------
+// DepthmapPruner::Prune (depthmap.cc:574-636): a pixel of view 0 is dropped
+// when another view sees, at the (rounded) projection, a depth beyond
+// (1 - threshold) times the point's depth with a larger "area"
+// -n_z / depth * f.  Types as the reference: cv::normalize of a Vec3f
+// (FP64 norm and scale, FP32 result), FP64 Backproject/Project, FP32 area.
+// Two passes give the kept pixels in row-major order: counts per
+// 256-pixel block, then a scatter at the blocks' exclusive prefix offsets.
+
+__device__ __forceinline__ void normalize3f(const float* v, float* out) {
+  double s = 0.0;
+  s += (double)v[0] * (double)v[0];
+  s += (double)v[1] * (double)v[1];
+  s += (double)v[2] * (double)v[2];
+  const double nv = __dsqrt_rn(s);
+  const double a = nv != 0.0 ? __ddiv_rn(1.0, nv) : 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) out[k] = (float)((double)v[k] * a);
+}
+
+// x / z + 0.5 is rounded to std::int64_t and used through int arguments
+// (IsInsideImage, Mat::at): the low 32 bits.  Outside int64 or NaN the
+// conversion is undefined in C++; x86 gives INT64_MIN (low bits 0), and so
+// does this.
+__device__ __forceinline__ int ref_pixel_index(double u) {
+  if (!(fabs(u) < 9.223372036854775808e18)) return 0;
+  return (int)(unsigned int)(unsigned long long)(long long)u;
+}
+
+__device__ __forceinline__ void mat3_vec_d(const double* A, const double* x,
+                                           double* y) {
+  double t[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s += A[3 * r + k] * x[k];
+    t[r] = s;
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) y[r] = t[r];
+}
+
+// Whether pixel p of view 0 is kept; its point and normal when it is.
+__device__ __forceinline__ bool prune_pixel(const PrView* __restrict__ views,
+                                            int n, const PrRef& ref, float thr,
+                                            int p, float* point,
+                                            float* normal) {
+  const PrView& V0 = views[0];
+  const int i = p / V0.width, j = p - (p / V0.width) * V0.width;
+  const float depth = V0.depth[p];
+  if (depth <= 0.0f) return false;  // a NaN depth goes on, as in the reference
+  normalize3f(V0.plane + 3 * (size_t)p, normal);
+  const float area = (float)((double)__fdiv_rn(-normal[2], depth) * V0.K[0]);
+  double S[9], X[3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) S[k] = (double)depth * ref.Kinv[k];
+  const double q[3] = {(double)j, (double)i, 1.0};
+  mat3_vec_d(S, q, X);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) X[k] = X[k] - V0.t[k];
+  mat3_vec_d(ref.Rt, X, X);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) point[k] = (float)X[k];
+  for (int o = 1; o < n; ++o) {
+    const PrView& Vo = views[o];
+    const double pd[3] = {point[0], point[1], point[2]};
+    double y[3];
+    mat3_vec_d(Vo.R, pd, y);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) y[k] = y[k] + Vo.t[k];
+    mat3_vec_d(Vo.K, y, y);
+    if (y[2] < 1e-8 || isnan(y[2])) continue;
+    const int iu = ref_pixel_index(__ddiv_rn(y[0], y[2]) + 0.5);
+    const int iv = ref_pixel_index(__ddiv_rn(y[1], y[2]) + 0.5);
+    if (!(iv >= 0 && iv < Vo.height && iu >= 0 && iu < Vo.width)) continue;
+    const size_t po = (size_t)iv * Vo.width + (size_t)iu;
+    const float d_o = Vo.depth[po];
+    if ((double)d_o > (double)(1.0f - thr) * y[2]) {
+      float no[3];
+      normalize3f(Vo.plane + 3 * po, no);
+      if (d_o == 0.0f || (double)__fdiv_rn(-no[2], d_o) * Vo.K[0] > (double)area) return false;
+    }
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_prune_count(
+    const PrView* __restrict__ views, int n, PrRef ref, float thr, int npix,
+    int* __restrict__ counts) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  float pt[3], nm[3];
+  const bool keep = p < npix && prune_pixel(views, n, ref, thr, p, pt, nm);
+  const int c = __syncthreads_count(keep);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(256) void k_prune_write(
+    const PrView* __restrict__ views, int n, PrRef ref, float thr, int npix,
+    const long long* __restrict__ offsets, const uint8_t* __restrict__ color,
+    const uint8_t* __restrict__ label, float* __restrict__ points,
+    float* __restrict__ normals, uint8_t* __restrict__ out_color,
+    uint8_t* __restrict__ out_label) {
+  __shared__ int wave_base[4];
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  float pt[3], nm[3];
+  const bool keep = p < npix && prune_pixel(views, n, ref, thr, p, pt, nm);
+  const unsigned long long m = __ballot(keep);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rank = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wave_base[wave] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int w = 0; w < 4; ++w) {
+      const int c = wave_base[w];
+      wave_base[w] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  if (!keep) return;
+  const long long o = offsets[blockIdx.x] + wave_base[wave] + rank;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s += ref.Rinv[3 * r + k] * nm[k];
+    points[3 * o + r] = pt[r];
+    normals[3 * o + r] = s;
+    out_color[3 * o + r] = color[3 * (size_t)p + r];
+  }
+  out_label[o] = label[p];
+}
+
+// ---------------------------------------------------------------------------
 // Launch wrappers
 
 static inline dim3 grid16(int w, int h) {
@@ -1505,6 +1640,23 @@ hipError_t launch_gpu_clean(const float* ref_depth, const float* const* src,
                             int max_carved, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_gpu_clean, grid16(w, h), dim3(256), 0, s, ref_depth, src, cams, w, h,
                      nsrc, thr, min_views, carving, max_carved, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_prune_count(const PrView* views, int n, const PrRef& ref,
+                              float thr, int npix, int* counts, hipStream_t s) {
+  hipLaunchKernelGGL(k_prune_count, dim3((npix + 255) / 256), dim3(256), 0, s, views, n, ref,
+                     thr, npix, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_prune_write(const PrView* views, int n, const PrRef& ref,
+                              float thr, int npix, const long long* offsets,
+                              const uint8_t* color, const uint8_t* label,
+                              float* points, float* normals, uint8_t* out_color,
+                              uint8_t* out_label, hipStream_t s) {
+  hipLaunchKernelGGL(k_prune_write, dim3((npix + 255) / 256), dim3(256), 0, s, views, n, ref,
+                     thr, npix, offsets, color, label, points, normals, out_color, out_label);
   return hipGetLastError();
 }
 

@@ -85,6 +85,21 @@ struct GcCamera {
   float K[9], R[9], t[3];
 };
 
+// A view of the pruner (DepthmapPruner::AddView, depthmap.cc:563-575).
+struct PrView {
+  const float* depth;
+  const float* plane;  // H x W x 3
+  int width, height;
+  double K[9], R[9], t[3];
+};
+
+// View 0 of the pruner: K^-1, R^T (FP64) and R^T in FP32 (the normals'
+// rotation, a cv::Matx33f in the reference).
+struct PrRef {
+  double Kinv[9], Rt[9];
+  float Rinv[9];
+};
+
 // One depthmap of a batched clean: views[view0] is the map being cleaned,
 // views[view0 + 1 .. view0 + nviews - 1] the maps it is checked against.
 struct ClJob {

@@ -1,8 +1,10 @@
 """``pydense``-compatible surface of the MI355X depthmap engine.
 
-Mirrors ``pydense.DepthmapEstimator`` and ``pydense.DepthmapCleaner``
-(opensfm/src/lib/dense/pydense.pyi:28-60; C++ classes in
-opensfm/src/lib/dense/depthmap.h:57-129): same method names, argument order,
+Mirrors ``pydense.DepthmapEstimator``, ``DepthmapCleaner``,
+``DepthmapPruner`` and ``GPUDepthmapCleaner``
+(opensfm/src/lib/dense/pydense.pyi:28-77, 129-148; C++ classes in
+opensfm/src/lib/dense/depthmap.h:57-153 and cleaner.h:16-63): same method
+names, argument order,
 defaults and return layout, so code written against the reference reads the
 same.  Every compute runs on the GPU through the C ABI in
 include/opensfm_amd.h; the calls release the GIL (ctypes does) like the
@@ -318,3 +320,66 @@ class GPUDepthmapCleaner:
         except RuntimeError:
             return False
 
+
+
+class DepthmapPruner:
+    """Merges view 0's depthmap into a point cloud, dropping the points that
+    another view samples more finely (DepthmapPruner, depthmap.h:131-153;
+    Python surface pydense.pyi:63-77)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _capi.load()
+        h = C.c_void_p()
+        _capi.check(self._lib.pm_pruner_create(int(device), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.pm_pruner_destroy(h)
+            self._h = None
+
+    def set_same_depth_threshold(self, t: float) -> None:
+        _capi.check(self._lib.pm_pruner_set_same_depth_threshold(self._h, float(t)))
+
+    def add_view(self, K, R, t, depth, plane, color, label) -> None:
+        """depth (H, W), plane (H, W, 3), color (H, W, 3) uint8, label (H, W)
+        uint8, as DepthmapPruner::AddView (depthmap.cc:563-575)."""
+        K = _mat(K, (3, 3), "K")
+        R = _mat(R, (3, 3), "R")
+        t = _mat(t, (3,), "t")
+        depth = np.asarray(depth)
+        if depth.ndim != 2:
+            raise ValueError("depth must be a 2-D array")
+        h, w = depth.shape
+        depth = np.ascontiguousarray(depth, dtype=np.float32)
+        plane = np.ascontiguousarray(np.asarray(plane, dtype=np.float32))
+        color = np.ascontiguousarray(np.asarray(color, dtype=np.uint8))
+        label = np.ascontiguousarray(np.asarray(label, dtype=np.uint8))
+        if plane.shape != (h, w, 3) or color.shape != (h, w, 3) or label.shape != (h, w):
+            raise ValueError("plane and color must be (H, W, 3) and label (H, W) like depth")
+        _capi.check(self._lib.pm_pruner_add_view(
+            self._h, _capi.ptr(K, C.c_double), _capi.ptr(R, C.c_double),
+            _capi.ptr(t, C.c_double), _capi.ptr(depth, C.c_float),
+            _capi.ptr(plane, C.c_float), _capi.ptr(color, C.c_uint8),
+            _capi.ptr(label, C.c_uint8), w, h))
+
+    def prune(self) -> list:
+        """[points (N, 3) f32, normals (N, 3) f32, colors (N, 3) u8,
+        labels (N,) u8], kept pixels in row-major order."""
+        n = C.c_int64(0)
+        _capi.check(self._lib.pm_pruner_prune(self._h, C.byref(n)))
+        m = int(n.value)
+        points = np.zeros((m, 3), np.float32)
+        normals = np.zeros((m, 3), np.float32)
+        colors = np.zeros((m, 3), np.uint8)
+        labels = np.zeros(m, np.uint8)
+        _capi.check(self._lib.pm_pruner_download(
+            self._h, _capi.ptr(points, C.c_float), _capi.ptr(normals, C.c_float),
+            _capi.ptr(colors, C.c_uint8), _capi.ptr(labels, C.c_uint8)))
+        return [points, normals, colors, labels]
+
+    def last_ms(self) -> float:
+        ms = C.c_float(0)
+        _capi.check(self._lib.pm_pruner_last_ms(self._h, C.byref(ms)))
+        return ms.value

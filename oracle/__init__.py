@@ -94,6 +94,11 @@ def lib():
     L.orc_clean.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.POINTER(C.c_void_p),
                             _i32p, _i32p, C.c_float, C.c_int, _f32p]
     L.orc_clean.restype = C.c_int
+    L.orc_prune.argtypes = [C.c_int, _f64p, _f64p, _f64p, C.POINTER(C.c_void_p),
+                            C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                            C.POINTER(C.c_void_p), _i32p, _i32p, C.c_float, C.c_long,
+                            _f32p, _f32p, _u8p, _u8p]
+    L.orc_prune.restype = C.c_long
     _lib = L
     return L
 
@@ -183,6 +188,38 @@ def clean(views, same_depth_threshold=0.01, min_consistent_views=2):
     if rc != 0:
         raise ValueError("clean needs at least one view")
     return out
+
+
+def prune(views, same_depth_threshold=0.01):
+    """DepthmapPruner::Prune (depthmap.cc:574-636).  views: list of (K, R, t,
+    depth (H, W) f32, plane (H, W, 3) f32, color (H, W, 3) u8, label (H, W)
+    u8); view 0 is pruned.  Returns (points (N, 3), normals (N, 3), colors
+    (N, 3) u8, labels (N,) u8)."""
+    n = len(views)
+    Ks = np.ascontiguousarray(np.stack([np.asarray(v[0], np.float64).reshape(9) for v in views]))
+    Rs = np.ascontiguousarray(np.stack([np.asarray(v[1], np.float64).reshape(9) for v in views]))
+    ts = np.ascontiguousarray(np.stack([np.asarray(v[2], np.float64).reshape(3) for v in views]))
+    keep = []
+    depths = [np.ascontiguousarray(v[3], np.float32) for v in views]
+    planes = [np.ascontiguousarray(v[4], np.float32) for v in views]
+    colors = [np.ascontiguousarray(v[5], np.uint8) for v in views]
+    labels = [np.ascontiguousarray(v[6], np.uint8) for v in views]
+    keep += depths + planes + colors + labels
+    arr = lambda xs: (C.c_void_p * n)(*[x.ctypes.data for x in xs])  # noqa: E731
+    ws = np.array([d.shape[1] for d in depths], np.int32)
+    hs = np.array([d.shape[0] for d in depths], np.int32)
+    cap = depths[0].size
+    pts = np.zeros((cap, 3), np.float32)
+    nrm = np.zeros((cap, 3), np.float32)
+    col = np.zeros((cap, 3), np.uint8)
+    lab = np.zeros(cap, np.uint8)
+    cnt = lib().orc_prune(n, Ks.reshape(-1), Rs.reshape(-1), ts.reshape(-1), arr(depths),
+                          arr(planes), arr(colors), arr(labels), ws, hs,
+                          same_depth_threshold, cap, pts.reshape(-1), nrm.reshape(-1),
+                          col.reshape(-1), lab)
+    if cnt < 0:
+        raise ValueError("prune needs at least one view")
+    return pts[:cnt], nrm[:cnt], col[:cnt], lab[:cnt]
 
 
 class OracleEstimator:

@@ -801,3 +801,97 @@ int orc_clean(int n, const double* Ks, const double* Rs, const double* ts,
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * DepthmapPruner::Prune  depthmap.cc:574-636.  Keeps a pixel of view 0
+ * unless some other view sees, at the pixel the point projects to (rounded),
+ * a depth beyond (1 - threshold) times the point's depth with a larger
+ * "area" -n_z / depth * f (a finer sample of the same surface).  Outputs in
+ * row-major pixel order: point (world, float), normal rotated to the world
+ * (float Matx33f(R0^T) * n), color (3 u8), label.  Types as the reference:
+ * cv::normalize of Vec3f (double norm and scale, float result), FP64
+ * Backproject/Project, float area. */
+static void normalize3f(const float* v, float* out) {
+  double s = 0;
+  for (int k = 0; k < 3; ++k) s += (double)v[k] * (double)v[k];
+  const double nv = sqrt(s);
+  const double a = nv != 0 ? 1.0 / nv : 0.0;
+  for (int k = 0; k < 3; ++k) out[k] = (float)((double)v[k] * a);
+}
+
+/* The reference rounds x / z + 0.5 to std::int64_t and hands it to
+ * IsInsideImage(const cv::Mat&, int, int) and Mat::at(int, int): the low 32
+ * bits are used.  A value outside int64 (or NaN) is undefined behaviour in
+ * C++; x86's truncating convert returns INT64_MIN there, whose low 32 bits
+ * are 0, and that is what is returned here. */
+static int ref_pixel_index(double u) {
+  if (!(fabs(u) < 9.223372036854775808e18)) return 0;
+  return (int)(unsigned int)(unsigned long long)(long long)u;
+}
+
+long orc_prune(int n, const double* Ks, const double* Rs, const double* ts,
+               const float* const* depths, const float* const* planes,
+               const unsigned char* const* colors,
+               const unsigned char* const* labels, const int* widths,
+               const int* heights, float same_depth_threshold, long capacity,
+               float* points, float* normals, unsigned char* out_colors,
+               unsigned char* out_labels) {
+  if (n < 1) return -1;
+  const int W = widths[0], H = heights[0];
+  double Kinv0[9], R0t[9];
+  orc_inv3(Ks, Kinv0);
+  mat3_transpose(Rs, R0t);
+  float Rinv[9];
+  for (int k = 0; k < 9; ++k) Rinv[k] = (float)R0t[k];
+  long count = 0;
+  for (int i = 0; i < H; ++i) {
+    for (int j = 0; j < W; ++j) {
+      const size_t p = (size_t)i * W + j;
+      const float depth = depths[0][p];
+      if (depth <= 0) continue;
+      float normal[3];
+      normalize3f(planes[0] + 3 * p, normal);
+      const float area = (float)((double)(-normal[2] / depth) * Ks[0]);
+      double S[9], X[3];
+      for (int k = 0; k < 9; ++k) S[k] = (double)depth * Kinv0[k];
+      const double q[3] = {(double)j, (double)i, 1};
+      mat3_vec(S, q, X);
+      for (int k = 0; k < 3; ++k) X[k] = X[k] - ts[k];
+      mat3_vec(R0t, X, X);
+      const float point[3] = {(float)X[0], (float)X[1], (float)X[2]};
+      int keep = 1;
+      for (int o = 1; o < n && keep; ++o) {
+        const double pd[3] = {point[0], point[1], point[2]};
+        double y[3];
+        mat3_vec(Rs + 9 * o, pd, y);
+        for (int k = 0; k < 3; ++k) y[k] = y[k] + ts[3 * o + k];
+        mat3_vec(Ks + 9 * o, y, y);
+        if (y[2] < 1e-8 || isnan(y[2])) continue;
+        const int iu = ref_pixel_index(y[0] / y[2] + 0.5);
+        const int iv = ref_pixel_index(y[1] / y[2] + 0.5);
+        const double depth_of_point = y[2];
+        if (!(iv >= 0 && iv < heights[o] && iu >= 0 && iu < widths[o])) continue;
+        const size_t po = (size_t)iv * widths[o] + (size_t)iu;
+        const float d_o = depths[o][po];
+        if ((double)d_o > (double)(1 - same_depth_threshold) * depth_of_point) {
+          float no[3];
+          normalize3f(planes[o] + 3 * po, no);
+          if (d_o == 0.0 || (double)(-no[2] / d_o) * Ks[9 * o] > (double)area) keep = 0;
+        }
+      }
+      if (!keep) continue;
+      if (count < capacity) {
+        for (int r = 0; r < 3; ++r) {
+          float s = 0.0f;
+          for (int k = 0; k < 3; ++k) s += Rinv[3 * r + k] * normal[k];
+          points[3 * count + r] = point[r];
+          normals[3 * count + r] = s;
+          out_colors[3 * count + r] = colors[0][3 * p + r];
+        }
+        out_labels[count] = labels[0][p];
+      }
+      count++;
+    }
+  }
+  return count;
+}
