@@ -276,6 +276,60 @@ int pm_pruner_download(pm_pruner* c, float* points, float* normals,
                        uint8_t* colors, uint8_t* labels);
 int pm_pruner_last_ms(pm_pruner* c, float* ms);
 
+/* ---- DepthmapFuser (fuser.h:28-75, src/fuser.cc:37-480) -------------- */
+
+/* Fuses the primary views' depthmaps into a point cloud: a pixel whose
+ * point is seen consistently (depth within max_depth_error in front and
+ * behind_depth_factor times that behind, forward-backward reprojection
+ * within max_reproj_error px, normals within max_normal_error degrees) by
+ * at least min_num_consistent observations (itself included) among its
+ * neighbour views' not-yet-fused pixels emits the component-wise medians,
+ * and all pixels it used are marked fused.  Then statistical outlier
+ * removal (mean distance to the sor_knn nearest points above mean +
+ * factor * std is dropped; knn 0 disables it).  Float arithmetic in the
+ * reference's order.  Views are fused in index order, pixels in row-major
+ * order: the result of the reference with one thread (its multi-threaded
+ * runs share the fused flags between threads without order, so their
+ * result depends on the schedule); set_num_threads is accepted and has no
+ * effect on the result.  Fused flags persist across pm_fuser_fuse calls, as
+ * in the reference.  Limits: 64 valid neighbours per primary view, k <= 32.
+ * Defaults: 3 views, 2 px, 0.01, 10 degrees, margin 5, k 12, factor 2.5,
+ * behind 0.3. */
+typedef struct pm_fuser pm_fuser;
+int pm_fuser_create(int device, pm_fuser** out);
+int pm_fuser_destroy(pm_fuser* f);
+int pm_fuser_set_min_num_consistent(pm_fuser* f, int n);
+int pm_fuser_set_max_reproj_error(pm_fuser* f, float px);
+int pm_fuser_set_max_depth_error(pm_fuser* f, float ratio);
+int pm_fuser_set_max_normal_error(pm_fuser* f, float degrees);
+/* negative margins clamp to 0 (fuser.cc:62-64) */
+int pm_fuser_set_border_margin(pm_fuser* f, int px);
+int pm_fuser_set_num_threads(pm_fuser* f, int n);
+/* negative k clamps to 0 (fuser.cc:68-71) */
+int pm_fuser_set_sor_params(pm_fuser* f, int knn, float stddev_factor);
+/* clamped to [0, 1] (fuser.cc:73-75) */
+int pm_fuser_set_behind_depth_factor(pm_fuser* f, float v);
+/* DepthmapFuser::AddView (fuser.cc:77-93): K, R row-major 3x3, t 3; depth
+ * H*W f32, normal H*W*3 f32 (camera frame), color H*W*3 u8, mask H*W u8 or
+ * NULL for none; neighbour view indices (entries outside [0, views) at fuse
+ * time are skipped); primary 0 makes the view a source only. */
+int pm_fuser_add_view(pm_fuser* f, const double* K, const double* R,
+                      const double* t, const float* depth,
+                      const float* normal, const uint8_t* color,
+                      const uint8_t* mask, int width, int height,
+                      const int32_t* neighbor_ids, int n_neighbors,
+                      int primary);
+/* DepthmapFuser::Fuse: *count receives the number of points, which stay on
+ * the device until pm_fuser_download copies count*3 floats of points and of
+ * normals and count*3 bytes of colours. */
+int pm_fuser_fuse(pm_fuser* f, int64_t* count);
+int pm_fuser_download(pm_fuser* f, float* points, float* normals,
+                      uint8_t* colors);
+/* Points before outlier removal, greedy rounds run, device time of the
+ * fusion and of the outlier removal in the last fuse. */
+int pm_fuser_stats(pm_fuser* f, int64_t* fused_before_sor, int64_t* rounds,
+                   float* fuse_ms, float* sor_ms);
+
 /* ---- Batched estimation ---------------------------------------------- */
 
 /* Many reference views ("jobs") of one pool of views, run together: each

@@ -7,7 +7,7 @@ include/opensfm_amd.h.
 """
 from .batch import DepthmapBatch  # noqa: F401
 from .pydense import (  # noqa: F401
-    DepthmapCleaner, DepthmapEstimator, DepthmapPruner, GPUDepthmapCleaner)
+    DepthmapCleaner, DepthmapEstimator, DepthmapFuser, DepthmapPruner, GPUDepthmapCleaner)
 
 __all__ = ["DepthmapEstimator", "DepthmapCleaner", "GPUDepthmapCleaner", "DepthmapPruner",
-           "DepthmapBatch"]
+           "DepthmapFuser", "DepthmapBatch"]

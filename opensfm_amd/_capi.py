@@ -105,6 +105,21 @@ SIGNATURES = {
     "pm_pruner_prune": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "pm_pruner_download": (C.c_int, [_vp, _fp, _fp, _u8p, _u8p]),
     "pm_pruner_last_ms": (C.c_int, [_vp, _fp]),
+    "pm_fuser_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "pm_fuser_destroy": (C.c_int, [_vp]),
+    "pm_fuser_set_min_num_consistent": (C.c_int, [_vp, C.c_int]),
+    "pm_fuser_set_max_reproj_error": (C.c_int, [_vp, C.c_float]),
+    "pm_fuser_set_max_depth_error": (C.c_int, [_vp, C.c_float]),
+    "pm_fuser_set_max_normal_error": (C.c_int, [_vp, C.c_float]),
+    "pm_fuser_set_border_margin": (C.c_int, [_vp, C.c_int]),
+    "pm_fuser_set_num_threads": (C.c_int, [_vp, C.c_int]),
+    "pm_fuser_set_sor_params": (C.c_int, [_vp, C.c_int, C.c_float]),
+    "pm_fuser_set_behind_depth_factor": (C.c_int, [_vp, C.c_float]),
+    "pm_fuser_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _fp, _fp, _u8p, _u8p, C.c_int,
+                                    C.c_int, _ip, C.c_int, C.c_int]),
+    "pm_fuser_fuse": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "pm_fuser_download": (C.c_int, [_vp, _fp, _fp, _u8p]),
+    "pm_fuser_stats": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _fp, _fp]),
     "pm_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pm_batch_destroy": (C.c_int, [_vp]),
     "pm_batch_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _u8p, _u8p, C.c_int, C.c_int,

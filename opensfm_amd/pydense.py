@@ -383,3 +383,104 @@ class DepthmapPruner:
         ms = C.c_float(0)
         _capi.check(self._lib.pm_pruner_last_ms(self._h, C.byref(ms)))
         return ms.value
+
+
+class DepthmapFuser:
+    """Fuses depthmaps into a point cloud by multi-view consistency with
+    outlier removal (DepthmapFuser, fuser.h:28-75; Python surface
+    pydense.pyi:78-100).  Views fuse in index order and pixels in row-major
+    order -- the reference's result with one thread; ``set_num_threads`` is
+    kept for the surface and does not change the result."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _capi.load()
+        h = C.c_void_p()
+        _capi.check(self._lib.pm_fuser_create(int(device), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.pm_fuser_destroy(h)
+            self._h = None
+
+    def set_min_num_consistent(self, n: int) -> None:
+        _capi.check(self._lib.pm_fuser_set_min_num_consistent(self._h, int(n)))
+
+    def set_max_reproj_error(self, px: float) -> None:
+        _capi.check(self._lib.pm_fuser_set_max_reproj_error(self._h, float(px)))
+
+    def set_max_depth_error(self, ratio: float) -> None:
+        _capi.check(self._lib.pm_fuser_set_max_depth_error(self._h, float(ratio)))
+
+    def set_max_normal_error(self, degrees: float) -> None:
+        _capi.check(self._lib.pm_fuser_set_max_normal_error(self._h, float(degrees)))
+
+    def set_border_margin(self, px: int) -> None:
+        _capi.check(self._lib.pm_fuser_set_border_margin(self._h, int(px)))
+
+    def set_num_threads(self, n: int) -> None:
+        _capi.check(self._lib.pm_fuser_set_num_threads(self._h, int(n)))
+
+    def set_sor_params(self, knn: int, stddev_factor: float) -> None:
+        _capi.check(self._lib.pm_fuser_set_sor_params(self._h, int(knn), float(stddev_factor)))
+
+    def set_behind_depth_factor(self, f: float) -> None:
+        _capi.check(self._lib.pm_fuser_set_behind_depth_factor(self._h, float(f)))
+
+    def add_view(self, K, R, t, depth, normal, color, mask, neighbor_ids,
+                 primary: bool = True) -> None:
+        """As DepthmapFuserWrapper::AddView (depthmap_bind.h:244-264): depth
+        (H, W), normal and color (H, W, 3); mask (H, W) uint8, or empty /
+        None for no mask."""
+        K = _mat(K, (3, 3), "K")
+        R = _mat(R, (3, 3), "R")
+        t = _mat(t, (3,), "t")
+        depth = np.asarray(depth)
+        if depth.ndim != 2:
+            raise ValueError("depth must be a 2-D array")
+        h, w = depth.shape
+        depth = np.ascontiguousarray(depth, dtype=np.float32)
+        normal = np.asarray(normal)
+        if normal.ndim != 3 or normal.shape[0] != h or normal.shape[1] != w:
+            raise ValueError("depth and normal must have matching shapes.")
+        color = np.asarray(color)
+        if color.ndim != 3 or color.shape[0] != h or color.shape[1] != w:
+            raise ValueError("depth and color must have matching shapes.")
+        if normal.shape[2] != 3 or color.shape[2] != 3:
+            raise ValueError("normal and color must have 3 channels")
+        normal = np.ascontiguousarray(normal, dtype=np.float32)
+        color = np.ascontiguousarray(color, dtype=np.uint8)
+        mptr = None
+        if mask is not None and np.asarray(mask).size > 0:
+            mask = np.ascontiguousarray(np.asarray(mask), dtype=np.uint8)
+            if mask.shape != (h, w):
+                raise ValueError("mask must be empty or match the depth map's shape")
+            mptr = _capi.ptr(mask, C.c_uint8)
+        nb = np.ascontiguousarray(np.asarray(neighbor_ids, np.int32).reshape(-1))
+        _capi.check(self._lib.pm_fuser_add_view(
+            self._h, _capi.ptr(K, C.c_double), _capi.ptr(R, C.c_double),
+            _capi.ptr(t, C.c_double), _capi.ptr(depth, C.c_float),
+            _capi.ptr(normal, C.c_float), _capi.ptr(color, C.c_uint8), mptr, w, h,
+            _capi.ptr(nb, C.c_int32), int(nb.size), int(bool(primary))))
+
+    def fuse(self) -> list:
+        """[points (N, 3) f32, normals (N, 3) f32, colors (N, 3) u8]."""
+        n = C.c_int64(0)
+        _capi.check(self._lib.pm_fuser_fuse(self._h, C.byref(n)))
+        m = int(n.value)
+        points = np.zeros((m, 3), np.float32)
+        normals = np.zeros((m, 3), np.float32)
+        colors = np.zeros((m, 3), np.uint8)
+        _capi.check(self._lib.pm_fuser_download(
+            self._h, _capi.ptr(points, C.c_float), _capi.ptr(normals, C.c_float),
+            _capi.ptr(colors, C.c_uint8)))
+        return [points, normals, colors]
+
+    def stats(self) -> dict:
+        nf, rounds = C.c_int64(0), C.c_int64(0)
+        fms, sms = C.c_float(0), C.c_float(0)
+        _capi.check(self._lib.pm_fuser_stats(self._h, C.byref(nf), C.byref(rounds),
+                                             C.byref(fms), C.byref(sms)))
+        return {"fused_before_sor": nf.value, "rounds": rounds.value,
+                "fuse_ms": fms.value, "sor_ms": sms.value}

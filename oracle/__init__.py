@@ -99,6 +99,14 @@ def lib():
                             C.POINTER(C.c_void_p), _i32p, _i32p, C.c_float, C.c_long,
                             _f32p, _f32p, _u8p, _u8p]
     L.orc_prune.restype = C.c_long
+    L.orc_fuse.argtypes = [C.c_int, _f64p, _f64p, _f64p] + [C.POINTER(C.c_void_p)] * 4 + [
+        _i32p, _i32p, _i32p, _i32p, _i32p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int,
+        C.c_int, C.c_float, C.c_float, C.c_long, _f32p, _f32p, _u8p, C.POINTER(C.c_long)]
+    L.orc_fuse.restype = C.c_long
+    L.orc_sor.argtypes = [C.c_long, _f32p, _f32p, _u8p, C.c_int, C.c_float, _f32p]
+    L.orc_sor.restype = C.c_long
+    L.orc_fuse_inverse3f.argtypes = [_f32p, _f32p]
+    L.orc_fuse_inverse3f.restype = None
     _lib = L
     return L
 
@@ -220,6 +228,78 @@ def prune(views, same_depth_threshold=0.01):
     if cnt < 0:
         raise ValueError("prune needs at least one view")
     return pts[:cnt], nrm[:cnt], col[:cnt], lab[:cnt]
+
+
+FUSER_DEFAULTS = dict(min_num_consistent=3, max_reproj_error=2.0, max_depth_error=0.01,
+                      max_normal_error=10.0, border_margin=5, sor_knn=12,
+                      sor_stddev_factor=2.5, behind_depth_factor=0.3)
+
+
+def fuse(views, **params):
+    """DepthmapFuser::Fuse (fuser.cc:113-480) with one thread.  views: list of
+    (K, R, t, depth (H, W) f32, normal (H, W, 3) f32, color (H, W, 3) u8,
+    mask (H, W) u8 or None, neighbor_ids, primary=True); params: the
+    setters' arguments (FUSER_DEFAULTS).  Returns (points, normals, colors)
+    and the fused count before outlier removal."""
+    p = dict(FUSER_DEFAULTS)
+    p.update(params)
+    n = len(views)
+    if n == 0:
+        return (np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32),
+                np.zeros((0, 3), np.uint8)), 0
+    Ks = np.stack([np.asarray(v[0], np.float64).reshape(9) for v in views])
+    Rs = np.stack([np.asarray(v[1], np.float64).reshape(9) for v in views])
+    ts = np.stack([np.asarray(v[2], np.float64).reshape(3) for v in views])
+    depths = [np.ascontiguousarray(v[3], np.float32) for v in views]
+    normals = [np.ascontiguousarray(v[4], np.float32) for v in views]
+    colors = [np.ascontiguousarray(v[5], np.uint8) for v in views]
+    masks = [None if v[6] is None or np.asarray(v[6]).size == 0
+             else np.ascontiguousarray(v[6], np.uint8) for v in views]
+    nbs = [np.asarray(v[7], np.int32).reshape(-1) for v in views]
+    prim = np.array([int(bool(v[8])) if len(v) > 8 else 1 for v in views], np.int32)
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum([len(x) for x in nbs])
+    ids = np.ascontiguousarray(np.concatenate(nbs + [np.zeros(1, np.int32)]), np.int32)
+    arr = lambda xs: (C.c_void_p * n)(*[0 if x is None else x.ctypes.data for x in xs])  # noqa: E731
+    ws = np.array([d.shape[1] for d in depths], np.int32)
+    hs = np.array([d.shape[0] for d in depths], np.int32)
+    cap = int(sum(d.size for d, pr in zip(depths, prim) if pr)) + 1
+    pts = np.zeros((cap, 3), np.float32)
+    nrm = np.zeros((cap, 3), np.float32)
+    col = np.zeros((cap, 3), np.uint8)
+    nf = C.c_long(0)
+    cnt = lib().orc_fuse(
+        n, np.ascontiguousarray(Ks).reshape(-1), np.ascontiguousarray(Rs).reshape(-1),
+        np.ascontiguousarray(ts).reshape(-1), arr(depths), arr(normals), arr(colors),
+        arr(masks), ws, hs, off, ids, prim, int(p["min_num_consistent"]),
+        float(p["max_reproj_error"]), float(p["max_depth_error"]), float(p["max_normal_error"]),
+        int(p["border_margin"]), int(p["sor_knn"]), float(p["sor_stddev_factor"]),
+        float(p["behind_depth_factor"]), cap, pts.reshape(-1), nrm.reshape(-1),
+        col.reshape(-1), C.byref(nf))
+    assert cnt >= 0
+    return (pts[:cnt].copy(), nrm[:cnt].copy(), col[:cnt].copy()), int(nf.value)
+
+
+def sor(points, normals, colors, knn=12, factor=2.5):
+    """Statistical outlier removal of the fuser (fuser.cc:369-478) by brute
+    force: returns the kept (points, normals, colors) and every point's mean
+    distance to its knn nearest others."""
+    pts = np.ascontiguousarray(points, np.float32).copy()
+    nrm = np.ascontiguousarray(normals, np.float32).copy()
+    col = np.ascontiguousarray(colors, np.uint8).copy()
+    mk = np.zeros(max(len(pts), 1), np.float32)
+    kept = lib().orc_sor(len(pts), pts.reshape(-1), nrm.reshape(-1), col.reshape(-1),
+                         int(knn), float(factor), mk)
+    return (pts[:kept], nrm[:kept], col[:kept]), mk[:len(pts)]
+
+
+def inverse3f(m):
+    """Eigen's fixed-size 3x3 float inverse (cofactor form) as the fuser
+    uses it (Kf.inverse(), fuser.cc:162)."""
+    a = np.ascontiguousarray(np.asarray(m, np.float32).reshape(9))
+    out = np.zeros(9, np.float32)
+    lib().orc_fuse_inverse3f(a, out)
+    return out.reshape(3, 3)
 
 
 class OracleEstimator:

@@ -29,6 +29,7 @@
  *   - LinearInterpolation treats a NaN coordinate as outside the image (the
  *     reference casts NaN to int, which is undefined behaviour).
  */
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -894,4 +895,386 @@ long orc_prune(int n, const double* Ks, const double* Rs, const double* ts,
     }
   }
   return count;
+}
+
+/* ------------------------------------------------------------------------
+ * DepthmapFuser::Fuse  fuser.cc:113-480, single-threaded (SetNumThreads(1)):
+ * primary views in index order, pixels in row-major order inside the border
+ * margin, neighbours in the order given.  A reference pixel gathers the
+ * neighbour pixels its point projects to (rounded) that pass the depth
+ * (asymmetric), forward-backward reprojection and normal tests and are not
+ * yet fused; with at least min_consistent observations (itself included)
+ * it emits the component-wise medians and marks itself and the used pixels
+ * fused.  Then statistical outlier removal over the fused points (mean
+ * distance to the k nearest neighbours against mean + factor * std).
+ *
+ * Float arithmetic as the reference's Eigen 3.4 fixed-size expressions
+ * (third-party, absent from the reference tree; Ubuntu 24.04 ships 3.4.0):
+ * a 3-term product/dot reduces as c0 + (c1 + c2) (the unrolled reduction
+ * splits the range in halves), a 2-term one as c0 + c1, (s * A) * v is
+ * sum_k (s * A_ik) * v_k, A.inverse() is the cofactor form: cofactors of
+ * column 0, det = c00 m00 + (c10 m10 + c20 m20), invdet = 1 / det,
+ * inverse(i, j) = cofactor(j, i) * invdet; v.norm() = sqrt(v0^2 + (v1^2 +
+ * v2^2)); v /= s divides.  The k-nearest distances are vlfeat's float L2
+ * ((dx^2 + dy^2) + dz^2, dimension 3 takes its scalar tail), exact search
+ * (no comparison limit). */
+
+static float fu_dot3(const float* a, const float* b) {
+  return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]);
+}
+
+static void fu_mv3(const float* M, const float* x, float* y) {
+  float t[3];
+  for (int i = 0; i < 3; ++i) t[i] = fu_dot3(M + 3 * i, x);
+  for (int i = 0; i < 3; ++i) y[i] = t[i];
+}
+
+static float fu_cof(const float* m, int i, int j) {
+  const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+  return m[3 * i1 + j1] * m[3 * i2 + j2] - m[3 * i1 + j2] * m[3 * i2 + j1];
+}
+
+void orc_fuse_inverse3f(const float* m, float* r) {
+  const float c00 = fu_cof(m, 0, 0), c10 = fu_cof(m, 1, 0), c20 = fu_cof(m, 2, 0);
+  const float det = c00 * m[0] + (c10 * m[3] + c20 * m[6]);
+  const float invdet = 1.0f / det;
+  r[3] = fu_cof(m, 0, 1) * invdet;
+  r[4] = fu_cof(m, 1, 1) * invdet;
+  r[6] = fu_cof(m, 0, 2) * invdet;
+  r[5] = fu_cof(m, 2, 1) * invdet;
+  r[7] = fu_cof(m, 1, 2) * invdet;
+  r[8] = fu_cof(m, 2, 2) * invdet;
+  r[0] = c00 * invdet;
+  r[1] = c10 * invdet;
+  r[2] = c20 * invdet;
+}
+
+/* Rinv * (d * Kinv * (x, y, 1) - t) */
+static void fu_backproject(const float* Kinv, const float* Rinv, const float* t, float d,
+                           float x, float y, float* out) {
+  float w[3];
+  for (int i = 0; i < 3; ++i) {
+    const float a = d * Kinv[3 * i], b = d * Kinv[3 * i + 1], c = d * Kinv[3 * i + 2];
+    w[i] = a * x + (b * y + c * 1.0f);
+    w[i] = w[i] - t[i];
+  }
+  fu_mv3(Rinv, w, out);
+}
+
+/* K * (R * X + t) */
+static void fu_project(const float* K, const float* R, const float* t, const float* X,
+                       float* out) {
+  float a[3];
+  fu_mv3(R, X, a);
+  for (int i = 0; i < 3; ++i) a[i] = a[i] + t[i];
+  fu_mv3(K, a, out);
+}
+
+static float fu_norm3(const float* v) { return sqrtf(v[0] * v[0] + (v[1] * v[1] + v[2] * v[2])); }
+
+/* static_cast<int>(std::round(x)): outside int (or NaN) the conversion is
+ * undefined in C++; x86 returns INT_MIN, which every bounds test rejects. */
+static int fu_round_int(float x) {
+  const float r = roundf(x);
+  if (!(fabsf(r) < 2147483648.0f)) return INT_MIN;
+  return (int)r;
+}
+
+/* BilinearDepth (fuser.cc:95-111) over foundation::BilinearInterpolation
+ * (foundation/interpolation.h:15-32). */
+static float fu_bilinear_depth(const float* D, int W, int H, float u, float v) {
+  const int x0 = (int)floorf(u), y0 = (int)floorf(v);
+  const int x1 = x0 + 1, y1 = y0 + 1;
+  if (x0 < 0 || y0 < 0 || x1 >= W || y1 >= H) return -1.0f;
+  if (D[(size_t)y0 * W + x0] <= 0.0f || D[(size_t)y0 * W + x1] <= 0.0f ||
+      D[(size_t)y1 * W + x0] <= 0.0f || D[(size_t)y1 * W + x1] <= 0.0f)
+    return -1.0f;
+  if (u < 0.0f || u >= (float)(W - 1) || v < 0.0f || v >= (float)(H - 1)) return -1.0f;
+  const int ix = (int)u, iy = (int)v;
+  const float dx = u - (float)ix, dy = v - (float)iy;
+  const float v00 = D[(size_t)iy * W + ix], v01 = D[(size_t)iy * W + ix + 1];
+  const float v10 = D[(size_t)(iy + 1) * W + ix], v11 = D[(size_t)(iy + 1) * W + ix + 1];
+  const float a = (1 - dx) * v00 + dx * v01;
+  const float b = (1 - dx) * v10 + dx * v11;
+  return (1 - dy) * a + dy * b;
+}
+
+/* std::nth_element(v, v + n/2, v + n) then v[n/2]: the (n/2)-th smallest
+ * value; ties are broken by position (which only decides between +0 and -0). */
+static float fu_median_f(const float* v, int n) {
+  const int k = n / 2;
+  for (int i = 0; i < n; ++i) {
+    int rank = 0;
+    for (int j = 0; j < n; ++j)
+      if (v[j] < v[i] || (v[j] == v[i] && j < i)) ++rank;
+    if (rank == k) return v[i];
+  }
+  return v[k];  /* unreachable without NaN */
+}
+
+static unsigned char fu_median_u8(const unsigned char* v, int n) {
+  const int k = n / 2;
+  for (int i = 0; i < n; ++i) {
+    int rank = 0;
+    for (int j = 0; j < n; ++j)
+      if (v[j] < v[i] || (v[j] == v[i] && j < i)) ++rank;
+    if (rank == k) return v[i];
+  }
+  return v[k];
+}
+
+typedef struct {
+  float K[9], R[9], t[3], Kinv[9], Rinv[9];
+} FuCam;
+
+static void fu_camera(const double* K, const double* R, const double* t, FuCam* c) {
+  for (int k = 0; k < 9; ++k) {
+    c->K[k] = (float)K[k];
+    c->R[k] = (float)R[k];
+  }
+  for (int k = 0; k < 3; ++k) c->t[k] = (float)t[k];
+  orc_fuse_inverse3f(c->K, c->Kinv);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) c->Rinv[3 * i + j] = c->R[3 * j + i];
+}
+
+/* Statistical outlier removal (fuser.cc:369-478) over n points in place;
+ * returns the kept count.  mean_knn (n floats) receives each point's mean
+ * distance to its knn nearest others. */
+long orc_sor(long n, float* points, float* normals, unsigned char* colors, int knn,
+             float factor, float* mean_knn) {
+  if (knn <= 0 || n <= (long)knn + 1) return n;
+  const int K = knn;
+  float* bd = (float*)malloc(sizeof(float) * (K + 1));
+  long* bi = (long*)malloc(sizeof(long) * (K + 1));
+  for (long i = 0; i < n; ++i) {
+    const float* q = points + 3 * i;
+    int m = 0;
+    for (long j = 0; j < n; ++j) {
+      const float* p = points + 3 * j;
+      float acc = 0.0f;
+      for (int k = 0; k < 3; ++k) {
+        const float d = q[k] - p[k];
+        acc += d * d;
+      }
+      /* keep the K + 1 smallest by (distance, index), sorted */
+      if (m < K + 1) {
+        int s = m++;
+        while (s > 0 && bd[s - 1] > acc) {
+          bd[s] = bd[s - 1];
+          bi[s] = bi[s - 1];
+          --s;
+        }
+        bd[s] = acc;
+        bi[s] = j;
+      } else if (acc < bd[K]) {
+        int s = K;
+        while (s > 0 && bd[s - 1] > acc) {
+          bd[s] = bd[s - 1];
+          bi[s] = bi[s - 1];
+          --s;
+        }
+        bd[s] = acc;
+        bi[s] = j;
+      }
+    }
+    float sum = 0.0f;
+    int count = 0;
+    for (int j = 0; j < m && count < K; ++j) {
+      if (bi[j] == i) continue;
+      sum += sqrtf(bd[j]);
+      ++count;
+    }
+    mean_knn[i] = count > 0 ? sum / (float)count : 1e30f;
+  }
+  free(bd);
+  free(bi);
+  double sum_d = 0, sum_d2 = 0;
+  for (long i = 0; i < n; ++i) {
+    const double d = mean_knn[i];
+    if (d < 1e20f) {
+      sum_d += d;
+      sum_d2 += d * d;
+    }
+  }
+  const double mean_d = sum_d / n;
+  const double var_d = sum_d2 / n - mean_d * mean_d;
+  const double std_d = sqrt(var_d > 0.0 ? var_d : 0.0);
+  const float threshold = (float)(mean_d + factor * std_d);
+  long kept = 0;
+  for (long i = 0; i < n; ++i) {
+    if (mean_knn[i] <= threshold) {
+      for (int k = 0; k < 3; ++k) {
+        points[3 * kept + k] = points[3 * i + k];
+        normals[3 * kept + k] = normals[3 * i + k];
+        colors[3 * kept + k] = colors[3 * i + k];
+      }
+      ++kept;
+    }
+  }
+  return kept;
+}
+
+/* views: n; per view K, R (row-major 3x3), t, depth H*W, normal H*W*3,
+ * color H*W*3, mask H*W or NULL, neighbours nb_ids[nb_off[v] .. nb_off[v+1]),
+ * primary flag.  Parameters as the setters take them (fuser.cc:52-81).
+ * Returns the number of points written (after SOR), or -1 when capacity is
+ * too small (the fused count before SOR is then *n_fused). */
+long orc_fuse(int n, const double* Ks, const double* Rs, const double* ts,
+              const float* const* depths, const float* const* normals,
+              const unsigned char* const* colors, const unsigned char* const* masks,
+              const int* widths, const int* heights, const int* nb_off, const int* nb_ids,
+              const int* primary, int min_consistent, float max_reproj_px,
+              float max_depth_error, float max_normal_deg, int border_margin, int sor_knn,
+              float sor_factor, float behind_factor, long capacity, float* out_points,
+              float* out_normals, unsigned char* out_colors, long* n_fused) {
+  const float max_reproj_sq = max_reproj_px * max_reproj_px;
+  const float min_cos = cosf(max_normal_deg * (float)3.14159265358979323846 / 180.0f);
+  const int bm = border_margin > 0 ? border_margin : 0;
+  const int knn = sor_knn > 0 ? sor_knn : 0;
+  const float behind = behind_factor < 0.0f ? 0.0f : (behind_factor > 1.0f ? 1.0f : behind_factor);
+  *n_fused = 0;
+  if (n <= 0) return 0;
+  FuCam* cams = (FuCam*)malloc(sizeof(FuCam) * n);
+  unsigned char** fused = (unsigned char**)malloc(sizeof(unsigned char*) * n);
+  for (int v = 0; v < n; ++v) {
+    fu_camera(Ks + 9 * v, Rs + 9 * v, ts + 3 * v, &cams[v]);
+    fused[v] = (unsigned char*)calloc((size_t)widths[v] * heights[v], 1);
+  }
+  int max_nb = 0;
+  for (int v = 0; v < n; ++v)
+    if (nb_off[v + 1] - nb_off[v] > max_nb) max_nb = nb_off[v + 1] - nb_off[v];
+  const int cap_obs = max_nb + 1;
+  float* pts = (float*)malloc(sizeof(float) * 3 * cap_obs);
+  float* nrm = (float*)malloc(sizeof(float) * 3 * cap_obs);
+  unsigned char* clr = (unsigned char*)malloc(3 * cap_obs);
+  int* used_v = (int*)malloc(sizeof(int) * cap_obs);
+  long* used_p = (long*)malloc(sizeof(long) * cap_obs);
+  float* comp = (float*)malloc(sizeof(float) * cap_obs);
+  unsigned char* comp8 = (unsigned char*)malloc(cap_obs);
+  long count = 0;
+  int overflow = 0;
+  for (int vi = 0; vi < n; ++vi) {
+    if (!primary[vi]) continue;
+    const FuCam* A = &cams[vi];
+    const int W = widths[vi], H = heights[vi];
+    for (int r = bm; r < H - bm; ++r) {
+      for (int c = bm; c < W - bm; ++c) {
+        const size_t p = (size_t)r * W + c;
+        if (fused[vi][p]) continue;
+        if (masks[vi] && masks[vi][p] == 0) continue;
+        const float d = depths[vi][p];
+        if (d <= 0.0f) continue;
+        float P[3], nc[3], nw[3];
+        fu_backproject(A->Kinv, A->Rinv, A->t, d, (float)c, (float)r, P);
+        for (int k = 0; k < 3; ++k) nc[k] = normals[vi][3 * p + k];
+        const float nlen = fu_norm3(nc);
+        if (nlen < 1e-6f) continue;
+        for (int k = 0; k < 3; ++k) nc[k] = nc[k] / nlen;
+        fu_mv3(A->Rinv, nc, nw);
+        int m = 0, nu = 0;
+        for (int k = 0; k < 3; ++k) {
+          pts[k] = P[k];
+          nrm[k] = nw[k];
+          clr[k] = colors[vi][3 * p + k];
+        }
+        m = 1;
+        for (int s = nb_off[vi]; s < nb_off[vi + 1]; ++s) {
+          const int o = nb_ids[s];
+          if (o < 0 || o >= n) continue;
+          const FuCam* S = &cams[o];
+          const int Ws = widths[o], Hs = heights[o];
+          float proj[3];
+          fu_project(S->K, S->R, S->t, P, proj);
+          if (proj[2] < 1e-6f) continue;
+          const float su = proj[0] / proj[2], sv = proj[1] / proj[2];
+          const int sc = fu_round_int(su), sr = fu_round_int(sv);
+          if (sc < bm || sc >= Ws - bm || sr < bm || sr >= Hs - bm) continue;
+          const size_t sp = (size_t)sr * Ws + sc;
+          if (masks[o] && masks[o][sp] == 0) continue;
+          if (fused[o][sp]) continue;
+          float sd = fu_bilinear_depth(depths[o], Ws, Hs, su, sv);
+          if (sd <= 0.0f) {
+            sd = depths[o][sp];
+            if (sd <= 0.0f) continue;
+          }
+          const float dp = proj[2];
+          const float front = max_depth_error * dp;
+          const float back = front * behind;
+          const float diff = sd - dp;
+          if (diff > front || diff < -back) continue;
+          float SP[3], rp[3];
+          fu_backproject(S->Kinv, S->Rinv, S->t, sd, (float)sc, (float)sr, SP);
+          fu_project(A->K, A->R, A->t, SP, rp);
+          if (rp[2] < 1e-6f) continue;
+          const float e0 = rp[0] / rp[2] - (float)c, e1 = rp[1] / rp[2] - (float)r;
+          const float err = e0 * e0 + e1 * e1;
+          if (err > max_reproj_sq) continue;
+          float sn[3], snw[3];
+          for (int k = 0; k < 3; ++k) sn[k] = normals[o][3 * sp + k];
+          const float snlen = fu_norm3(sn);
+          if (snlen < 1e-6f) continue;
+          for (int k = 0; k < 3; ++k) sn[k] = sn[k] / snlen;
+          fu_mv3(S->Rinv, sn, snw);
+          const float cosang = fu_dot3(nw, snw);
+          if (cosang < min_cos) continue;
+          for (int k = 0; k < 3; ++k) {
+            pts[3 * m + k] = SP[k];
+            nrm[3 * m + k] = snw[k];
+            clr[3 * m + k] = colors[o][3 * sp + k];
+          }
+          ++m;
+          used_v[nu] = o;
+          used_p[nu] = (long)sp;
+          ++nu;
+        }
+        if (m < min_consistent) continue;
+        float mp[3], mn[3];
+        unsigned char mc[3];
+        for (int k = 0; k < 3; ++k) {
+          for (int j = 0; j < m; ++j) comp[j] = pts[3 * j + k];
+          mp[k] = fu_median_f(comp, m);
+        }
+        for (int k = 0; k < 3; ++k) {
+          for (int j = 0; j < m; ++j) comp[j] = nrm[3 * j + k];
+          mn[k] = fu_median_f(comp, m);
+        }
+        const float mlen = fu_norm3(mn);
+        if (mlen > 1e-6f)
+          for (int k = 0; k < 3; ++k) mn[k] = mn[k] / mlen;
+        for (int k = 0; k < 3; ++k) {
+          for (int j = 0; j < m; ++j) comp8[j] = clr[3 * j + k];
+          mc[k] = fu_median_u8(comp8, m);
+        }
+        if (count < capacity) {
+          for (int k = 0; k < 3; ++k) {
+            out_points[3 * count + k] = mp[k];
+            out_normals[3 * count + k] = mn[k];
+            out_colors[3 * count + k] = mc[k];
+          }
+        } else {
+          overflow = 1;
+        }
+        ++count;
+        fused[vi][p] = 1;
+        for (int u = 0; u < nu; ++u) fused[used_v[u]][used_p[u]] = 1;
+      }
+    }
+  }
+  for (int v = 0; v < n; ++v) free(fused[v]);
+  free(fused);
+  free(cams);
+  free(pts);
+  free(nrm);
+  free(clr);
+  free(used_v);
+  free(used_p);
+  free(comp);
+  free(comp8);
+  *n_fused = count;
+  if (overflow) return -1;
+  float* mk = (float*)malloc(sizeof(float) * (count > 0 ? count : 1));
+  const long kept = orc_sor(count, out_points, out_normals, out_colors, knn, sor_factor, mk);
+  free(mk);
+  return kept;
 }
