@@ -209,6 +209,76 @@ def batch_bench(n_views, steps=3, seed=4000):
     }
 
 
+def fusion_views(scene, seed=77):
+    """The 1600x1200 scene's 9 views as fuser inputs: ground-truth depth with
+    0.2 % noise and 2 % holes, normals from the ground truth's depth
+    gradient, every view a neighbour (the pipeline's cluster lists)."""
+    from opensfm_amd import scenes
+    rng = np.random.default_rng(seed)
+    n = len(scene.views)
+    views = []
+    for v in scene.views:
+        d = scenes.noisy_depth(v, rng)
+        views.append([v.K, v.R, v.t, d, scenes.normals_from_depth(v.K, v.depth),
+                      np.stack([v.image] * 3, -1), None, list(range(n)), True])
+    return views
+
+
+def fusion_bench(views):
+    """The steps after estimation: DepthmapFuser over all views, 4 consistent
+    views, margin 10, with and without outlier removal (k 12); then
+    DepthmapPruner of view 0 against the others."""
+    from opensfm_amd import DepthmapFuser, DepthmapPruner
+    mpix = sum(v[3].size for v in views) / 1e6
+    out = {"workload": f"{len(views)} views {views[0][3].shape[1]}x{views[0][3].shape[0]}, "
+                       "all views neighbours, 4 consistent, margin 10"}
+    for knn in (-1, 0, 12):           # -1: an untimed warm-up run
+        f = DepthmapFuser()
+        f.set_min_num_consistent(4)
+        f.set_border_margin(10)
+        f.set_sor_params(max(knn, 0), 2.5)
+        for v in views:
+            f.add_view(*v)
+        t0 = time.perf_counter()
+        pts = f.fuse()[0]
+        wall = time.perf_counter() - t0
+        st = f.stats()
+        if knn < 0:
+            continue
+        out[f"gpu_sor{knn}"] = {"points": len(pts), "fused_before_sor": st["fused_before_sor"],
+                                "rounds": st["rounds"], "fuse_ms": round(st["fuse_ms"], 2),
+                                "sor_ms": round(st["sor_ms"], 2), "wall_ms": round(wall * 1e3, 2),
+                                "mpix_s": round(mpix / wall, 2)}
+    p = DepthmapPruner()
+    for v in views:
+        p.add_view(*v[:6], np.zeros(v[3].shape, np.uint8))
+    p.prune()
+    t0 = time.perf_counter()
+    kept = len(p.prune()[0])
+    out["prune"] = {"kept": kept, "gpu_ms": round(p.last_ms(), 3),
+                    "gpu_wall_ms": round((time.perf_counter() - t0) * 1e3, 2)}
+    return out
+
+
+def fusion_cpu_baseline(views, primary=2):
+    """CPU baseline of the fusion section: the oracle's one-thread fuser over
+    the first `primary` views (all views as sources) and the oracle pruner
+    of view 0, on this host."""
+    import oracle
+    sub = [v[:8] + [k < primary] for k, v in enumerate(views)]
+    t0 = time.perf_counter()
+    _, nf = oracle.fuse(sub, min_num_consistent=4, border_margin=10, sor_knn=0)
+    fuse_s = time.perf_counter() - t0
+    mpix = sum(v[3].size for v in views[:primary]) / 1e6
+    pv = [tuple(v[:6]) + (np.zeros(v[3].shape, np.uint8),) for v in views]
+    t0 = time.perf_counter()
+    kept = len(oracle.prune(pv)[0])
+    prune_s = time.perf_counter() - t0
+    return {"fuse": {"primary_views": primary, "points": nf, "s": round(fuse_s, 2),
+                     "mpix_s": round(mpix / fuse_s, 3), "cores": 1, "kind": "port"},
+            "prune": {"kept": kept, "ms": round(prune_s * 1e3, 1), "cores": 1, "kind": "port"}}
+
+
 def learned_bench(width=2000, height=1504, views=4, seed=5000):
     """Learned initialiser vs plain PatchMatch on nadir views of fractal
     terrain (a 4 x 4 camera grid, 10 neighbours each; the first `views`
@@ -242,6 +312,8 @@ def main():
                          "views (0: skip; 128 is the full batched configuration)")
     ap.add_argument("--learned-views", type=int, default=4, metavar="N",
                     help="also time the learned initialiser on N nadir views (0: skip)")
+    ap.add_argument("--no-fusion", action="store_true",
+                    help="skip the fusion / pruning section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (torch) so that "
                          "profiler byte counters can be scaled against a known size")
@@ -394,6 +466,14 @@ def main():
     if args.batch_views > 0:
         out["batched"] = batch_bench(args.batch_views)
         log(f"batched: {out['batched']}")
+    if not args.no_fusion:
+        fv = fusion_views(scene)
+        out["fusion"] = fusion_bench(fv)
+        if not args.no_cpu_baseline:
+            out["fusion"]["cpu_baseline"] = fusion_cpu_baseline(fv)
+            out["fusion"]["fuse_gpu_over_cpu"] = round(
+                out["fusion"]["gpu_sor0"]["mpix_s"] / out["fusion"]["cpu_baseline"]["fuse"]["mpix_s"], 1)
+        log(f"fusion: {out['fusion']}")
     if args.learned_views > 0:
         out["learned_init"] = learned_bench(views=args.learned_views)
         log(f"learned initialiser: {out['learned_init']}")

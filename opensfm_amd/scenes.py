@@ -374,6 +374,28 @@ def scene_config2(seed=2000, width=1600, height=1200):
                       name="cfg2")
 
 
+def normals_from_depth(K, depth):
+    """Camera-frame normals (facing the camera, unnormalised) of a depthmap:
+    the cross product of the back-projected point's horizontal and vertical
+    central differences; (0, 0, -1) where that vanishes."""
+    h, w = depth.shape
+    jj, ii = np.meshgrid(np.arange(w, dtype=np.float64), np.arange(h, dtype=np.float64))
+    P = (np.linalg.inv(K) @ np.stack([jj.ravel(), ii.ravel(), np.ones(jj.size)])).T
+    P = (P * depth.reshape(-1, 1).astype(np.float64)).reshape(h, w, 3)
+    n = np.cross(np.gradient(P, axis=1), np.gradient(P, axis=0))
+    n[n[..., 2] > 0] *= -1
+    n[np.linalg.norm(n, axis=-1) == 0] = (0, 0, -1)
+    return n.astype(np.float32)
+
+
+def noisy_depth(view, rng, noise=0.002, holes=0.02):
+    """The view's ground-truth depth with relative Gaussian noise and holes
+    (depth 0), as float32."""
+    d = view.depth.astype(np.float32) * (1 + rng.normal(0, noise, view.depth.shape)).astype(np.float32)
+    d[rng.random(d.shape) < holes] = 0
+    return d.astype(np.float32)
+
+
 def depth_accuracy(depth, gt, min_depth=None, max_depth=None, rel=0.01):
     """Fraction of ground-truth-valid pixels whose estimate lies within `rel`
     relative error (depth 0 counts as a miss)."""
