@@ -330,6 +330,58 @@ int pm_fuser_download(pm_fuser* f, float* points, float* normals,
 int pm_fuser_stats(pm_fuser* f, int64_t* fused_before_sor, int64_t* rounds,
                    float* fuse_ms, float* sor_ms);
 
+/* ---- Neighbour selection (sfm/dense_helpers.h:40-67) ----------------- */
+
+/* BuildCovisibilityGraph (dense_helpers.cc:30-326) over plain arrays:
+ * n_points landmarks (FP64 x, y, z) with their observing shots in CSR form
+ * (obs_off[n_points + 1], obs_shot: processable shot indices; entries
+ * outside [0, n_shots) are not processable).  Points seen by >= 2 shots are
+ * merged with their k = max(1, min(fuse_knn, m - 1)) nearest neighbours
+ * (float L2, ties to the lower index) within fuse_radius_factor times the
+ * median of the per-point (k/2)-th nearest distances (union-find) into
+ * super-points: FP64 centroid in member order, sorted shot union, members.
+ * Edges are the shot pairs (lo < hi) sharing a super-point, weighted by the
+ * count.  Super-points are numbered by their smallest member and edges in
+ * (lo, hi) order (the reference's orders come from hash-map iteration).
+ * Limits: n_shots <= 16384, k <= 32. */
+typedef struct pm_covis pm_covis;
+int pm_covis_create(int device, pm_covis** out);
+int pm_covis_destroy(pm_covis* c);
+int pm_covis_build(pm_covis* c, int64_t n_points, const double* coords,
+                   const int64_t* obs_off, const int32_t* obs_shot,
+                   int n_shots, int fuse_knn, double fuse_radius_factor,
+                   int64_t* n_super_points, int64_t* n_vis,
+                   int64_t* n_edges);
+/* point_sp[n_points] (-1: not merged), sp_coords[3 n_sp],
+ * sp_member_off[n_sp + 1] + sp_members (landmark indices), sp_vis_off
+ * [n_sp + 1] + sp_vis[n_vis], edges[2 n_edges], weights[n_edges], the
+ * scale (median k-nearest distance).  Any output may be NULL. */
+int pm_covis_download(pm_covis* c, int32_t* point_sp, double* sp_coords,
+                      int64_t* sp_member_off, int32_t* sp_members,
+                      int64_t* sp_vis_off, int32_t* sp_vis, int32_t* edges,
+                      int32_t* weights, float* scale);
+/* SelectNeighbors (dense_helpers.cc:331-512): camera centres origins
+ * [3 n_shots], super-points (centroid, visibility CSR, track count).  Per
+ * shot pair: count and score (tracks with the angle at the centroid in
+ * (theta_min, theta_max)) and the angle sum (cos rounded to 2^-32, fixed
+ * point: independent of order; the reference sums FP64 in a
+ * thread-dependent order).  best[i] = i, then up to num_neighbors shots
+ * with count > min_point_best, mean angle within [theta_min, theta_max] and
+ * score > min_point_best, by score descending (ties: lower index); all[i] =
+ * best[i], then the shots with count > min_point_all in index order.
+ * Limits: n_shots <= 16384, num_neighbors <= 64. */
+int pm_covis_select_neighbors(pm_covis* c, int n_shots, const double* origins,
+                              int64_t n_sp, const double* sp_coords,
+                              const int64_t* sp_vis_off, const int32_t* sp_vis,
+                              const int32_t* sp_ntracks, int num_neighbors,
+                              int min_point_best, int min_point_all,
+                              double theta_min_deg, double theta_max_deg,
+                              int64_t* n_best, int64_t* n_all);
+/* best_off[n_shots + 1] + best[n_best], all_off[n_shots + 1] + all[n_all] */
+int pm_covis_download_neighbors(pm_covis* c, int64_t* best_off, int32_t* best,
+                                int64_t* all_off, int32_t* all);
+int pm_covis_last_ms(pm_covis* c, float* ms);
+
 /* ---- Batched estimation ---------------------------------------------- */
 
 /* Many reference views ("jobs") of one pool of views, run together: each

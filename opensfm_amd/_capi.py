@@ -53,6 +53,7 @@ _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
 _u8p = C.POINTER(C.c_uint8)
+_i64p = C.POINTER(C.c_int64)
 SIGNATURES = {
     "pm_last_error": (C.c_char_p, []),
     "pm_device_count": (C.c_int, [_ip]),
@@ -120,6 +121,17 @@ SIGNATURES = {
     "pm_fuser_fuse": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "pm_fuser_download": (C.c_int, [_vp, _fp, _fp, _u8p]),
     "pm_fuser_stats": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _fp, _fp]),
+    "pm_covis_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "pm_covis_destroy": (C.c_int, [_vp]),
+    "pm_covis_build": (C.c_int, [_vp, C.c_int64, _dp, _i64p, _ip, C.c_int, C.c_int, C.c_double,
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int64)]),
+    "pm_covis_download": (C.c_int, [_vp, _ip, _dp, _i64p, _ip, _i64p, _ip, _ip, _ip, _fp]),
+    "pm_covis_select_neighbors": (C.c_int, [_vp, C.c_int, _dp, C.c_int64, _dp, _i64p, _ip, _ip,
+                                            C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                            C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "pm_covis_download_neighbors": (C.c_int, [_vp, _i64p, _ip, _i64p, _ip]),
+    "pm_covis_last_ms": (C.c_int, [_vp, _fp]),
     "pm_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pm_batch_destroy": (C.c_int, [_vp]),
     "pm_batch_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _u8p, _u8p, C.c_int, C.c_int,

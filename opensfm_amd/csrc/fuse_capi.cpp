@@ -15,6 +15,7 @@
 
 #include "../../include/opensfm_amd.h"
 #include "fuse_kernels.h"
+#include "knn_grid.h"
 #include "pm_host.h"
 
 using namespace pmhost;
@@ -103,9 +104,8 @@ struct pm_fuser {
   // work buffers
   DevBuf table, active[2], counts, masks, st, ok, res, res_clr, blk, offs;
   DevBuf points, normals, colors;        // fused points (before / after SOR)
-  DevBuf knn_pts, knn_nrm, knn_clr, mean_knn, codes[2], idx[2], sorted, sort_tmp, lohi,
-      ncells;
-  std::vector<std::unique_ptr<DevBuf>> tables;
+  DevBuf knn_pts, knn_nrm, knn_clr, mean_knn;
+  fu::KnnGrid grid;
   int64_t count = -1, fused_before_sor = 0;
   int64_t rounds = 0;
   float fuse_ms = 0, sor_ms = 0;
@@ -205,72 +205,10 @@ long long fuse_view(pm_fuser* f, int vi, long long total) {
 long long remove_outliers(pm_fuser* f, long long n) {
   const int K = f->sor_knn;
   hipStream_t s = f->stream;
-  f->lohi.ensure(6 * sizeof(float));
-  HIP_CHECK(fu::launch_bbox(f->points.as<float>(), n, f->lohi.as<float>(), s));
-  float lohi[6];
-  HIP_CHECK(hipMemcpyAsync(lohi, f->lohi.p, sizeof(lohi), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  float extent = 0.0f;
-  for (int a = 0; a < 3; ++a) extent = std::max(extent, lohi[3 + a] - lohi[a]);
-  const float quantum = extent > 0.0f && std::isfinite(extent) ? extent / 2097152.0f : 1.0f;
-  for (int b = 0; b < 2; ++b) {
-    f->codes[b].ensure(n * sizeof(unsigned long long));
-    f->idx[b].ensure(n * sizeof(unsigned int));
-  }
-  HIP_CHECK(fu::launch_morton(f->points.as<float>(), n, f->lohi.as<float>(), quantum,
-                              f->codes[0].as<unsigned long long>(),
-                              f->idx[0].as<unsigned int>(), s));
-  size_t tmp_bytes = 0;
-  HIP_CHECK(fu::sort_codes(f->codes[0].as<unsigned long long>(),
-                           f->codes[1].as<unsigned long long>(), f->idx[0].as<unsigned int>(),
-                           f->idx[1].as<unsigned int>(), (int)n, nullptr, &tmp_bytes, s));
-  f->sort_tmp.ensure(tmp_bytes);
-  HIP_CHECK(fu::sort_codes(f->codes[0].as<unsigned long long>(),
-                           f->codes[1].as<unsigned long long>(), f->idx[0].as<unsigned int>(),
-                           f->idx[1].as<unsigned int>(), (int)n, f->sort_tmp.p, &tmp_bytes, s));
-  const unsigned long long* codes = f->codes[1].as<unsigned long long>();
-  const unsigned int* order = f->idx[1].as<unsigned int>();
-  f->sorted.ensure(n * 3 * sizeof(float));
-  HIP_CHECK(fu::launch_gather_sorted(f->points.as<float>(), order, n, f->sorted.as<float>(), s));
-  // cells per level, then the first level whose cells hold ~6 points
-  f->ncells.ensure(22 * sizeof(unsigned int));
-  HIP_CHECK(hipMemsetAsync(f->ncells.p, 0, 22 * sizeof(unsigned int), s));
-  for (int L = 0; L < 22; ++L)
-    HIP_CHECK(fu::launch_level_count(codes, n, L, f->ncells.as<unsigned int>() + L, s));
-  unsigned int ncells[22];
-  HIP_CHECK(hipMemcpyAsync(ncells, f->ncells.p, sizeof(ncells), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  int L0 = 21;
-  for (int L = 0; L < 22; ++L)
-    if ((double)n >= 6.0 * ncells[L]) {
-      L0 = L;
-      break;
-    }
-  fu::Grid g;
-  memset(&g, 0, sizeof(g));
-  for (int a = 0; a < 3; ++a) g.lo[a] = lohi[a];
-  g.quantum = quantum;
-  g.level0 = L0;
-  g.levels = 22 - L0;
-  f->tables.clear();
-  for (int L = L0; L < 22; ++L) {
-    unsigned int size = 1;
-    while (size < 2u * std::max(ncells[L], 1u)) size <<= 1;
-    const size_t bytes = (size_t)size * (sizeof(unsigned long long) + 2 * sizeof(unsigned int));
-    std::unique_ptr<DevBuf> t(new DevBuf());
-    t->ensure(bytes);
-    HIP_CHECK(hipMemsetAsync(t->p, 0, (size_t)size * sizeof(unsigned long long), s));
-    const int li = L - L0;
-    g.keys[li] = t->as<unsigned long long>();
-    g.start[li] = reinterpret_cast<unsigned int*>(g.keys[li] + size);
-    g.cnt[li] = g.start[li] + size;
-    g.mask[li] = size - 1;
-    HIP_CHECK(fu::launch_level_build(codes, n, L, g.keys[li], g.start[li], g.cnt[li], size - 1,
-                                     s));
-    f->tables.push_back(std::move(t));
-  }
+  f->grid.build(f->points.as<float>(), n, s);
+  const unsigned int* order = f->grid.order();
   f->mean_knn.ensure(n * sizeof(float));
-  HIP_CHECK(fu::launch_knn_mean(g, f->sorted.as<float>(), order, n, K,
+  HIP_CHECK(fu::launch_knn_mean(f->grid.g, f->grid.sorted_points(), order, n, K,
                                 f->mean_knn.as<float>(), s));
   std::vector<float> mk(n);
   HIP_CHECK(hipMemcpyAsync(mk.data(), f->mean_knn.p, n * sizeof(float), hipMemcpyDeviceToHost, s));

@@ -73,6 +73,10 @@ hipError_t launch_level_build(const unsigned long long* codes, long long n, int 
                               unsigned int* cnt, unsigned int mask, hipStream_t s);
 hipError_t launch_knn_mean(const Grid& g, const float* sorted, const unsigned int* idx,
                            long long n, int k, float* mean_knn, hipStream_t s);
+// The k nearest others of each point: out_idx / out_d2 (n x k, by original
+// index) ascending by (squared distance, index); -1 where fewer exist.
+hipError_t launch_knn_list(const Grid& g, const float* sorted, const unsigned int* idx,
+                           long long n, int k, int* out_idx, float* out_d2, hipStream_t s);
 // hipCUB radix sort of (code, index) pairs over bits 0..62; with temp null
 // only *temp_bytes is set.
 hipError_t sort_codes(unsigned long long* keys_in, unsigned long long* keys_out,

@@ -621,8 +621,8 @@ __global__ __launch_bounds__(256) void k_knn_mean(Grid g, const float* __restric
       if (c3[a] > 0) gq = fminf(gq, fq[a] - (float)(c3[a] - 1) * step);
       if (c3[a] < top) gq = fminf(gq, (float)(c3[a] + 2) * step - fq[a]);
     }
+    if (gq == INFINITY) break;  // the block holds every point
     if (found == k) {
-      if (gq == INFINITY) break;
       float worst = 0.0f;
 #pragma unroll
       for (int t = 0; t < kMaxKnn; ++t)
@@ -636,6 +636,115 @@ __global__ __launch_bounds__(256) void k_knn_mean(Grid g, const float* __restric
   for (int t = 0; t < kMaxKnn; ++t)
     if (t < found) sum += sqrtf(bd[t]);
   mean_knn[idx[i]] = found > 0 ? __fdiv_rn(sum, (float)found) : 1e30f;
+}
+
+// The k nearest other points of every point as (index, squared distance),
+// ascending by (distance, index): ties at the k-th place go to the lower
+// index.  Fewer than k others leave index -1.
+__device__ __forceinline__ bool knn_before(float d, unsigned int i, float e, unsigned int j) {
+  return d < e || (d == e && i < j);
+}
+
+__global__ __launch_bounds__(256) void k_knn_list(Grid g, const float* __restrict__ sorted,
+                                                  const unsigned int* __restrict__ idx,
+                                                  long long n, int k, int* __restrict__ out_idx,
+                                                  float* __restrict__ out_d2) {
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  const float q[3] = {sorted[3 * i], sorted[3 * i + 1], sorted[3 * i + 2]};
+  float fq[3];
+  unsigned int iq[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    fq[a] = __fdiv_rn(q[a] - g.lo[a], g.quantum);
+    iq[a] = quantise(q[a], g.lo[a], g.quantum);
+  }
+  float bd[kMaxKnn];
+  unsigned int bi[kMaxKnn];
+  int found = 0;
+  for (int L = g.level0; L < g.level0 + g.levels; ++L) {
+    const int li = L - g.level0;
+    const unsigned int top = (kCells >> L) - 1;
+    const unsigned int cx = iq[0] >> L, cy = iq[1] >> L, cz = iq[2] >> L;
+    found = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+      if ((dz < 0 && cz == 0) || (dz > 0 && cz == top)) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        if ((dy < 0 && cy == 0) || (dy > 0 && cy == top)) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          if ((dx < 0 && cx == 0) || (dx > 0 && cx == top)) continue;
+          const unsigned long long key = morton3(cx + dx, cy + dy, cz + dz) + 1;
+          const int s = lookup(g.keys[li], g.mask[li], key);
+          if (s < 0) continue;
+          const unsigned int b = g.start[li][s], e = b + g.cnt[li][s];
+          for (unsigned int j = b; j < e; ++j) {
+            if ((long long)j == i) continue;
+            const float d0 = q[0] - sorted[3 * (size_t)j];
+            const float d1 = q[1] - sorted[3 * (size_t)j + 1];
+            const float d2 = q[2] - sorted[3 * (size_t)j + 2];
+            float acc = 0.0f;
+            acc += d0 * d0;
+            acc += d1 * d1;
+            acc += d2 * d2;
+            const unsigned int oj = idx[j];
+            float wd = 0.0f;
+            unsigned int wi = 0;
+#pragma unroll
+            for (int t = 0; t < kMaxKnn; ++t)
+              if (t == k - 1) {
+                wd = bd[t];
+                wi = bi[t];
+              }
+            if (found == k && !knn_before(acc, oj, wd, wi)) continue;
+            const int last = found < k ? found : k - 1;
+#pragma unroll
+            for (int t = 0; t < kMaxKnn; ++t)
+              if (t == last) {
+                bd[t] = acc;
+                bi[t] = oj;
+              }
+#pragma unroll
+            for (int t = kMaxKnn - 1; t > 0; --t) {
+              if (t <= last && knn_before(bd[t], bi[t], bd[t - 1], bi[t - 1])) {
+                const float x = bd[t - 1];
+                const unsigned int y = bi[t - 1];
+                bd[t - 1] = bd[t];
+                bi[t - 1] = bi[t];
+                bd[t] = x;
+                bi[t] = y;
+              }
+            }
+            if (found < k) ++found;
+          }
+        }
+      }
+    }
+    float gq = INFINITY;
+    const unsigned int c3[3] = {cx, cy, cz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float step = (float)(1u << L);
+      if (c3[a] > 0) gq = fminf(gq, fq[a] - (float)(c3[a] - 1) * step);
+      if (c3[a] < top) gq = fminf(gq, (float)(c3[a] + 2) * step - fq[a]);
+    }
+    if (gq == INFINITY) break;  // the block holds every point
+    if (found == k) {
+      float worst = 0.0f;
+#pragma unroll
+      for (int t = 0; t < kMaxKnn; ++t)
+        if (t == k - 1) worst = bd[t];
+      const float gr = gq * g.quantum * (1.0f - 1e-5f) - g.quantum * 2.0f;
+      if (gr > 0.0f && worst < gr * gr) break;
+    }
+  }
+  const size_t o = (size_t)idx[i] * k;
+#pragma unroll
+  for (int t = 0; t < kMaxKnn; ++t) {
+    if (t < k) {
+      out_idx[o + t] = t < found ? (int)bi[t] : -1;
+      out_d2[o + t] = t < found ? bd[t] : 0.0f;
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_sor_count(const float* __restrict__ mean_knn,
@@ -759,6 +868,13 @@ hipError_t launch_knn_mean(const Grid& g, const float* sorted, const unsigned in
                            long long n, int k, float* mean_knn, hipStream_t s) {
   hipLaunchKernelGGL(k_knn_mean, dim3(blocks(n)), dim3(256), 0, s, g, sorted, idx, n, k,
                      mean_knn);
+  return hipGetLastError();
+}
+
+hipError_t launch_knn_list(const Grid& g, const float* sorted, const unsigned int* idx,
+                           long long n, int k, int* out_idx, float* out_d2, hipStream_t s) {
+  hipLaunchKernelGGL(k_knn_list, dim3(blocks(n)), dim3(256), 0, s, g, sorted, idx, n, k, out_idx,
+                     out_d2);
   return hipGetLastError();
 }
 

@@ -105,6 +105,15 @@ def lib():
     L.orc_fuse.restype = C.c_long
     L.orc_sor.argtypes = [C.c_long, _f32p, _f32p, _u8p, C.c_int, C.c_float, _f32p]
     L.orc_sor.restype = C.c_long
+    _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+    L.orc_covis.argtypes = [C.c_long, _f64p, _i64p, _i32p, C.c_int, C.c_int, C.c_double, _i32p,
+                            _f64p, _i64p, _i32p, _i64p, _i32p, C.c_long, _i32p, _i32p, C.c_long,
+                            C.POINTER(C.c_long), C.POINTER(C.c_float)]
+    L.orc_covis.restype = C.c_long
+    L.orc_select_neighbors.argtypes = [C.c_int, _f64p, C.c_long, _f64p, _i64p, _i32p, _i32p,
+                                       C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       _i64p, _i32p, C.c_long, _i64p, _i32p, C.c_long]
+    L.orc_select_neighbors.restype = C.c_long
     L.orc_fuse_inverse3f.argtypes = [_f32p, _f32p]
     L.orc_fuse_inverse3f.restype = None
     _lib = L
@@ -300,6 +309,70 @@ def inverse3f(m):
     out = np.zeros(9, np.float32)
     lib().orc_fuse_inverse3f(a, out)
     return out.reshape(3, 3)
+
+
+def _csr(lists):
+    off = np.zeros(len(lists) + 1, np.int64)
+    off[1:] = np.cumsum([len(x) for x in lists])
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(x, np.int32) for x in lists] +
+                                              [np.zeros(0, np.int32)]), np.int32)
+    return off, idx
+
+
+def covis(coords, observers, n_shots, fuse_knn=15, fuse_radius_factor=0.5):
+    """BuildCovisibilityGraph (dense_helpers.cc:30-326) on plain arrays:
+    coords (n, 3) float64, observers: per point a list of shot indices.
+    Returns dict(point_sp, sp_coord, sp_members (list of arrays), sp_vis
+    (list of arrays), edges (E, 2), weights (E,), scale)."""
+    coords = np.ascontiguousarray(np.asarray(coords, np.float64).reshape(-1, 3))
+    n = len(coords)
+    off, shots = _csr(observers)
+    point_sp = np.zeros(max(n, 1), np.int32)
+    sp_coord = np.zeros((max(n, 1), 3))
+    moff = np.zeros(n + 2, np.int64)
+    members = np.zeros(max(n, 1), np.int32)
+    voff = np.zeros(n + 2, np.int64)
+    cap_v = max(int(off[-1]), 1)
+    vis = np.zeros(cap_v, np.int32)
+    cap_e = max(n_shots * (n_shots - 1) // 2, 1)
+    edges = np.zeros((cap_e, 2), np.int32)
+    weights = np.zeros(cap_e, np.int32)
+    ne = C.c_long(0)
+    scale = C.c_float(0)
+    nsp = lib().orc_covis(n, coords.reshape(-1), off, shots, int(n_shots), int(fuse_knn),
+                          float(fuse_radius_factor), point_sp, sp_coord.reshape(-1), moff,
+                          members, voff, vis, cap_v, edges.reshape(-1), weights, cap_e,
+                          C.byref(ne), C.byref(scale))
+    assert nsp >= 0
+    return dict(point_sp=point_sp[:n].copy(), sp_coord=sp_coord[:nsp].copy(),
+                sp_members=[members[moff[s]:moff[s + 1]].copy() for s in range(nsp)],
+                sp_vis=[vis[voff[s]:voff[s + 1]].copy() for s in range(nsp)],
+                edges=edges[:ne.value].copy(), weights=weights[:ne.value].copy(),
+                scale=scale.value)
+
+
+def select_neighbors(origins, sp_coord, sp_vis, sp_ntracks, num_neighbors, min_point_best=20,
+                     min_point_all=40, theta_min_deg=3.0, theta_max_deg=60.0):
+    """SelectNeighbors (dense_helpers.cc:331-512) on plain arrays: origins
+    (N, 3) camera centres, super-point centroids, visibility lists and track
+    counts.  Returns (best, all): per shot index a list of shot indices."""
+    origins = np.ascontiguousarray(np.asarray(origins, np.float64).reshape(-1, 3))
+    N = len(origins)
+    sp_coord = np.ascontiguousarray(np.asarray(sp_coord, np.float64).reshape(-1, 3))
+    voff, vis = _csr(sp_vis)
+    nt = np.ascontiguousarray(np.asarray(sp_ntracks, np.int32))
+    boff = np.zeros(N + 1, np.int64)
+    aoff = np.zeros(N + 1, np.int64)
+    cap = max(N * N, 1)
+    best = np.zeros(cap, np.int32)
+    allv = np.zeros(cap, np.int32)
+    r = lib().orc_select_neighbors(N, origins.reshape(-1), len(sp_coord), sp_coord.reshape(-1),
+                                   voff, vis, nt, int(num_neighbors), int(min_point_best),
+                                   int(min_point_all), float(theta_min_deg),
+                                   float(theta_max_deg), boff, best, cap, aoff, allv, cap)
+    assert r >= 0
+    return ([best[boff[i]:boff[i + 1]].tolist() for i in range(N)],
+            [allv[aoff[i]:aoff[i + 1]].tolist() for i in range(N)])
 
 
 class OracleEstimator:

@@ -1278,3 +1278,316 @@ long orc_fuse(int n, const double* Ks, const double* Rs, const double* ts,
   free(mk);
   return kept;
 }
+
+/* ------------------------------------------------------------------------
+ * Neighbour selection (opensfm/src/lib/sfm/src/dense_helpers.cc).
+ *
+ * orc_covis: BuildCovisibilityGraph (dense_helpers.cc:30-326).  Points seen
+ * by >= 2 processable shots (shot indices in [0, n_shots)) are kept in input
+ * order; k = max(1, min(fuse_knn, m - 1)) nearest others by brute force over
+ * float coordinates (vlfeat's float L2, ties to the lower index); scale =
+ * (m/2)-th smallest of the per-point (k/2)-th nearest distances; points i < j
+ * with j among i's k nearest at squared distance <= (float)(r * r), r =
+ * factor * scale, are merged (union-find); super-points are the components,
+ * numbered by their smallest member, with the FP64 centroid summed in member
+ * order, the sorted union of the members' shots, and the members.  Edges are
+ * the shot pairs (lo < hi) sharing a super-point, in (lo, hi) order, weighted
+ * by the count.  The reference's orders of super-points and edges come from
+ * hash-map iteration and are unspecified; these are the canonical ones.
+ * Returns the number of super-points, or -1 when a capacity is too small. */
+
+static int uf_find(int* parent, int x) {
+  while (parent[x] != x) {
+    parent[x] = parent[parent[x]];
+    x = parent[x];
+  }
+  return x;
+}
+
+static int cmp_int(const void* a, const void* b) {
+  const int x = *(const int*)a, y = *(const int*)b;
+  return (x > y) - (x < y);
+}
+
+long orc_covis(long n, const double* coords, const long* obs_off, const int* obs_shot,
+               int n_shots, int fuse_knn, double fuse_radius_factor, int* point_sp,
+               double* sp_coord, long* sp_member_off, int* sp_members, long* sp_vis_off,
+               int* sp_vis, long vis_cap, int* edges, int* weights, long edge_cap,
+               long* n_edges, float* scale_out) {
+  long m = 0;
+  long* keep = (long*)malloc(sizeof(long) * (n > 0 ? n : 1));
+  for (long i = 0; i < n; ++i) {
+    int c = 0;
+    for (long o = obs_off[i]; o < obs_off[i + 1]; ++o)
+      c += obs_shot[o] >= 0 && obs_shot[o] < n_shots;
+    point_sp[i] = -1;
+    if (c >= 2) keep[m++] = i;
+  }
+  *n_edges = 0;
+  *scale_out = 0.0f;
+  if (m == 0) {
+    free(keep);
+    return 0;
+  }
+  int k = fuse_knn < (int)(m - 1) ? fuse_knn : (int)(m - 1);
+  if (k < 1) k = 1;
+  float* fc = (float*)malloc(sizeof(float) * 3 * m);
+  for (long i = 0; i < m; ++i)
+    for (int a = 0; a < 3; ++a) fc[3 * i + a] = (float)coords[3 * keep[i] + a];
+  int* kidx = (int*)malloc(sizeof(int) * m * k);
+  float* kd2 = (float*)malloc(sizeof(float) * m * k);
+  float* kdist = (float*)malloc(sizeof(float) * m);
+  for (long i = 0; i < m; ++i) {
+    int found = 0;
+    int* bi = kidx + i * k;
+    float* bd = kd2 + i * k;
+    for (long j = 0; j < m; ++j) {
+      if (j == i) continue;
+      float acc = 0.0f;
+      for (int a = 0; a < 3; ++a) {
+        const float d = fc[3 * i + a] - fc[3 * j + a];
+        acc += d * d;
+      }
+      /* keep the k smallest by (distance, index) */
+      if (found == k && !(acc < bd[k - 1])) continue;  /* j > every kept index on ties */
+      int s = found < k ? found++ : k - 1;
+      while (s > 0 && bd[s - 1] > acc) {
+        bd[s] = bd[s - 1];
+        bi[s] = bi[s - 1];
+        --s;
+      }
+      bd[s] = acc;
+      bi[s] = (int)j;
+    }
+    for (int t = found; t < k; ++t) {
+      bi[t] = -1;
+      bd[t] = 0.0f;
+    }
+    kdist[i] = found > 0 ? sqrtf(bd[found / 2]) : 0.0f;
+  }
+  float* sorted = (float*)malloc(sizeof(float) * m);
+  memcpy(sorted, kdist, sizeof(float) * m);
+  qsort(sorted, m, sizeof(float), cmp_float);
+  const double scale = (double)sorted[m / 2];
+  const double radius = fuse_radius_factor * scale;
+  const float r2 = (float)(radius * radius);
+  *scale_out = (float)scale;
+  int* parent = (int*)malloc(sizeof(int) * m);
+  for (long i = 0; i < m; ++i) parent[i] = (int)i;
+  for (long i = 0; i < m; ++i)
+    for (int s = 0; s < k; ++s) {
+      const int o = kidx[i * k + s];
+      if (o < 0) break;
+      if (o > i && kd2[i * k + s] <= r2) {
+        int a = uf_find(parent, (int)i), b = uf_find(parent, o);
+        if (a != b) {
+          if (a < b) parent[b] = a;
+          else parent[a] = b;
+        }
+      }
+    }
+  /* components numbered by smallest member */
+  int* sp = (int*)malloc(sizeof(int) * m);
+  long nsp = 0;
+  for (long i = 0; i < m; ++i) {
+    const int r = uf_find(parent, (int)i);
+    if (r == i) sp[i] = (int)nsp++;
+  }
+  for (long i = 0; i < m; ++i) sp[i] = sp[uf_find(parent, (int)i)];
+  for (long i = 0; i < m; ++i) point_sp[keep[i]] = sp[i];
+  /* members in ascending order, centroids summed in that order */
+  for (long s = 0; s <= nsp; ++s) sp_member_off[s] = 0;
+  for (long i = 0; i < m; ++i) sp_member_off[sp[i] + 1]++;
+  for (long s = 0; s < nsp; ++s) sp_member_off[s + 1] += sp_member_off[s];
+  long* fill = (long*)malloc(sizeof(long) * (nsp > 0 ? nsp : 1));
+  for (long s = 0; s < nsp; ++s) fill[s] = sp_member_off[s];
+  for (long i = 0; i < m; ++i) sp_members[fill[sp[i]]++] = (int)keep[i];
+  for (long s = 0; s < nsp; ++s) {
+    double c[3] = {0, 0, 0};
+    for (long t = sp_member_off[s]; t < sp_member_off[s + 1]; ++t)
+      for (int a = 0; a < 3; ++a) c[a] += coords[3 * (long)sp_members[t] + a];
+    const double cnt = (double)(sp_member_off[s + 1] - sp_member_off[s]);
+    for (int a = 0; a < 3; ++a) sp_coord[3 * s + a] = c[a] / cnt;
+  }
+  /* visibility: sorted unique shots of the members */
+  int* buf = (int*)malloc(sizeof(int) * (n_shots > 0 ? n_shots : 1));
+  unsigned char* seen = (unsigned char*)calloc(n_shots > 0 ? n_shots : 1, 1);
+  long nv = 0;
+  int overflow = 0;
+  for (long s = 0; s < nsp; ++s) {
+    sp_vis_off[s] = nv;
+    int nb = 0;
+    for (long t = sp_member_off[s]; t < sp_member_off[s + 1]; ++t) {
+      const long p = sp_members[t];
+      for (long o = obs_off[p]; o < obs_off[p + 1]; ++o) {
+        const int sh = obs_shot[o];
+        if (sh < 0 || sh >= n_shots || seen[sh]) continue;
+        seen[sh] = 1;
+        buf[nb++] = sh;
+      }
+    }
+    qsort(buf, nb, sizeof(int), cmp_int);
+    for (int b = 0; b < nb; ++b) {
+      seen[buf[b]] = 0;
+      if (nv < vis_cap) sp_vis[nv] = buf[b];
+      else overflow = 1;
+      ++nv;
+    }
+  }
+  sp_vis_off[nsp] = nv;
+  /* edges */
+  if (!overflow) {
+    int* cnt = (int*)calloc((size_t)n_shots * n_shots, sizeof(int));
+    for (long s = 0; s < nsp; ++s)
+      for (long a = sp_vis_off[s]; a < sp_vis_off[s + 1]; ++a)
+        for (long b = a + 1; b < sp_vis_off[s + 1]; ++b)
+          cnt[(size_t)sp_vis[a] * n_shots + sp_vis[b]]++;
+    long ne = 0;
+    for (long a = 0; a < n_shots; ++a)
+      for (long b = a + 1; b < n_shots; ++b) {
+        const int c = cnt[(size_t)a * n_shots + b];
+        if (c < 1) continue;
+        if (ne < edge_cap) {
+          edges[2 * ne] = (int)a;
+          edges[2 * ne + 1] = (int)b;
+          weights[ne] = c;
+        } else {
+          overflow = 1;
+        }
+        ++ne;
+      }
+    *n_edges = ne;
+    free(cnt);
+  }
+  free(buf);
+  free(seen);
+  free(fill);
+  free(sp);
+  free(parent);
+  free(sorted);
+  free(kdist);
+  free(kd2);
+  free(kidx);
+  free(fc);
+  free(keep);
+  return overflow ? -1 : nsp;
+}
+
+/* orc_select_neighbors: SelectNeighbors (dense_helpers.cc:331-512).  Per
+ * super-point and shot pair (vis entries in [0, n_shots) with camera-to-
+ * centroid norm >= 1e-12): cos of the angle at the centroid (FP64, clamped),
+ * count += n_tracks, score_best += n_tracks when cos(theta_max) < cos <
+ * cos(theta_min), and sum_cos += cos * n_tracks in fixed point (cos rounded
+ * to a multiple of 2^-32, so the sum does not depend on the order of
+ * super-points; the reference sums FP64 in a thread-dependent order).  Per
+ * shot: "best" = itself then up to num_neighbors others with count >
+ * min_point_best, mean angle within [theta_min, theta_max] (compared as
+ * cosines) and score_best > min_point_best, by score descending (ties: lower
+ * index first); "all" = best then, in index order, the other shots with
+ * count > min_point_all.  Returns the total of the all lists, or -1 when a
+ * capacity is too small. */
+long orc_select_neighbors(int n_shots, const double* origins, long n_sp, const double* sp_coord,
+                          const long* sp_vis_off, const int* sp_vis, const int* sp_ntracks,
+                          int num_neighbors, int min_point_best, int min_point_all,
+                          double theta_min_deg, double theta_max_deg, long* best_off,
+                          int* best, long best_cap, long* all_off, int* all, long all_cap) {
+  const double pi = 3.14159265358979323846;
+  const double cos_lo = cos(theta_max_deg * pi / 180.0);
+  const double cos_hi = cos(theta_min_deg * pi / 180.0);
+  const size_t NN = (size_t)n_shots * n_shots;
+  long* count = (long*)calloc(NN > 0 ? NN : 1, sizeof(long));
+  long* score = (long*)calloc(NN > 0 ? NN : 1, sizeof(long));
+  long long* sumfx = (long long*)calloc(NN > 0 ? NN : 1, sizeof(long long));
+  int* cams = (int*)malloc(sizeof(int) * (n_shots > 0 ? n_shots : 1));
+  double* cv = (double*)malloc(sizeof(double) * 4 * (n_shots > 0 ? n_shots : 1));
+  for (long s = 0; s < n_sp; ++s) {
+    int nc = 0;
+    for (long t = sp_vis_off[s]; t < sp_vis_off[s + 1]; ++t)
+      if (sp_vis[t] >= 0 && sp_vis[t] < n_shots) cams[nc++] = sp_vis[t];
+    if (nc < 2) continue;
+    const long nt = sp_ntracks[s];
+    for (int c = 0; c < nc; ++c) {
+      double* v = cv + 4 * c;
+      for (int a = 0; a < 3; ++a) v[a] = origins[3 * cams[c] + a] - sp_coord[3 * s + a];
+      v[3] = sqrt(v[0] * v[0] + (v[1] * v[1] + v[2] * v[2]));
+    }
+    for (int ci = 0; ci < nc; ++ci) {
+      if (cv[4 * ci + 3] < 1e-12) continue;
+      for (int cj = ci + 1; cj < nc; ++cj) {
+        if (cv[4 * cj + 3] < 1e-12) continue;
+        const double* u = cv + 4 * ci;
+        const double* w = cv + 4 * cj;
+        double c = (u[0] * w[0] + (u[1] * w[1] + u[2] * w[2])) / (u[3] * w[3]);
+        c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+        const int good = c > cos_lo && c < cos_hi;
+        const long long fx = llrint(c * 4294967296.0) * nt;
+        const int a = cams[ci], b = cams[cj];
+        const size_t ab = (size_t)a * n_shots + b, ba = (size_t)b * n_shots + a;
+        count[ab] += nt;
+        count[ba] += nt;
+        score[ab] += good ? nt : 0;
+        score[ba] += good ? nt : 0;
+        sumfx[ab] += fx;
+        sumfx[ba] += fx;
+      }
+    }
+  }
+  long nb_total = 0, na_total = 0;
+  int overflow = 0;
+  int* cand = (int*)malloc(sizeof(int) * (n_shots > 0 ? n_shots : 1));
+  unsigned char* in_best = (unsigned char*)calloc(n_shots > 0 ? n_shots : 1, 1);
+  for (int i = 0; i < n_shots; ++i) {
+    int nb = 0;
+    for (int j = 0; j < n_shots; ++j) {
+      const size_t ij = (size_t)i * n_shots + j;
+      if (count[ij] == 0 || count[ij] <= min_point_best) continue;
+      double avg = (double)sumfx[ij] / 4294967296.0 / (double)count[ij];
+      avg = avg < -1.0 ? -1.0 : (avg > 1.0 ? 1.0 : avg);
+      if (avg > cos_hi || avg < cos_lo) continue;   /* mean angle outside [min, max] */
+      if (score[ij] > min_point_best) cand[nb++] = j;
+    }
+    /* score descending, index ascending (insertion sort: stable) */
+    for (int a = 1; a < nb; ++a) {
+      const int x = cand[a];
+      int b = a;
+      while (b > 0 && score[(size_t)i * n_shots + cand[b - 1]] < score[(size_t)i * n_shots + x]) {
+        cand[b] = cand[b - 1];
+        --b;
+      }
+      cand[b] = x;
+    }
+    const int take = nb < num_neighbors ? nb : (num_neighbors > 0 ? num_neighbors : 0);
+    best_off[i] = nb_total;
+    all_off[i] = na_total;
+    const int row[1] = {i};
+    for (int t = 0; t < 1 + take; ++t) {
+      const int v = t == 0 ? row[0] : cand[t - 1];
+      in_best[v] = 1;
+      if (nb_total < best_cap) best[nb_total] = v;
+      else overflow = 1;
+      if (na_total < all_cap) all[na_total] = v;
+      else overflow = 1;
+      ++nb_total;
+      ++na_total;
+    }
+    for (int j = 0; j < n_shots; ++j) {
+      const size_t ij = (size_t)i * n_shots + j;
+      if (count[ij] > min_point_all && !in_best[j]) {
+        if (na_total < all_cap) all[na_total] = j;
+        else overflow = 1;
+        ++na_total;
+      }
+    }
+    for (int t = 0; t < 1 + take; ++t) in_best[t == 0 ? i : cand[t - 1]] = 0;
+  }
+  best_off[n_shots] = nb_total;
+  all_off[n_shots] = na_total;
+  free(cand);
+  free(in_best);
+  free(cams);
+  free(cv);
+  free(count);
+  free(score);
+  free(sumfx);
+  return overflow ? -1 : na_total;
+}
