@@ -279,6 +279,52 @@ def fusion_cpu_baseline(views, primary=2):
             "prune": {"kept": kept, "ms": round(prune_s * 1e3, 1), "cores": 1, "kind": "port"}}
 
 
+def neighbors_bench(nx=25, ny=20, n_points=300000):
+    """Neighbour selection before the depthmaps on a synthetic sparse
+    reconstruction (nx x ny cameras, n_points landmarks on a wavy ground,
+    each seen by the cameras within 25 m): covisibility graph (k 15, radius
+    factor 0.5) and 10 neighbours per shot, on the GPU: device time between
+    the first upload and the last kernel, and wall time including the
+    Python objects of the result."""
+    from opensfm_amd import neighbors, scenes
+    origins, coords, obs = scenes.sfm_landmarks(nx, ny, n_points)
+    ids = list(range(len(origins)))
+    off = np.zeros(n_points + 1, np.int64)
+    off[1:] = np.cumsum([len(o) for o in obs])
+    obs = (off, np.concatenate([np.asarray(o, np.int32) for o in obs]))   # CSR form
+    neighbors.build_covisibility_graph(coords[:2000], (off[:2001], obs[1][:off[2000]]), ids)
+    t0 = time.perf_counter()
+    g = neighbors.build_covisibility_graph(coords, obs, ids)
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = neighbors.select_neighbors(origins, g.super_points, ids, 10)
+    select_s = time.perf_counter() - t0
+    nobs = int(off[-1])
+    return {"workload": f"{len(ids)} shots, {n_points} landmarks, {nobs} observations",
+            "super_points": len(g.super_points), "edges": len(g.edges),
+            "build_device_ms": round(g.device_ms, 2), "select_device_ms": round(r.device_ms, 2),
+            "build_wall_ms": round(build_s * 1e3, 1), "select_wall_ms": round(select_s * 1e3, 1),
+            "mean_best": round(float(np.mean([len(v) for v in r.best_neighbors.values()])), 2),
+            "mpts_s": round(n_points / (g.device_ms / 1e3) / 1e6, 3)}
+
+
+def neighbors_cpu_baseline(n_points=20000, nx=25, ny=20):
+    """CPU baseline of the neighbour section: the oracle's graph (brute-force
+    k nearest) on a bounded sample of the same reconstruction, one thread."""
+    import oracle
+    from opensfm_amd import scenes
+    origins, coords, obs = scenes.sfm_landmarks(nx, ny, n_points)
+    t0 = time.perf_counter()
+    o = oracle.covis(coords, obs, len(origins))
+    s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.select_neighbors(origins, o["sp_coord"], o["sp_vis"],
+                            [len(m) for m in o["sp_members"]], 10)
+    return {"sample_points": n_points, "build_s": round(s, 2),
+            "select_s": round(time.perf_counter() - t0, 2),
+            "mpts_s": round(n_points / s / 1e6, 4), "cores": 1, "kind": "port"}
+
+
 def learned_bench(width=2000, height=1504, views=4, seed=5000):
     """Learned initialiser vs plain PatchMatch on nadir views of fractal
     terrain (a 4 x 4 camera grid, 10 neighbours each; the first `views`
@@ -314,6 +360,8 @@ def main():
                     help="also time the learned initialiser on N nadir views (0: skip)")
     ap.add_argument("--no-fusion", action="store_true",
                     help="skip the fusion / pruning section")
+    ap.add_argument("--no-neighbors", action="store_true",
+                    help="skip the neighbour-selection section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (torch) so that "
                          "profiler byte counters can be scaled against a known size")
@@ -474,6 +522,11 @@ def main():
             out["fusion"]["fuse_gpu_over_cpu"] = round(
                 out["fusion"]["gpu_sor0"]["mpix_s"] / out["fusion"]["cpu_baseline"]["fuse"]["mpix_s"], 1)
         log(f"fusion: {out['fusion']}")
+    if not args.no_neighbors:
+        out["neighbors"] = neighbors_bench()
+        if not args.no_cpu_baseline:
+            out["neighbors"]["cpu_baseline"] = neighbors_cpu_baseline()
+        log(f"neighbors: {out['neighbors']}")
     if args.learned_views > 0:
         out["learned_init"] = learned_bench(views=args.learned_views)
         log(f"learned initialiser: {out['learned_init']}")

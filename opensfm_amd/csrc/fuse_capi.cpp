@@ -166,9 +166,10 @@ long long fuse_view(pm_fuser* f, int vi, long long total) {
                                  f->st.as<uint8_t>(), f->ok.as<uint8_t>(), f->stream));
   int cur = 0;
   int host_count = 0;
+  int batch = 2;  // rounds between reads of the live count; grows while rounds continue
   while (live > 0) {
     const int grid = (int)std::min<long long>((live + 255) / 256, 4096);
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < batch; ++j) {
       ++f->round;
       ++f->rounds;
       HIP_CHECK(fu::launch_fuse_round(
@@ -182,6 +183,7 @@ long long fuse_view(pm_fuser* f, int vi, long long total) {
                              hipMemcpyDeviceToHost, f->stream));
     HIP_CHECK(hipStreamSynchronize(f->stream));
     live = host_count;
+    batch = std::min(batch * 2, 16);
   }
   const uint8_t* ok = f->ok.as<uint8_t>();
   const long long added = block_offsets(f, npix, [&](int* counts) {

@@ -42,19 +42,26 @@ class CovisibilityGraph:
     shot_order: list           # index -> shot id
     scale: float = 0.0         # median k-nearest distance of the landmarks
     point_sp: np.ndarray = None  # per landmark: its super-point, -1 if unused
+    device_ms: float = 0.0     # device time of the construction
 
 
 @dataclasses.dataclass
 class NeighborResult:
     best_neighbors: Dict[object, list]
     all_neighbors: Dict[object, list]
+    device_ms: float = 0.0
 
 
 def _csr(lists, index):
+    """(offsets, processable indices) of per-item id lists; a pair of arrays
+    (offsets, indices) is taken as already in that form."""
+    if isinstance(lists, tuple) and len(lists) == 2 and isinstance(lists[0], np.ndarray):
+        return (np.ascontiguousarray(lists[0], np.int64),
+                np.ascontiguousarray(lists[1], np.int32))
     off = np.zeros(len(lists) + 1, np.int64)
     off[1:] = np.cumsum([len(x) for x in lists])
-    flat = [index.get(s, -1) for x in lists for s in x]
-    return off, np.ascontiguousarray(np.asarray(flat, np.int32).reshape(-1))
+    flat = np.fromiter((index.get(s, -1) for x in lists for s in x), np.int32, int(off[-1]))
+    return off, flat
 
 
 class _Handle:
@@ -77,16 +84,17 @@ def build_covisibility_graph(coords, observers: Sequence[Sequence], processable:
     """BuildCovisibilityGraph (dense_helpers.cc:30-326).
 
     coords: (n, 3) landmark positions; observers: per landmark the ids of the
-    shots observing it (ids outside ``processable`` are ignored); track_ids:
+    shots observing it (ids outside ``processable`` are ignored), or a pair
+    of arrays (offsets (n + 1,), processable indices) in CSR form; track_ids:
     per landmark its track id (default: its index)."""
     processable = list(processable)
     index = {sid: i for i, sid in enumerate(processable)}
     coords = np.ascontiguousarray(np.asarray(coords, np.float64).reshape(-1, 3))
     n = len(coords)
-    if len(observers) != n:
-        raise ValueError("one observer list per landmark is needed")
-    tids = list(range(n)) if track_ids is None else list(track_ids)
     off, shots = _csr(observers, index)
+    if len(off) != n + 1:
+        raise ValueError("one observer list per landmark is needed")
+    tids = range(n) if track_ids is None else list(track_ids)
     h = _Handle(device)
     nsp, nv, ne = C.c_int64(0), C.c_int64(0), C.c_int64(0)
     _capi.check(h.lib.pm_covis_build(
@@ -108,6 +116,8 @@ def build_covisibility_graph(coords, observers: Sequence[Sequence], processable:
         _capi.ptr(moff, C.c_int64), _capi.ptr(members, C.c_int32), _capi.ptr(voff, C.c_int64),
         _capi.ptr(vis, C.c_int32), _capi.ptr(edges, C.c_int32), _capi.ptr(weights, C.c_int32),
         C.byref(scale)))
+    ms = C.c_float(0)
+    _capi.check(h.lib.pm_covis_last_ms(h.h, C.byref(ms)))
     sps = [SuperPoint(coord=sp_coord[s].copy(),
                       vis=[processable[v] for v in vis[voff[s]:voff[s + 1]]],
                       tracks=[tids[m] for m in members[moff[s]:moff[s + 1]]])
@@ -115,7 +125,7 @@ def build_covisibility_graph(coords, observers: Sequence[Sequence], processable:
     return CovisibilityGraph(super_points=sps, edges=[tuple(e) for e in edges[:ne].tolist()],
                              weights=[float(w) for w in weights[:ne]],
                              shot_order=processable, scale=scale.value,
-                             point_sp=point_sp[:n].copy())
+                             point_sp=point_sp[:n].copy(), device_ms=ms.value)
 
 
 def select_neighbors(origins, super_points: Sequence[SuperPoint], processable: Sequence,
@@ -153,7 +163,10 @@ def select_neighbors(origins, super_points: Sequence[SuperPoint], processable: S
     _capi.check(h.lib.pm_covis_download_neighbors(
         h.h, _capi.ptr(boff, C.c_int64), _capi.ptr(best, C.c_int32), _capi.ptr(aoff, C.c_int64),
         _capi.ptr(allv, C.c_int32)))
+    ms = C.c_float(0)
+    _capi.check(h.lib.pm_covis_last_ms(h.h, C.byref(ms)))
     return NeighborResult(
+        device_ms=ms.value,
         best_neighbors={processable[i]: [processable[v] for v in best[boff[i]:boff[i + 1]]]
                         for i in range(N)},
         all_neighbors={processable[i]: [processable[v] for v in allv[aoff[i]:aoff[i + 1]]]

@@ -582,3 +582,39 @@ def nadir_scene(rows, cols, width, height, seed=5000, neighbours=10, altitude=40
         views = list(ex.map(one, range(n)))
     return BatchScene(views=views, jobs=jobs, min_depth=min_depth, max_depth=max_depth,
                       name=name)
+
+
+def sfm_landmarks(nx, ny, n_points, seed=6000, spacing=10.0, altitude=30.0, radius=25.0,
+                  dup=0.05):
+    """A synthetic sparse reconstruction for neighbour selection: nx x ny
+    camera centres on a grid at `altitude`, n_points landmarks on a wavy
+    ground (a fraction `dup` repeated exactly, as re-triangulated tracks),
+    each seen by the cameras within `radius` of it horizontally (at least
+    one).  Returns (origins (N, 3), coords (n, 3), observers: per landmark
+    the camera indices)."""
+    rng = np.random.default_rng(seed)
+    gx, gy = np.meshgrid(np.arange(nx) * spacing, np.arange(ny) * spacing)
+    origins = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, altitude)], 1)
+    origins += rng.normal(0, 0.5, origins.shape)
+    xy = rng.uniform([-spacing, -spacing], [nx * spacing, ny * spacing], (n_points, 2))
+    z = 2.0 * np.sin(xy[:, 0] / 17.0) * np.cos(xy[:, 1] / 23.0)
+    coords = np.concatenate([xy, z[:, None]], 1)
+    nd = int(dup * n_points)
+    if nd:
+        coords[-nd:] = coords[:nd]
+    cx = np.clip(np.round(xy[:, 0] / spacing).astype(int), 0, nx - 1)
+    cy = np.clip(np.round(xy[:, 1] / spacing).astype(int), 0, ny - 1)
+    reach = int(np.ceil(radius / spacing))
+    obs = [[] for _ in range(n_points)]
+    for dy in range(-reach, reach + 1):
+        for dx in range(-reach, reach + 1):
+            ox, oy = cx + dx, cy + dy
+            ok = (ox >= 0) & (ox < nx) & (oy >= 0) & (oy < ny)
+            cam = np.where(ok, oy * nx + ox, 0)
+            d = np.hypot(origins[cam, 0] - coords[:, 0], origins[cam, 1] - coords[:, 1])
+            for i in np.flatnonzero(ok & (d < radius)):
+                obs[i].append(int(cam[i]))
+    for i in range(n_points):
+        if not obs[i]:
+            obs[i] = [int(cy[i] * nx + cx[i])]
+    return origins, coords, obs
