@@ -382,6 +382,19 @@ int pm_covis_download_neighbors(pm_covis* c, int64_t* best_off, int32_t* best,
                                 int64_t* all_off, int32_t* all);
 int pm_covis_last_ms(pm_covis* c, float* ms);
 
+/* ---- LZ4 block codec of the depthmap files (dataset.py:1080-1149) ----- */
+
+/* Host-side helpers with no counterpart in the reference, which calls the
+ * lz4 Python module: the block format of the LZ4 specification.  Decoding
+ * appends to out[out_pos ..] (matches may reach back into earlier output,
+ * as linked blocks do) and returns the new end in *new_pos; corrupt input
+ * or too small an output gives PM_ERR_INVALID.  Compression is a greedy
+ * single-probe hash matcher (independent block). */
+int pm_lz4_block_decompress(const uint8_t* src, int64_t src_size, uint8_t* out,
+                            int64_t out_pos, int64_t out_cap, int64_t* new_pos);
+int pm_lz4_block_compress(const uint8_t* src, int64_t n, uint8_t* dst,
+                          int64_t cap, int64_t* written);
+
 /* ---- Batched estimation ---------------------------------------------- */
 
 /* Many reference views ("jobs") of one pool of views, run together: each
@@ -447,6 +460,13 @@ int pm_batch_download(pm_batch* b, int job, float* depth, float* plane,
 int pm_batch_clean(pm_batch* b, float same_depth_threshold,
                    int min_consistent_views);
 int pm_batch_download_clean(pm_batch* b, int job, float* depth);
+/* Job `job`'s result in the layout of the cluster estimator's DepthmapResult
+ * (estimator.h:33-38, depthmap_bind.h:53-73): depth H*W, unit camera-frame
+ * normal H*W*3 (plane / |plane|, 0 without depth), cost H*W (1 - NCC score;
+ * 2 without depth), confidence H*W (1 / (1 + exp((cost - 0.5) * 5)) where
+ * depth > 0, else 0; estimator.cc:1022-1040). */
+int pm_batch_download_cluster(pm_batch* b, int job, float* depth, float* normal,
+                              float* cost, float* confidence);
 /* Last run: scorings, launches = red/black launches, total_ms = dominant_ms
  * = the whole run on the device. */
 int pm_batch_stats(pm_batch* b, pm_stats* out);

@@ -6,8 +6,9 @@ by hand-written HIP kernels for gfx950 behind the C ABI in
 include/opensfm_amd.h.
 """
 from .batch import DepthmapBatch  # noqa: F401
+from .cluster import DepthmapClusterEstimator  # noqa: F401
 from .pydense import (  # noqa: F401
     DepthmapCleaner, DepthmapEstimator, DepthmapFuser, DepthmapPruner, GPUDepthmapCleaner)
 
 __all__ = ["DepthmapEstimator", "DepthmapCleaner", "GPUDepthmapCleaner", "DepthmapPruner",
-           "DepthmapFuser", "DepthmapBatch"]
+           "DepthmapFuser", "DepthmapBatch", "DepthmapClusterEstimator"]

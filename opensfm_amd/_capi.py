@@ -132,6 +132,10 @@ SIGNATURES = {
                                             C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "pm_covis_download_neighbors": (C.c_int, [_vp, _i64p, _ip, _i64p, _ip]),
     "pm_covis_last_ms": (C.c_int, [_vp, _fp]),
+    "pm_lz4_block_decompress": (C.c_int, [_u8p, C.c_int64, _u8p, C.c_int64, C.c_int64,
+                                          C.POINTER(C.c_int64)]),
+    "pm_lz4_block_compress": (C.c_int, [_u8p, C.c_int64, _u8p, C.c_int64,
+                                        C.POINTER(C.c_int64)]),
     "pm_batch_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pm_batch_destroy": (C.c_int, [_vp]),
     "pm_batch_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _u8p, _u8p, C.c_int, C.c_int,
@@ -152,6 +156,7 @@ SIGNATURES = {
     "pm_batch_download": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _ip]),
     "pm_batch_clean": (C.c_int, [_vp, C.c_float, C.c_int]),
     "pm_batch_download_clean": (C.c_int, [_vp, C.c_int, _fp]),
+    "pm_batch_download_cluster": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _fp]),
     "pm_batch_stats": (C.c_int, [_vp, C.POINTER(pm_stats)]),
     "pm_batch_clean_ms": (C.c_int, [_vp, _fp]),
     "pm_nn_conv_kpad": (C.c_int, [C.c_int]),

@@ -132,6 +132,19 @@ class DepthmapBatch:
             _capi.ptr(score, C.c_float), _capi.ptr(nghbr, C.c_int32)))
         return [depth, plane, score, nghbr]
 
+    def cluster_result(self, job: int) -> list:
+        """[depth, normal, cost, confidence] of a job in the cluster
+        estimator's layout (DepthmapResult, estimator.h:33-38)."""
+        h, w = self.job_shape(job)
+        depth = np.zeros((h, w), np.float32)
+        normal = np.zeros((h, w, 3), np.float32)
+        cost = np.zeros((h, w), np.float32)
+        conf = np.zeros((h, w), np.float32)
+        _capi.check(self._lib.pm_batch_download_cluster(
+            self._h, int(job), _capi.ptr(depth, C.c_float), _capi.ptr(normal, C.c_float),
+            _capi.ptr(cost, C.c_float), _capi.ptr(conf, C.c_float)))
+        return [depth, normal, cost, conf]
+
     def clean(self, same_depth_threshold: float = 0.01, min_consistent_views: int = 2) -> None:
         """DepthmapCleaner::Clean of every job against the jobs of its
         neighbour views (defaults of depthmap.cc:504-505)."""

@@ -1,0 +1,187 @@
+"""``pydense.DepthmapClusterEstimator``-compatible surface over the engine.
+
+The live pipeline (opensfm/dense.py:657-731) builds a cluster of reference
+views, each with its source views, runs it, and saves per reference view
+``[depth, normal, cost, confidence]`` (depthmap_bind.h:19-114,
+dense/cluster.h:42-78).  This class keeps that call sequence and result
+layout and computes every reference view with the engine's batched
+PatchMatch (one ``DepthmapBatch`` job per reference view, all jobs in one
+set of launches): depth, the unit camera-frame normal of the winning
+plane, cost = 1 - NCC score, and the reference's logistic confidence
+(estimator.cc:1022-1040).
+
+The reference's estimator is a different PatchMatch (multi-scale pyramid,
+top-k view costs with census, SfM planar prior, cross-view geometric
+consistency).  Settings with a counterpart here are applied
+(``max_iterations``, ``patch_size``); the others are accepted and kept
+(``max_image_size`` -- the caller already scales images --,
+``hierarchy_levels``, ``sigma_spatial``, ``sigma_color``, ``top_k``,
+``census_weight``, ``smooth_weight``, the geometric-consistency weight and
+links, the SfM points) without changing the result.
+"""
+from __future__ import annotations
+
+from typing import List
+
+import numpy as np
+
+from . import _capi
+from .batch import DepthmapBatch
+
+
+class DepthmapClusterEstimator:
+    """Cluster-of-reference-views estimator (pydense.pyi DepthmapClusterEstimator)."""
+
+    def __init__(self):
+        self._device = 0
+        self._params = dict(max_iterations=3, patch_size=11, max_image_size=3200,
+                            hierarchy_levels=1, sigma_spatial=5.0, sigma_color=3.0 / 255.0,
+                            top_k=4, census_weight=0.3, smooth_weight=0.0,
+                            geom_consistency_weight=0.0)
+        self.clear()
+
+    # ---- parameters (depthmap_bind.h:21-33) ----
+    def set_max_iterations(self, n: int) -> None:
+        self._params["max_iterations"] = int(n)
+
+    def set_patch_size(self, size: int) -> None:
+        self._params["patch_size"] = int(size)
+
+    def set_max_image_size(self, size: int) -> None:
+        self._params["max_image_size"] = int(size)
+
+    def set_hierarchy_levels(self, levels: int) -> None:
+        self._params["hierarchy_levels"] = int(levels)
+
+    def set_sigma_spatial(self, v: float) -> None:
+        self._params["sigma_spatial"] = float(v)
+
+    def set_sigma_color(self, v: float) -> None:
+        self._params["sigma_color"] = float(v)
+
+    def set_top_k(self, k: int) -> None:
+        self._params["top_k"] = int(k)
+
+    def set_census_weight(self, w: float) -> None:
+        self._params["census_weight"] = float(w)
+
+    def set_smooth_weight(self, w: float) -> None:
+        self._params["smooth_weight"] = float(w)
+
+    def set_geom_consistency_weight(self, w: float) -> None:
+        self._params["geom_consistency_weight"] = float(w)
+
+    def set_device(self, idx: int) -> None:
+        self._device = int(idx)
+
+    # ---- cluster contents (depthmap_bind.h:36-53) ----
+    def begin_ref_view(self, K, R, t, image, depth_min: float, depth_max: float) -> int:
+        self._refs.append(dict(view=self._view(K, R, t, image), sources=[],
+                               depth_range=(float(depth_min), float(depth_max)),
+                               sfm_points=None))
+        return len(self._refs) - 1
+
+    def add_source_view(self, K, R, t, image) -> None:
+        if not self._refs:
+            raise RuntimeError("add_source_view before begin_ref_view")
+        self._refs[-1]["sources"].append(self._view(K, R, t, image))
+
+    def set_sfm_points(self, points) -> None:
+        if not self._refs:
+            raise RuntimeError("set_sfm_points before begin_ref_view")
+        self._refs[-1]["sfm_points"] = np.asarray(points, np.float64).reshape(-1, 3)
+
+    def add_geom_link(self, ref_idx: int, source_view_idx: int, from_ref_idx: int) -> None:
+        self._links.append((int(ref_idx), int(source_view_idx), int(from_ref_idx)))
+
+    def run(self) -> List[list]:
+        """Per reference view, in begin_ref_view order: [depth (H, W),
+        normal (H, W, 3), cost (H, W), confidence (H, W)], float32."""
+        if not self._refs:
+            return []
+        b = DepthmapBatch(self._device)
+        pool = {}
+
+        def view_index(v):
+            # the same shot arrives once per reference view that uses it:
+            # one pool view per distinct (camera, image)
+            key = (v["image"].shape, v["K"].tobytes(), v["R"].tobytes(), v["t"].tobytes(),
+                   hash(v["image"].tobytes()))
+            if key not in pool:
+                pool[key] = b.add_view(v["K"], v["R"], v["t"], v["image"])
+            return pool[key]
+
+        for ref in self._refs:
+            r = view_index(ref["view"])
+            nb = [view_index(s) for s in ref["sources"]]
+            lo, hi = ref["depth_range"]
+            b.add_job(r, nb, lo, hi)
+        b.set_patchmatch_iterations(max(1, self._params["max_iterations"]))
+        b.set_patch_size(self._params["patch_size"])
+        b.run()
+        self.last_stats = b.stats()
+        return [b.cluster_result(k) for k in range(len(self._refs))]
+
+    def clear(self) -> None:
+        self._refs = []
+        self._links = []
+        self.last_stats = None
+
+    def num_ref_views(self) -> int:
+        return len(self._refs)
+
+    @staticmethod
+    def _view(K, R, t, image):
+        image = np.ascontiguousarray(np.asarray(image), np.uint8)
+        if image.ndim != 2:
+            raise ValueError("image must be a 2-D uint8 array")
+        return dict(K=np.asarray(K, np.float64), R=np.asarray(R, np.float64),
+                    t=np.asarray(t, np.float64).reshape(3), image=image)
+
+    # ---- devices (depthmap_bind.h:77-107) ----
+    @staticmethod
+    def is_available() -> bool:
+        try:
+            return _capi.device_count() > 0
+        except RuntimeError:
+            return False
+
+    @staticmethod
+    def num_devices() -> int:
+        return _capi.device_count()
+
+    @staticmethod
+    def _props(idx: int):
+        import torch
+        return torch.cuda.get_device_properties(int(idx))
+
+    @staticmethod
+    def device_name(idx: int) -> str:
+        return DepthmapClusterEstimator._props(idx).name
+
+    @staticmethod
+    def device_memory_bytes(idx: int) -> int:
+        return int(DepthmapClusterEstimator._props(idx).total_memory)
+
+    @staticmethod
+    def device_is_gpu(idx: int) -> bool:
+        return True
+
+    @staticmethod
+    def device_available_memory(idx: int) -> int:
+        import torch
+        free, _ = torch.cuda.mem_get_info(int(idx))
+        return int(free)
+
+    @staticmethod
+    def gpu_memory_bytes() -> int:
+        return DepthmapClusterEstimator.device_memory_bytes(0)
+
+    @staticmethod
+    def reserve_device_memory(idx: int, nbytes: int) -> None:
+        """Bookkeeping only in the reference's OpenCL context; device memory
+        here is allocated on use."""
+
+    @staticmethod
+    def release_device_memory(idx: int, nbytes: int) -> None:
+        """See reserve_device_memory."""

@@ -60,6 +60,7 @@ struct pm_batch {
   // device state of the prepared configuration
   DevBuf viewtab, jobtab, wlut, state, score, nghbr;
   DevBuf out_depth, out_plane, out_score, out_nghbr, queue, counter;
+  DevBuf cluster_out;  // staging of pm_batch_download_cluster
   DevBuf cl_views, cl_jobs, cl_out;
   int wlut_patch = -1;
   int prepared_jobs = -1;  // jobs in the uploaded table (-1: stale)
@@ -605,6 +606,40 @@ int pm_batch_download(pm_batch* b, int job, float* depth, float* plane,
                              hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipMemcpyAsync(nghbr, b->out_nghbr.as<int>() + J.off, np * sizeof(int32_t),
                              hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    return PM_OK;
+  });
+}
+
+// The cluster estimator's result of one job (DepthmapClusterEstimator::Run,
+// depthmap_bind.h:53-73): depth, unit normal, cost, confidence.
+int pm_batch_download_cluster(pm_batch* b, int job, float* depth, float* normal, float* cost,
+                              float* confidence) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (!b->have_result) return fail(PM_ERR_INVALID, "no computed maps to download");
+  if (job < 0 || job >= (int)b->jobs.size())
+    return fail(PM_ERR_RANGE, "job " + std::to_string(job) + " out of range");
+  if (!depth || !normal || !cost || !confidence) return fail(PM_ERR_INVALID, "null output buffer");
+  return guarded([&] {
+    DeviceGuard dg(b->device);
+    const BatchJob& J = b->jobs[job];
+    const PoolView& R = *b->views[J.ref];
+    const size_t np = (size_t)R.width * R.height;
+    b->cluster_out.ensure(np * 5 * sizeof(float));
+    float* nrm = b->cluster_out.as<float>();
+    float* cst = nrm + 3 * np;
+    float* cnf = cst + np;
+    HIP_CHECK(pm::launch_cluster_out(b->out_depth.as<float>() + J.off,
+                                     b->out_plane.as<float>() + 3 * J.off,
+                                     b->out_score.as<float>() + J.off, (int)np, nrm, cst, cnf,
+                                     b->stream));
+    HIP_CHECK(hipMemcpyAsync(depth, b->out_depth.as<float>() + J.off, np * sizeof(float),
+                             hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipMemcpyAsync(normal, nrm, 3 * np * sizeof(float), hipMemcpyDeviceToHost,
+                             b->stream));
+    HIP_CHECK(hipMemcpyAsync(cost, cst, np * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipMemcpyAsync(confidence, cnf, np * sizeof(float), hipMemcpyDeviceToHost,
+                             b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
     return PM_OK;
   });

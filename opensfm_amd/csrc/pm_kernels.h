@@ -54,6 +54,9 @@ hipError_t launch_pm_half_batch(const PmJob* jobs, int njobs, int total,
                                 unsigned long long* count, hipStream_t s);
 hipError_t launch_pack_batch(const PmJob* jobs, int njobs, int total,
                              int median, hipStream_t s);
+// Cluster-estimator outputs (unit normal, cost, confidence) of n pixels.
+hipError_t launch_cluster_out(const float* depth, const float* plane, const float* score, int n,
+                              float* normal, float* cost, float* conf, hipStream_t s);
 hipError_t launch_clean_batch(const ClJob* jobs, int njobs, int total,
                               const ClView* views, float thr, int min_views,
                               hipStream_t s);

@@ -1617,6 +1617,39 @@ hipError_t launch_pack_batch(const PmJob* jobs, int njobs, int total,
   return hipGetLastError();
 }
 
+// The cluster estimator's result layout (DepthmapResult, estimator.h:33-38)
+// from a PatchMatch result: unit normal = plane / |plane| (camera frame,
+// facing the camera), cost = 1 - score (the NCC cost, opencl_kernels.h:460;
+// 2 where there is no depth), confidence = 1 / (1 + exp((cost - 0.5) * 5))
+// where depth > 0, else 0 (estimator.cc:1022-1040).
+__global__ __launch_bounds__(256) void k_cluster_out(const float* __restrict__ depth,
+                                                     const float* __restrict__ plane,
+                                                     const float* __restrict__ score, int n,
+                                                     float* __restrict__ normal,
+                                                     float* __restrict__ cost,
+                                                     float* __restrict__ conf) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float d = depth[i];
+  const float p0 = plane[3 * (size_t)i], p1 = plane[3 * (size_t)i + 1], p2 = plane[3 * (size_t)i + 2];
+  const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
+  const bool ok = d > 0.0f && nrm > 0.0f;
+  normal[3 * (size_t)i] = ok ? p0 / nrm : 0.0f;
+  normal[3 * (size_t)i + 1] = ok ? p1 / nrm : 0.0f;
+  normal[3 * (size_t)i + 2] = ok ? p2 / nrm : 0.0f;
+  const float c = d > 0.0f ? 1.0f - score[i] : 2.0f;
+  cost[i] = c;
+  conf[i] = d > 0.0f ? 1.0f / (1.0f + expf((c - 0.5f) * 5.0f)) : 0.0f;
+}
+
+hipError_t launch_cluster_out(const float* depth, const float* plane, const float* score, int n,
+                              float* normal, float* cost, float* conf, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cluster_out, dim3((n + 255) / 256), dim3(256), 0, s, depth, plane, score,
+                     n, normal, cost, conf);
+  return hipGetLastError();
+}
+
 hipError_t launch_clean_batch(const ClJob* jobs, int njobs, int total,
                               const ClView* views, float thr, int min_views,
                               hipStream_t s) {

@@ -1,0 +1,64 @@
+"""GPU tests of the DepthmapClusterEstimator surface (depthmap_bind.h:19-114,
+the call sequence of opensfm/dense.py:657-731): results in begin_ref_view
+order with the reference's layout; depth equal to the batched engine's for
+the same jobs; normal, cost and confidence as their definitions
+(plane / |plane|, 1 - score, the logistic of estimator.cc:1022-1040);
+accurate depths on a rendered scene."""
+import numpy as np
+import pytest
+
+from opensfm_amd import DepthmapBatch, DepthmapClusterEstimator, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cluster_surface(cfg1_scene):
+    print("DepthmapClusterEstimator over the batched engine", flush=True)
+    sc = cfg1_scene
+    v = sc.views
+    refs = [(0, [1, 2, 3, 4]), (2, [0, 1, 3]), (4, [3, 2, 0])]
+    ce = DepthmapClusterEstimator()
+    ce.set_max_iterations(4)
+    ce.set_patch_size(7)
+    ce.set_top_k(3)                        # kept, no effect
+    for r, nb in refs:
+        idx = ce.begin_ref_view(v[r].K, v[r].R, v[r].t, v[r].image.copy(), sc.min_depth,
+                                sc.max_depth)
+        ce.set_sfm_points(np.zeros((5, 3)))
+        for s in nb:                       # fresh copies, as dense.py loads them
+            ce.add_source_view(v[s].K, v[s].R, v[s].t, v[s].image.copy())
+    ce.add_geom_link(0, 1, 1)
+    res = ce.run()
+    assert len(res) == 3
+    b = DepthmapBatch()
+    ids = [b.add_view(x.K, x.R, x.t, x.image) for x in v]
+    for r, nb in refs:
+        b.add_job(ids[r], [ids[s] for s in nb], sc.min_depth, sc.max_depth)
+    b.set_patchmatch_iterations(4)
+    b.set_patch_size(7)
+    b.run()
+    for k, (r, _) in enumerate(refs):
+        depth, normal, cost, conf = res[k]
+        h, w = v[r].image.shape
+        assert depth.shape == (h, w) and normal.shape == (h, w, 3)
+        assert cost.shape == (h, w) and conf.shape == (h, w)
+        assert all(a.dtype == np.float32 for a in res[k])
+        d, plane, score, _ = b.result(k)
+        assert np.array_equal(depth, d)
+        ok = d > 0
+        nl = np.linalg.norm(plane, axis=-1, keepdims=True)
+        n = plane / np.where(nl > 0, nl, 1)
+        np.testing.assert_allclose(normal[ok], n[ok], rtol=1e-6, atol=1e-6)
+        assert (normal[~ok] == 0).all()
+        assert np.array_equal(cost[ok], (np.float32(1) - score)[ok])
+        assert (cost[~ok] == 2).all()
+        want = np.float32(1) / (np.float32(1) + np.exp((cost - np.float32(0.5)) * np.float32(5)))
+        np.testing.assert_allclose(conf[ok], want[ok], rtol=3e-7, atol=1e-7)
+        assert (conf[~ok] == 0).all()
+        assert (normal[ok][:, 2] < 0).mean() > 0.99           # facing the camera
+        acc = scenes.depth_accuracy(depth, v[r].depth, sc.min_depth, sc.max_depth)
+        print(k, round(acc, 4), flush=True)
+        assert acc > 0.8
+    ce.clear()
+    assert ce.run() == [] and ce.num_ref_views() == 0
+    assert DepthmapClusterEstimator.is_available() and DepthmapClusterEstimator.num_devices() >= 1
