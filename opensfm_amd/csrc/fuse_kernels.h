@@ -66,8 +66,9 @@ hipError_t launch_morton(const float* points, long long n, const float* lo, floa
                          unsigned long long* codes, unsigned int* idx, hipStream_t s);
 hipError_t launch_gather_sorted(const float* points, const unsigned int* idx, long long n,
                                 float* sorted, hipStream_t s);
-hipError_t launch_level_count(const unsigned long long* codes, long long n, int level,
-                              unsigned int* n_cells, hipStream_t s);
+// The cell counts of all 22 levels in one pass (n_cells[22], zeroed).
+hipError_t launch_level_count_all(const unsigned long long* codes, long long n,
+                                  unsigned int* n_cells, hipStream_t s);
 hipError_t launch_level_build(const unsigned long long* codes, long long n, int level,
                               unsigned long long* keys, unsigned int* start,
                               unsigned int* cnt, unsigned int mask, hipStream_t s);

@@ -251,36 +251,115 @@ __global__ __launch_bounds__(256) void k_fuse_claim(
   }
 }
 
-__global__ __launch_bounds__(256) void k_fuse_decide(
-    const View* __restrict__ views, Params prm, const int* __restrict__ active,
-    const int* __restrict__ count, unsigned int round,
-    const unsigned long long* __restrict__ masks, uint8_t* __restrict__ st,
-    float* __restrict__ res, uint8_t* __restrict__ res_clr, uint8_t* __restrict__ ok) {
-  const int n = *count;
-  const View& A = views[prm.ref];
-  for (int idx = blockIdx.x * 256 + threadIdx.x; idx < n; idx += gridDim.x * 256) {
-    const int p = active[idx];
-    if (st[p]) continue;
-    const unsigned long long key = claim_key(round, p);
-    if (A.claim[p] != key) continue;
-    const int r = p / A.width, c = p - (p / A.width) * A.width;
-    float P[3], nw[3];
-    own(A, p, c, r, P, nw);
-    const unsigned long long m = masks[idx];
-    bool mine = true;
-    for (unsigned long long b = m; b && mine; b &= b - 1) {
-      const int s = __ffsll((long long)b) - 1;
-      const View& S = views[prm.nb[s]];
-      float proj[3], su, sv;
-      int sc, sr;
-      const long long sp = land(S, P, prm.bm, proj, &su, &sv, &sc, &sr);
-      mine = S.claim[sp] == key;
+// (n/2)-th smallest of the first n of N register-held values, ties by
+// position: entries from n on are padded with `pad` (above every value)
+// and an odd-even transposition sort -- stable, as it swaps only strictly
+// greater neighbours -- orders the array with static indices, so it stays
+// in registers.
+template <int N, typename T>
+__device__ __forceinline__ T median_n(const T (&in)[N], int n, T pad) {
+  T v[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) v[t] = t < n ? in[t] : pad;
+#pragma unroll
+  for (int pass = 0; pass < N; ++pass) {
+#pragma unroll
+    for (int t = pass & 1; t + 1 < N; t += 2) {
+      const T a = v[t], b = v[t + 1];
+      const bool sw = a > b;
+      v[t] = sw ? b : a;
+      v[t + 1] = sw ? a : b;
     }
-    if (!mine) continue;
-    // decided: the reference's accumulation (fuser.cc:195-318) and medians
-    // (fuser.cc:320-357)
-    float ob[6][kMaxNeighbours + 1];
-    uint8_t oc[3][kMaxNeighbours + 1];
+  }
+  const int k = n / 2;
+  T out = v[0];
+#pragma unroll
+  for (int t = 0; t < N; ++t)
+    if (t == k) out = v[t];
+  return out;
+}
+
+// A decided pixel: gathers its observations (itself, then the passing
+// neighbours in slot order; fuser.cc:195-318), marks the neighbour pixels
+// fused and writes the medians (fuser.cc:320-357).
+//
+// Up to kSmall observations: three passes (point, normal, colour), each
+// recomputing the observations -- the computation is deterministic -- and
+// keeping only that pass's three components in registers (static indices),
+// so the kernel needs no private memory.  More observations: one pass into
+// private memory (a separate kernel, launched only for views with that many
+// neighbours).
+constexpr int kSmall = 12;
+
+template <int kPass>
+__device__ __forceinline__ void fuse_pass(const View* __restrict__ views, const Params& prm,
+                                          const View& A, int p, int c, int r, const float* P,
+                                          const float* nw, unsigned long long m, float* out) {
+  float v[3][kSmall];
+  uint32_t col[kSmall];
+  if constexpr (kPass == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k][0] = P[k];
+  } else if constexpr (kPass == 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k][0] = nw[k];
+  } else {
+    col[0] = A.color[3 * (size_t)p] | (A.color[3 * (size_t)p + 1] << 8) |
+             (A.color[3 * (size_t)p + 2] << 16);
+  }
+  int no = 1;
+  for (unsigned long long b = m; b; b &= b - 1) {
+    const int s = __ffsll((long long)b) - 1;
+    const View& S = views[prm.nb[s]];
+    float proj[3], su, sv, SP[3], snw[3];
+    int sc, sr;
+    const long long sp = land(S, P, prm.bm, proj, &su, &sv, &sc, &sr);
+    if constexpr (kPass == 2) {
+      const uint32_t cc =
+          S.color[3 * sp] | (S.color[3 * sp + 1] << 8) | (S.color[3 * sp + 2] << 16);
+#pragma unroll
+      for (int t = 1; t < kSmall; ++t)
+        if (t == no) col[t] = cc;
+      S.fused[sp] = 1;
+    } else {
+      consistent(A, S, prm, c, r, nw, proj, su, sv, sc, sr, sp, SP, snw);
+#pragma unroll
+      for (int t = 1; t < kSmall; ++t)
+        if (t == no) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) v[k][t] = kPass == 0 ? SP[k] : snw[k];
+        }
+    }
+    ++no;
+  }
+  if constexpr (kPass == 2) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      uint32_t ch[kSmall];
+#pragma unroll
+      for (int t = 0; t < kSmall; ++t) ch[t] = (col[t] >> (8 * k)) & 255u;
+      out[k] = (float)median_n<kSmall>(ch, no, 256u);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      out[k] = median_n<kSmall>(v[k], no, INFINITY);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+template <bool kLarge>
+__device__ __forceinline__ void fuse_emit(const View* __restrict__ views, const Params& prm,
+                                          const View& A, int p, int c, int r, const float* P,
+                                          const float* nw, unsigned long long m,
+                                          float* __restrict__ res,
+                                          uint8_t* __restrict__ res_clr) {
+  float mp[3], mn[3], mc[3];
+  if constexpr (kLarge) {
+    constexpr int N = kMaxNeighbours + 1;
+    float ob[6][N];
+    uint8_t oc[3][N];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       ob[k][0] = P[k];
@@ -295,7 +374,6 @@ __global__ __launch_bounds__(256) void k_fuse_decide(
       int sc, sr;
       const long long sp = land(S, P, prm.bm, proj, &su, &sv, &sc, &sr);
       consistent(A, S, prm, c, r, nw, proj, su, sv, sc, sr, sp, SP, snw);
-#pragma unroll
       for (int k = 0; k < 3; ++k) {
         ob[k][no] = SP[k];
         ob[3 + k][no] = snw[k];
@@ -304,23 +382,62 @@ __global__ __launch_bounds__(256) void k_fuse_decide(
       ++no;
       S.fused[sp] = 1;
     }
-    float mp[3], mn[3];
-#pragma unroll
     for (int k = 0; k < 3; ++k) {
       mp[k] = median(ob[k], no);
       mn[k] = median(ob[3 + k], no);
+      mc[k] = (float)median(oc[k], no);
     }
-    const float mlen = norm3(mn);
-    if (mlen > 1e-6f) {
+  } else {
+    fuse_pass<0>(views, prm, A, p, c, r, P, nw, m, mp);
+    fuse_pass<1>(views, prm, A, p, c, r, P, nw, m, mn);
+    fuse_pass<2>(views, prm, A, p, c, r, P, nw, m, mc);
+  }
+  const float mlen = norm3(mn);
+  if (mlen > 1e-6f) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) mn[k] = __fdiv_rn(mn[k], mlen);
-    }
+    for (int k = 0; k < 3; ++k) mn[k] = __fdiv_rn(mn[k], mlen);
+  }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      res[6 * (size_t)p + k] = mp[k];
-      res[6 * (size_t)p + 3 + k] = mn[k];
-      res_clr[3 * (size_t)p + k] = median(oc[k], no);
+  for (int k = 0; k < 3; ++k) {
+    res[6 * (size_t)p + k] = mp[k];
+    res[6 * (size_t)p + 3 + k] = mn[k];
+    res_clr[3 * (size_t)p + k] = (uint8_t)mc[k];
+  }
+}
+
+// Decides the pixels that hold all their claims and have at most kSmall
+// observations (kLarge: the ones with more), writes their points and marks
+// their pixels fused.
+
+template <bool kLarge>
+__global__ __launch_bounds__(256) void k_fuse_decide(
+    const View* __restrict__ views, Params prm, const int* __restrict__ active,
+    const int* __restrict__ count, unsigned int round,
+    const unsigned long long* __restrict__ masks, uint8_t* __restrict__ st,
+    float* __restrict__ res, uint8_t* __restrict__ res_clr, uint8_t* __restrict__ ok) {
+  const int n = *count;
+  const View& A = views[prm.ref];
+  for (int idx = blockIdx.x * 256 + threadIdx.x; idx < n; idx += gridDim.x * 256) {
+    const int p = active[idx];
+    if (st[p]) continue;
+    const unsigned long long m = masks[idx];
+    if ((1 + __popcll(m) > kSmall) != kLarge) continue;
+    const unsigned long long key = claim_key(round, p);
+    if (A.claim[p] != key) continue;
+    const int r = p / A.width, c = p - (p / A.width) * A.width;
+    float P[3], nw[3];
+    own(A, p, c, r, P, nw);
+    bool mine = true;
+    for (unsigned long long b = m; b && mine; b &= b - 1) {
+      const int s = __ffsll((long long)b) - 1;
+      const View& S = views[prm.nb[s]];
+      float proj[3], su, sv;
+      int sc, sr;
+      const long long sp = land(S, P, prm.bm, proj, &su, &sv, &sc, &sr);
+      mine = S.claim[sp] == key;
     }
+    if (!mine) continue;
+    fuse_emit<kLarge>(views, prm, A, p, c, r, P, nw, m, res, res_clr);
     ok[p] = 1;
     st[p] = 1;
     A.fused[p] = 1;
@@ -492,13 +609,19 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const float* __restrict__
   for (int k = 0; k < 3; ++k) out[3 * i + k] = pts[3 * (size_t)j + k];
 }
 
-__global__ __launch_bounds__(256) void k_level_count(const unsigned long long* __restrict__ codes,
-                                                     long long n, int level,
-                                                     unsigned int* __restrict__ n_cells) {
+// Cells of every level at once: one pass over the sorted codes.
+__global__ __launch_bounds__(256) void k_level_count_all(
+    const unsigned long long* __restrict__ codes, long long n, unsigned int* __restrict__ n_cells) {
   const long long i = blockIdx.x * 256ll + threadIdx.x;
-  const bool start = i < n && (i == 0 || (codes[i] >> (3 * level)) != (codes[i - 1] >> (3 * level)));
-  const unsigned long long b = __ballot(start);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(n_cells, (unsigned int)__popcll(b));
+  const unsigned long long cur = i < n ? codes[i] : 0ull;
+  const unsigned long long prev = (i > 0 && i < n) ? codes[i - 1] : 0ull;
+  const unsigned long long diff = cur ^ prev;
+#pragma unroll
+  for (int L = 0; L < 22; ++L) {
+    const bool start = i < n && (i == 0 || (diff >> (3 * L)) != 0ull);
+    const unsigned long long b = __ballot(start);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&n_cells[L], (unsigned int)__popcll(b));
+  }
 }
 
 __global__ __launch_bounds__(256) void k_level_insert(const unsigned long long* __restrict__ codes,
@@ -806,8 +929,11 @@ hipError_t launch_fuse_round(const View* views, const Params& prm, const int* ac
                              hipStream_t s) {
   hipLaunchKernelGGL(k_fuse_claim, dim3(grid), dim3(256), 0, s, views, prm, active, count,
                      next_count, round, masks, st);
-  hipLaunchKernelGGL(k_fuse_decide, dim3(grid), dim3(256), 0, s, views, prm, active, count,
-                     round, masks, st, res, res_clr, ok);
+  hipLaunchKernelGGL(k_fuse_decide<false>, dim3(grid), dim3(256), 0, s, views, prm, active,
+                     count, round, masks, st, res, res_clr, ok);
+  if (prm.n_nb + 1 > kSmall)
+    hipLaunchKernelGGL(k_fuse_decide<true>, dim3(grid), dim3(256), 0, s, views, prm, active,
+                       count, round, masks, st, res, res_clr, ok);
   hipLaunchKernelGGL(k_fuse_compact, dim3(grid), dim3(256), 0, s, active, count, st,
                      next_active, next_count);
   return hipGetLastError();
@@ -848,9 +974,9 @@ hipError_t launch_gather_sorted(const float* points, const unsigned int* idx, lo
   return hipGetLastError();
 }
 
-hipError_t launch_level_count(const unsigned long long* codes, long long n, int level,
-                              unsigned int* n_cells, hipStream_t s) {
-  hipLaunchKernelGGL(k_level_count, dim3(blocks(n)), dim3(256), 0, s, codes, n, level, n_cells);
+hipError_t launch_level_count_all(const unsigned long long* codes, long long n,
+                                  unsigned int* n_cells, hipStream_t s) {
+  hipLaunchKernelGGL(k_level_count_all, dim3(blocks(n)), dim3(256), 0, s, codes, n, n_cells);
   return hipGetLastError();
 }
 

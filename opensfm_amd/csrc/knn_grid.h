@@ -58,8 +58,7 @@ struct KnnGrid {
     HIP_CHECK(launch_gather_sorted(points, order(), n, sorted.as<float>(), s));
     ncells.ensure(22 * sizeof(unsigned int));
     HIP_CHECK(hipMemsetAsync(ncells.p, 0, 22 * sizeof(unsigned int), s));
-    for (int L = 0; L < 22; ++L)
-      HIP_CHECK(launch_level_count(sc, n, L, ncells.as<unsigned int>() + L, s));
+    HIP_CHECK(launch_level_count_all(sc, n, ncells.as<unsigned int>(), s));
     unsigned int nc[22];
     HIP_CHECK(hipMemcpyAsync(nc, ncells.p, sizeof(nc), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
