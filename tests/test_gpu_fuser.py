@@ -95,6 +95,18 @@ def test_fuser_masks_sources_sizes_and_ids():
            max_depth_error=0.02)
 
 
+def test_fuser_many_observations():
+    print("DepthmapFuser vs oracle: 18 views, more than 12 observations per pixel", flush=True)
+    s = scenes.make_scene(96, 72, 17, 3, seed=61, min_depth=4.0, max_depth=20.0,
+                          spacing_deg=1.0)
+    views = _views(s, np.random.default_rng(5), noise=0.0005, holes=0.0)
+    for v, sv in zip(views, s.views):
+        v[4] = scenes.normals_from_depth(v[0], sv.depth)
+    pts, st = _check(views, sor_knn=0, min_num_consistent=13, border_margin=2)
+    assert len(pts) > 100
+    pts, st = _check(views, sor_knn=0, min_num_consistent=2, border_margin=2)
+
+
 def test_fuser_larger_views():
     print("DepthmapFuser vs oracle: 8 views 480x360", flush=True)
     s = scenes.make_scene(480, 360, 7, 6, seed=51, min_depth=2.0, max_depth=20.0,
