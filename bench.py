@@ -440,6 +440,23 @@ def main():
              "achieved_tflops": round(sweep_flop / (sst["dominant_ms"] / 1e3) / 1e12, 3)}
     log(f"plane sweep 64 planes: {sweep}")
 
+    # The reference's iteration is a forward and a backward sweep (each pixel
+    # updated twice, depthmap.cc:207-210); a red/black iteration updates each
+    # pixel once.  The same number of pixel updates: 2 x iterations.
+    est.set_patchmatch_iterations(2 * w["iterations"])
+    est.run(PM_MODE_PATCHMATCH)
+    t3 = time.perf_counter()
+    est.run(PM_MODE_PATCHMATCH)
+    upd_s = time.perf_counter() - t3
+    upd_depth = est.download()[0]
+    est.set_patchmatch_iterations(w["iterations"])
+    matched_updates = {
+        "iterations": 2 * w["iterations"],
+        "mpix_s": round(w["width"] * w["height"] / upd_s / 1e6, 3),
+        "accuracy_within_1pct": round(scenes.depth_accuracy(upd_depth, scene.ref.depth,
+                                                            w["min_depth"], w["max_depth"]), 4)}
+    log(f"matched pixel updates: {matched_updates}")
+
     npx = w["width"] * w["height"]
     ms_per_step = elapsed / args.steps * 1e3
     value = npx * args.steps / elapsed / 1e6
@@ -468,6 +485,7 @@ def main():
                    "width": w["width"], "height": w["height"], "neighbours": w["neighbours"],
                    "patch": w["patch"], "iterations": w["iterations"]},
         "accuracy_within_1pct": round(acc, 4),
+        "matched_pixel_updates": matched_updates,
         "mean_score": round(mean_score, 4),
         "pcie_inclusive_mpix_s": round(npx / pcie_s / 1e6, 3),
         "plane_sweep": sweep,
