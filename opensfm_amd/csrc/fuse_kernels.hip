@@ -612,6 +612,9 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const float* __restrict__
 // Cells of every level at once: one pass over the sorted codes.
 __global__ __launch_bounds__(256) void k_level_count_all(
     const unsigned long long* __restrict__ codes, long long n, unsigned int* __restrict__ n_cells) {
+  __shared__ unsigned int blk[22];
+  if (threadIdx.x < 22) blk[threadIdx.x] = 0;
+  __syncthreads();
   const long long i = blockIdx.x * 256ll + threadIdx.x;
   const unsigned long long cur = i < n ? codes[i] : 0ull;
   const unsigned long long prev = (i > 0 && i < n) ? codes[i - 1] : 0ull;
@@ -620,8 +623,10 @@ __global__ __launch_bounds__(256) void k_level_count_all(
   for (int L = 0; L < 22; ++L) {
     const bool start = i < n && (i == 0 || (diff >> (3 * L)) != 0ull);
     const unsigned long long b = __ballot(start);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&n_cells[L], (unsigned int)__popcll(b));
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&blk[L], (unsigned int)__popcll(b));
   }
+  __syncthreads();
+  if (threadIdx.x < 22 && blk[threadIdx.x]) atomicAdd(&n_cells[threadIdx.x], blk[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(256) void k_level_insert(const unsigned long long* __restrict__ codes,
