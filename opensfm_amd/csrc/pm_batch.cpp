@@ -28,6 +28,7 @@ struct PoolView {
   double K[9], R[9], t[3], Kinv[9];
   int width = 0, height = 0;
   DevBuf img, mask, quad;
+  DevBuf quadc;        // FP16 bilinear coefficients of the quads
   DevBuf quad16;       // 16-bit quads, built on first use
   int quad16_kind = 0;  // precision they were built for (0: none)
 };
@@ -153,6 +154,7 @@ void prepare(pm_batch* b, int count) {
     fill_view(vt.back(), R.quad.as<uint32_t>(), R.width, R.height, R.K, Q, a,
               R.Kinv);
     vt.back().quad16 = R.quad16.as<uint32_t>();
+    vt.back().quadc = R.quadc.as<uint32_t>();
     for (int v : J.nb) {
       const PoolView& V = *b->views[v];
       relative_pose(R.R, R.t, V.R, V.t, Q, a);
@@ -160,6 +162,7 @@ void prepare(pm_batch* b, int count) {
       fill_view(vt.back(), V.quad.as<uint32_t>(), V.width, V.height, V.K, Q,
                 a, R.Kinv);
       vt.back().quad16 = V.quad16.as<uint32_t>();
+      vt.back().quadc = V.quadc.as<uint32_t>();
     }
   }
   b->total_px = px;
@@ -416,8 +419,9 @@ int pm_batch_add_view(pm_batch* b, const double* K, const double* R,
     HIP_CHECK(hipMemcpy2DAsync(v->mask.p, width, mask, stride, width, height,
                                hipMemcpyHostToDevice, b->stream));
     v->quad.ensure((size_t)(width + 1) * (height + 1) * sizeof(uint32_t));
+    v->quadc.ensure((size_t)(width + 1) * (height + 1) * 8);
     HIP_CHECK(pm::launch_build_quad(v->img.as<uint8_t>(), v->quad.as<uint32_t>(),
-                                    width, height, b->stream));
+                                    v->quadc.as<uint32_t>(), width, height, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
     if (view_id) *view_id = (int)b->views.size();
     b->views.push_back(std::move(v));

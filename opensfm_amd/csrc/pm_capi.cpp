@@ -31,6 +31,7 @@ struct HostView {
   double K[9], R[9], t[3], Kinv[9], Q[9], a[3];
   int width, height;
   DevBuf img, quad;
+  DevBuf quadc;         // FP16 bilinear coefficients of the quads
   DevBuf quad16;        // 16-bit quads, built on first use
   int quad16_kind = 0;  // precision they were built for (0: none)
 };
@@ -149,6 +150,7 @@ pm::PmArgs prepare(pm_estimator* e) {
     fill_view(tab[v], V.quad.as<uint32_t>(), V.width, V.height, V.K, V.Q, V.a,
               r.Kinv);
     tab[v].quad16 = V.quad16.as<uint32_t>();
+    tab[v].quadc = V.quadc.as<uint32_t>();
   }
   e->viewtab.ensure(sizeof(pm::PmView) * n);
   HIP_CHECK(hipMemcpyAsync(e->viewtab.p, tab.data(), sizeof(pm::PmView) * n,
@@ -556,8 +558,9 @@ int pm_estimator_add_view(pm_estimator* e, const double* K, const double* R,
     HIP_CHECK(hipMemcpy2DAsync(v->img.p, width, image, stride, width, height,
                                hipMemcpyHostToDevice, e->stream));
     v->quad.ensure((size_t)(width + 1) * (height + 1) * sizeof(uint32_t));
+    v->quadc.ensure((size_t)(width + 1) * (height + 1) * 8);
     HIP_CHECK(pm::launch_build_quad(v->img.as<uint8_t>(), v->quad.as<uint32_t>(),
-                                    width, height, e->stream));
+                                    v->quadc.as<uint32_t>(), width, height, e->stream));
     if (e->views.empty()) {
       e->mask0.ensure(np);
       HIP_CHECK(hipMemcpy2DAsync(e->mask0.p, width, mask, stride, width, height,

@@ -245,23 +245,31 @@ __device__ __forceinline__ uint2 fetch_quad16(const QuadTex& t, float flx,
 
 // Texel sources for the fast scorer: the texel of floor coordinates, and
 // its bilinear blend at fractions (fx, fy).
-//
-// FP32: u8 quad, blended in FP32 with FMAs.
-struct GlobalFetch {
-  QuadTex tex;
-  __device__ __forceinline__ uint32_t operator()(float flx, float fly) const {
-    return fetch_quad(tex, flx, fly);
-  }
-  static __device__ __forceinline__ float bilerp(uint32_t t, float fx, float fy) {
-    const float a = ub0(t), b = ub1(t), c = ub2(t), d = ub3(t);
-    const float im0 = __builtin_fmaf(fx, b - a, a);
-    const float im1 = __builtin_fmaf(fx, d - c, c);
-    return __builtin_fmaf(fy, im1 - im0, im0);
-  }
-};
 
 typedef _Float16 pm_half2 __attribute__((ext_vector_type(2)));
 typedef __bf16 pm_bf162 __attribute__((ext_vector_type(2)));
+
+// FP32 from bilinear coefficients: the quad (a, b; c, d) is stored as
+// A = a, B = b - a, C = c - a, D = d - c - b + a in FP16 (integers of at most
+// 510 in magnitude, exact), so that the blend
+//   A + fx B + fy (C + fx D)
+// is two mixed-precision FMAs (v_fma_mix_f32 reads the FP16 halves
+// directly) and one FP32 FMA.  A + fx B is the top row's blend with the
+// same single rounding as fma(fx, b - a, a); C + fx D is the difference of
+// the two rows' blends with one rounding instead of three.
+struct CoefFetch {
+  QuadTex tex;
+  __device__ __forceinline__ uint2 operator()(float flx, float fly) const {
+    return fetch_quad16(tex, flx, fly);
+  }
+  static __device__ __forceinline__ float bilerp(uint2 t, float fx, float fy) {
+    const pm_half2 ab = __builtin_bit_cast(pm_half2, t.x);
+    const pm_half2 cd = __builtin_bit_cast(pm_half2, t.y);
+    const float top = __builtin_fmaf(fx, (float)ab[1], (float)ab[0]);
+    const float dif = __builtin_fmaf(fx, (float)cd[1], (float)cd[0]);
+    return __builtin_fmaf(fy, dif, top);
+  }
+};
 
 // FP16 / BF16: the quad holds (a, b) and (c, d) as 16-bit pairs; each row
 // is blended by one dot product with the weights (1 - fx, fx) rounded to
@@ -393,7 +401,7 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
 // ---------------------------------------------------------------------------
 // Fast scorer for PatchMatch: the same local-affine bilateral NCC, with FMAs
 // and one-ulp reciprocals.  Takes the pixel's warp and a texel source
-// (GlobalFetch or Global16Fetch: the texel of floor coordinates and its
+// (CoefFetch or Global16Fetch: the texel of floor coordinates and its
 // bilinear blend); the NCC sums are FP32 for every texel format.
 // A non-finite coordinate yields a NaN score,
 // which never wins a strict comparison.

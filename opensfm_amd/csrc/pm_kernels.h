@@ -7,8 +7,10 @@
 
 namespace pm {
 
-hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
-                             hipStream_t s);
+// u8 quads (exact scorer) and their FP16 bilinear coefficients (8 bytes
+// per texel; fast FP32 scorer) of a u8 image.
+hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, uint32_t* quadc,
+                             int W, int H, hipStream_t s);
 // 8-byte quads of FP16 (precision 1) or BF16 (precision 2) values.
 hipError_t launch_build_quad16(const uint8_t* img, uint32_t* quad16, int W,
                                int H, int precision, hipStream_t s);

@@ -56,20 +56,34 @@ static inline dim3 tile_grid(int tiles_x, int tiles_y) {
 // ---------------------------------------------------------------------------
 // Zero-bordered quad texture from a u8 image (see pm_device.h).
 
+// With it, the same quads as FP16 bilinear coefficients (pm_device.h,
+// CoefFetch).
 __global__ void k_build_quad(const uint8_t* __restrict__ img,
-                             uint32_t* __restrict__ quad, int W, int H) {
+                             uint32_t* __restrict__ quad,
+                             uint2* __restrict__ quadc, int W, int H) {
   const int tx = blockIdx.x * blockDim.x + threadIdx.x;
   const int ty = blockIdx.y * blockDim.y + threadIdx.y;
   if (tx > W || ty > H) return;
   const int ix = tx - 1, iy = ty - 1;
   uint32_t v = 0;
+  int a = 0, b = 0, c = 0, d = 0;
   if (ix >= 0 && iy >= 0 && ix <= W - 2 && iy <= H - 2) {
     const uint8_t* r0 = img + (size_t)iy * W + ix;
     const uint8_t* r1 = r0 + W;
-    v = (uint32_t)r0[0] | ((uint32_t)r0[1] << 8) | ((uint32_t)r1[0] << 16) |
-        ((uint32_t)r1[1] << 24);
+    a = r0[0];
+    b = r0[1];
+    c = r1[0];
+    d = r1[1];
+    v = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16) | ((uint32_t)d << 24);
   }
-  quad[(size_t)ty * (W + 1) + tx] = v;
+  const size_t o = (size_t)ty * (W + 1) + tx;
+  quad[o] = v;
+  pm_half2 ab, cd;
+  ab[0] = (_Float16)a;
+  ab[1] = (_Float16)(b - a);
+  cd[0] = (_Float16)(c - a);
+  cd[1] = (_Float16)(d - c - b + a);
+  quadc[o] = make_uint2(__builtin_bit_cast(uint32_t, ab), __builtin_bit_cast(uint32_t, cd));
 }
 
 // The same quads as 16-bit values: uint2 {(a, b), (c, d)} with a, b the top
@@ -291,8 +305,8 @@ __device__ __forceinline__ float score_view_fast(const RefPatch<P>& rp,
                                                  const Warp& wp,
                                                  const PmView& V) {
   if constexpr (Prec == 0) {
-    GlobalFetch g;
-    g.tex = make_tex(V.quad, V.width, V.height);
+    CoefFetch g;
+    g.tex = make_tex16(V.quadc, V.width, V.height);
     return score_fast_affine<P>(rp, wp, g);
   } else {
     Global16Fetch<Prec == 2> g;
@@ -1364,11 +1378,12 @@ hipError_t launch_build_quad16(const uint8_t* img, uint32_t* quad16, int W,
   return hipGetLastError();
 }
 
-hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, int W, int H,
-                             hipStream_t s) {
+hipError_t launch_build_quad(const uint8_t* img, uint32_t* quad, uint32_t* quadc,
+                             int W, int H, hipStream_t s) {
   dim3 blk(16, 16);
   dim3 grd((W + 1 + 15) / 16, (H + 1 + 15) / 16);
-  hipLaunchKernelGGL(k_build_quad, grd, blk, 0, s, img, quad, W, H);
+  hipLaunchKernelGGL(k_build_quad, grd, blk, 0, s, img, quad,
+                     reinterpret_cast<uint2*>(quadc), W, H);
   return hipGetLastError();
 }
 

@@ -16,6 +16,8 @@ struct PmView {
   const uint32_t* quad;  // zero-bordered 2x2 texel quads, pitch width + 1
   const uint32_t* quad16;  // the same quads as 4 FP16 or BF16 values (8 B
                            // per texel), for reduced-precision PatchMatch
+  const uint32_t* quadc;   // bilinear coefficients of the quads as 4 FP16
+                           // values (8 B per texel), for FP32 PatchMatch
   int width, height;
   float M[9];
   float b[3];
