@@ -1,0 +1,112 @@
+"""Degenerate and ragged inputs of DepthmapEstimator: the HIP engine against
+the CPU oracle (oracle/), through the C ABI.
+
+  * a fully masked or a textureless reference view: ComputeIgnoreMask
+    (depthmap.cc:258-281) drops every pixel, so PatchMatch returns all-zero
+    maps, equal to the oracle's;
+  * the plane sweep ignores the mask (ComputeBruteForce, depthmap.cc:184-200):
+    equal to the oracle's for a masked reference too;
+  * ragged sizes: a 67x45 reference (no multiple of the 16x16 tiles) with a
+    neighbour cropped to 50x33 (its principal point shifted by the crop) and
+    one at full size: the plane sweep equals the oracle's, PatchMatch keeps
+    its invariants;
+  * a reference exactly one patch in size (7x7) gives the oracle's sweep; one
+    pixel less raises ValueError, as does a compute without a neighbour view
+    (the reference would score every plane -1 there; the engine refuses).
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from opensfm_amd import DepthmapEstimator, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(est, views, planes=12, patch=7, iters=2, seed=None):
+    for v in views:
+        est.add_view(v.K, v.R, v.t, v.image, v.mask)
+    est.set_depth_range(2.0, 20.0, planes)
+    est.set_patch_size(patch)
+    est.set_min_patch_sd(1.0)
+    est.set_patchmatch_iterations(iters)
+    if seed is not None:
+        est.set_seed(seed)
+    return est
+
+
+@pytest.fixture(scope="module")
+def ragged():
+    return scenes.make_scene(67, 45, 3, 3, seed=21, min_depth=2.0, max_depth=20.0)
+
+
+def _cropped(v, x0, y0, w, h):
+    K = v.K.copy()
+    K[0, 2] -= x0
+    K[1, 2] -= y0
+    return dataclasses.replace(v, K=K, image=np.ascontiguousarray(v.image[y0:y0 + h, x0:x0 + w]),
+                               mask=np.ascontiguousarray(v.mask[y0:y0 + h, x0:x0 + w]))
+
+
+def _equal_maps(g, o):
+    for a, b in zip(g, o):
+        assert a.shape == b.shape
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["masked", "textureless"])
+def test_reference_without_usable_pixels(ragged, kind):
+    print(f"PatchMatch and sweep, {kind} 67x45 reference", flush=True)
+    v0 = ragged.views[0]
+    if kind == "masked":
+        ref = dataclasses.replace(v0, mask=np.zeros_like(v0.mask))
+    else:
+        ref = dataclasses.replace(v0, image=np.full_like(v0.image, 77))
+    views = [ref] + ragged.views[1:]
+    g = _setup(DepthmapEstimator(), views, seed=5).compute_patch_match()
+    o = _setup(oracle.OracleEstimator(), views, seed=5).compute_patch_match()
+    _equal_maps(g, o)
+    assert not g[0].any() and not g[2].any()
+    gs = _setup(DepthmapEstimator(), views).compute_plane_sweep()
+    os_ = _setup(oracle.OracleEstimator(), views).compute_brute_force()
+    assert np.max(np.abs(gs[2] - os_[2])) <= 1e-4
+    assert np.array_equal(gs[0], os_[0]) and np.array_equal(gs[3], os_[3])
+
+
+def test_ragged_sizes(ragged):
+    print("sweep and PatchMatch, 67x45 reference with a 50x33 cropped neighbour", flush=True)
+    views = [ragged.views[0], _cropped(ragged.views[1], 9, 5, 50, 33), ragged.views[2]]
+    g = _setup(DepthmapEstimator(), views, planes=24).compute_plane_sweep()
+    o = _setup(oracle.OracleEstimator(), views, planes=24).compute_brute_force()
+    assert np.max(np.abs(g[2] - o[2])) <= 1e-4
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1]) and np.array_equal(g[3], o[3])
+    assert (o[0] > 0).mean() > 0.5
+    depth, plane, score, nghbr = _setup(DepthmapEstimator(), views, iters=3).compute_patch_match()
+    H, W = depth.shape
+    border = np.ones_like(depth, bool)
+    border[3:H - 3, 3:W - 3] = False
+    assert np.all(depth[border] == 0) and np.all(score[border] == 0)
+    assert np.all(np.isfinite(depth)) and np.all(depth >= 0)
+    assert np.all(score <= 1.0 + 1e-5) and np.all(score >= -1.0)
+    live = ~border & (plane[..., 2] != 0)
+    assert live.any() and np.all((nghbr[live] >= 1) & (nghbr[live] < len(views)))
+    assert (depth > 0).mean() > 0.3
+
+
+def test_one_patch_reference_and_size_errors(ragged):
+    print("7x7 reference (one patch), 6x7 and no-neighbour errors", flush=True)
+    v0 = ragged.views[0]
+    views = [_cropped(v0, 30, 19, 7, 7)] + ragged.views[1:]
+    g = _setup(DepthmapEstimator(), views, planes=24).compute_plane_sweep()
+    o = _setup(oracle.OracleEstimator(), views, planes=24).compute_brute_force()
+    assert np.max(np.abs(g[2] - o[2])) <= 1e-4
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[3], o[3])
+    # only the centre pixel has a full patch
+    assert np.count_nonzero(g[0]) <= 1 and g[0][3, 3] == o[0][3, 3]
+    small = [_cropped(v0, 30, 19, 6, 7)] + ragged.views[1:]
+    with pytest.raises(ValueError):
+        _setup(DepthmapEstimator(), small).compute_plane_sweep()
+    with pytest.raises(ValueError):
+        _setup(DepthmapEstimator(), [v0]).compute_patch_match()
