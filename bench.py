@@ -490,7 +490,8 @@ def main():
         "pcie_inclusive_mpix_s": round(npx / pcie_s / 1e6, 3),
         "plane_sweep": sweep,
         "roofline": {
-            "kernel": "k_pm_half<7> (red/black PatchMatch half-iteration)",
+            "kernel": "k_pm_half<7> (red/black PatchMatch half-iteration; the first iteration's "
+                      "two launches read the u8 quads, the others the FP16 coefficients)",
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,

@@ -253,14 +253,18 @@ void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   HIP_CHECK(pm::launch_fill_u32(b->state.as<unsigned>(), (int)(4 * b->total_px), 0u, b->stream));
   HIP_CHECK(pm::launch_fill_u32(b->score.as<unsigned>(), (int)b->total_px, 0u, b->stream));
   HIP_CHECK(pm::launch_fill_u32(b->nghbr.as<unsigned>(), (int)b->total_px, 0u, b->stream));
+  // FP32: the random planes of the initialisation and the first iteration
+  // are scored from the u8 quads (pm_types.h, kPrecFp32Quads)
+  const int first_prec = b->precision == PM_PRECISION_FP32 ? pm::kPrecFp32Quads : b->precision;
   HIP_CHECK(pm::launch_pm_init_batch(jt, nj, b->pix_total, b->patch_size,
-                                     b->precision, sample, q, cnt, b->stream));
+                                     first_prec, sample, q, cnt, b->stream));
   if (half_ev) HIP_CHECK(hipEventRecord(half_ev[0], b->stream));
   int l = 0;
   for (int it = 0; it < b->iterations; ++it) {
     for (int colour = 0; colour < 2; ++colour) {
       HIP_CHECK(pm::launch_pm_half_batch(jt, nj, b->half_total, b->patch_size,
-                                         b->precision, sample, b->tile, colour, it,
+                                         it == 0 ? first_prec : b->precision, sample,
+                                         b->tile, colour, it,
                                          q + 1 + 8 * l, cnt + 1, b->stream));
       ++l;
       if (half_ev) HIP_CHECK(hipEventRecord(half_ev[l], b->stream));

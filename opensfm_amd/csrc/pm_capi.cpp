@@ -266,7 +266,12 @@ int run(pm_estimator* e, int mode) {
     launches = 1;
   } else {
     const bool sample = mode == PM_MODE_PATCHMATCH_SAMPLE;
-    HIP_CHECK(pm::launch_pm_init(a, sample, e->stream));
+    // unseeded FP32 runs score random planes in the initialisation and the
+    // first iteration: those launches read the u8 quads
+    const bool quads_first = a.precision == PM_PRECISION_FP32 && !a.init_map;
+    pm::PmArgs ai = a;
+    if (quads_first) ai.precision = pm::kPrecFp32Quads;
+    HIP_CHECK(pm::launch_pm_init(ai, sample, e->stream));
     pm::PmArgs ah = a;  // the red/black launches count into slot 1
     ah.counter = a.counter + 1;
     // one event between consecutive launches; no host wait inside the loop
@@ -281,6 +286,7 @@ int run(pm_estimator* e, int mode) {
     HIP_CHECK(pm::launch_fill_u32(e->queues.as<unsigned>(), 8 * nl, 0u, e->stream));
     HIP_CHECK(hipEventRecord(e->half_ev[0], e->stream));
     for (int it = 0; it < e->iterations; ++it) {
+      ah.precision = (quads_first && it == 0) ? pm::kPrecFp32Quads : a.precision;
       for (int colour = 0; colour < 2; ++colour) {
         HIP_CHECK(pm::launch_pm_half(ah, sample, colour, it,
                                      e->queues.as<unsigned>() + 8 * launches, e->stream));

@@ -246,6 +246,23 @@ __device__ __forceinline__ uint2 fetch_quad16(const QuadTex& t, float flx,
 // Texel sources for the fast scorer: the texel of floor coordinates, and
 // its bilinear blend at fractions (fx, fy).
 
+// FP32 from the u8 quads, blended in the coefficient form of CoefFetch
+// (below) with the coefficients formed exactly in integers: the same
+// samples, bit for bit, from 4-byte texels.
+struct QuadFetch {
+  QuadTex tex;
+  __device__ __forceinline__ uint32_t operator()(float flx, float fly) const {
+    return fetch_quad(tex, flx, fly);
+  }
+  static __device__ __forceinline__ float bilerp(uint32_t t, float fx, float fy) {
+    const int a = (int)(t & 0xffu), b = (int)((t >> 8) & 0xffu);
+    const int c = (int)((t >> 16) & 0xffu), d = (int)(t >> 24);
+    const float top = __builtin_fmaf(fx, (float)(b - a), (float)a);
+    const float dif = __builtin_fmaf(fx, (float)(d - c - b + a), (float)(c - a));
+    return __builtin_fmaf(fy, dif, top);
+  }
+};
+
 typedef _Float16 pm_half2 __attribute__((ext_vector_type(2)));
 typedef __bf16 pm_bf162 __attribute__((ext_vector_type(2)));
 
@@ -401,7 +418,7 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
 // ---------------------------------------------------------------------------
 // Fast scorer for PatchMatch: the same local-affine bilateral NCC, with FMAs
 // and one-ulp reciprocals.  Takes the pixel's warp and a texel source
-// (CoefFetch or Global16Fetch: the texel of floor coordinates and its
+// (CoefFetch, QuadFetch or Global16Fetch: the texel of floor coordinates and its
 // bilinear blend); the NCC sums are FP32 for every texel format.
 // A non-finite coordinate yields a NaN score,
 // which never wins a strict comparison.

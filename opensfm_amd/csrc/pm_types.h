@@ -7,6 +7,15 @@ namespace pm {
 
 constexpr int kMaxViews = 64;  // reference view + up to 63 neighbours
 
+// Sample source of the FP32 PatchMatch launches that score random planes
+// (the initialisation and the first red/black iteration of an unseeded run):
+// the 4-byte u8 quads instead of the 8-byte FP16 coefficients.  Random
+// planes scatter the gathers over whole epipolar segments, so those
+// launches are bound by the cache lines they fetch; once planes have
+// converged the gathers stay within a few lines and the coefficients'
+// shorter blend wins.
+constexpr int kPrecFp32Quads = 3;
+
 // One view as the kernels see it.  For neighbour view v (v >= 1) the plane-
 // induced homography of plane v_pi (in the reference camera frame) is
 //   H_v = K_v (Q_v + a_v v_pi^T) K_0^-1 = M_v + b_v (K_0^-T v_pi)^T
@@ -44,7 +53,8 @@ struct PmArgs {
   int n_far;           // far propagation probes per direction
   int n_refine;        // random refinements per pixel update (6-step schedule)
   int n_explore;       // exploration samples per pixel update (0: none)
-  int precision;       // PatchMatch bilinear samples: 0 FP32, 1 FP16, 2 BF16
+  int precision;       // PatchMatch bilinear samples: 0 FP32, 1 FP16, 2 BF16,
+                       // kPrecFp32Quads (FP32 from the u8 quads)
   // Learned initialisation map (nullptr: random initialisation only):
   // init_h x init_w cells of init_stride floats (t, sx, sy, c logit, ...),
   // cell (gx, gy) centred on pixel ((gx + 0.5) init_scale - 0.5, ...).
