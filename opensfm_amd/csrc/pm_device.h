@@ -218,6 +218,23 @@ __device__ __forceinline__ uint32_t fetch_quad(const QuadTex& t, float flx,
   return __builtin_amdgcn_raw_buffer_load_b32(t.rsrc, rec << 2, 0, 0);
 }
 
+// floor(x) as an int in one instruction (x within the int range).
+__device__ __forceinline__ int cvt_floor(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// The texel of fetch_quad from unfloored coordinates: clamping x into
+// [-1, xmax] and then flooring equals flooring and then clamping, since the
+// bounds are integers.
+__device__ __forceinline__ uint32_t fetch_quad_xy(const QuadTex& t, float x, float y) {
+  const int ix = cvt_floor(__builtin_amdgcn_fmed3f(x, -1.0f, t.xmax));
+  const int iy = cvt_floor(__builtin_amdgcn_fmed3f(y, -1.0f, t.ymax));
+  const int rec = __mul24(iy, t.pitch) + ix;
+  return __builtin_amdgcn_raw_buffer_load_b32(t.rsrc, rec << 2, 0, 0);
+}
+
 // Texture of 8-byte quads (FP16 or BF16 values) of a wave-uniform view.
 __device__ __forceinline__ QuadTex make_tex16(const uint32_t* quad16, int W,
                                               int H) {
@@ -234,25 +251,26 @@ __device__ __forceinline__ QuadTex make_tex16(const uint32_t* quad16, int W,
   return t;
 }
 
-__device__ __forceinline__ uint2 fetch_quad16(const QuadTex& t, float flx,
-                                              float fly) {
-  const int ix = (int)__builtin_amdgcn_fmed3f(flx, -1.0f, t.xmax);
-  const int iy = (int)__builtin_amdgcn_fmed3f(fly, -1.0f, t.ymax);
+// The same for the 8-byte texels of a make_tex16 texture.
+__device__ __forceinline__ uint2 fetch_quad16_xy(const QuadTex& t, float x, float y) {
+  const int ix = cvt_floor(__builtin_amdgcn_fmed3f(x, -1.0f, t.xmax));
+  const int iy = cvt_floor(__builtin_amdgcn_fmed3f(y, -1.0f, t.ymax));
   const int rec = __mul24(iy, t.pitch) + ix;
   const auto v = __builtin_amdgcn_raw_buffer_load_b64(t.rsrc, rec << 3, 0, 0);
   return make_uint2(v[0], v[1]);
 }
 
-// Texel sources for the fast scorer: the texel of floor coordinates, and
-// its bilinear blend at fractions (fx, fy).
+// Texel sources for the fast scorer: the texel holding (floor x, floor y)
+// from the unfloored coordinates, and its bilinear blend at fractions
+// (fx, fy).
 
 // FP32 from the u8 quads, blended in the coefficient form of CoefFetch
 // (below) with the coefficients formed exactly in integers: the same
 // samples, bit for bit, from 4-byte texels.
 struct QuadFetch {
   QuadTex tex;
-  __device__ __forceinline__ uint32_t operator()(float flx, float fly) const {
-    return fetch_quad(tex, flx, fly);
+  __device__ __forceinline__ uint32_t operator()(float x, float y) const {
+    return fetch_quad_xy(tex, x, y);
   }
   static __device__ __forceinline__ float bilerp(uint32_t t, float fx, float fy) {
     const int a = (int)(t & 0xffu), b = (int)((t >> 8) & 0xffu);
@@ -276,8 +294,8 @@ typedef __bf16 pm_bf162 __attribute__((ext_vector_type(2)));
 // the two rows' blends with one rounding instead of three.
 struct CoefFetch {
   QuadTex tex;
-  __device__ __forceinline__ uint2 operator()(float flx, float fly) const {
-    return fetch_quad16(tex, flx, fly);
+  __device__ __forceinline__ uint2 operator()(float x, float y) const {
+    return fetch_quad16_xy(tex, x, y);
   }
   static __device__ __forceinline__ float bilerp(uint2 t, float fx, float fy) {
     const pm_half2 ab = __builtin_bit_cast(pm_half2, t.x);
@@ -295,8 +313,8 @@ struct CoefFetch {
 template <bool kBF16>
 struct Global16Fetch {
   QuadTex tex;
-  __device__ __forceinline__ uint2 operator()(float flx, float fly) const {
-    return fetch_quad16(tex, flx, fly);
+  __device__ __forceinline__ uint2 operator()(float x, float y) const {
+    return fetch_quad16_xy(tex, x, y);
   }
   static __device__ __forceinline__ float bilerp(uint2 t, float fx, float fy) {
     float im0, im1;
@@ -443,9 +461,8 @@ __device__ __forceinline__ float score_fast_affine(const RefPatch<P>& rp,
       const int k = (dy + hpz) * P + (dx + hpz);
       const float x2 = __builtin_fmaf(dfdx_x, (float)dx, rx);
       const float y2 = __builtin_fmaf(dfdx_y, (float)dx, ry);
-      const float flx = floorf(x2), fly = floorf(y2);
-      const auto t = fetch_texel(flx, fly);
-      const float y = Fetch::bilerp(t, x2 - flx, y2 - fly);
+      const auto t = fetch_texel(x2, y2);
+      const float y = Fetch::bilerp(t, __builtin_amdgcn_fractf(x2), __builtin_amdgcn_fractf(y2));
       const float wy = rp.w[k] * y;
       sumy += wy;
       sumyy = __builtin_fmaf(wy, y, sumyy);
