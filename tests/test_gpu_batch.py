@@ -107,6 +107,29 @@ def test_batch_patch5_and_autotune(street):
         assert _equal(b.result(k), _single(street, k, patch=patch))
 
 
+@pytest.mark.parametrize("patch", [3, 9, 11])
+def test_batch_other_patch_sizes(street, patch):
+    """Every FP32 patch size: the first iteration's launches read the u8
+    quads, the later ones the FP16 coefficients, in both paths."""
+    print(f"batch vs single-view estimator, patch {patch}", flush=True)
+    b = _batch(street, iters=3, patch=patch)
+    b.run()
+    for k in (0, 4):
+        got = b.result(k)
+        want = _single(street, k, iters=3, patch=patch)
+        assert _equal(got, want), f"job {k}"
+        depth, plane, score, nghbr = got
+        hpz = (patch - 1) // 2
+        H, W = depth.shape
+        border = np.ones_like(depth, bool)
+        border[hpz:H - hpz, hpz:W - hpz] = False
+        assert np.all(depth[border] == 0) and np.all(depth >= 0)
+        assert np.all(score <= 1.0 + 1e-5) and np.all(score >= -1.0)
+        assert (depth > 0).mean() > 0.2  # 3x3 patches on 160x120 views: ~0.3
+    b.run(sample=True)
+    assert _equal(b.result(2), _single(street, 2, iters=3, patch=patch, sample=True))
+
+
 def test_batch_clean_equals_cleaner_and_oracle(street):
     print("batched cleaner vs DepthmapCleaner and oracle", flush=True)
     b = _batch(street)
