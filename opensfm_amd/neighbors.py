@@ -18,6 +18,7 @@ results do not depend on the order of the device's atomics.
 """
 from __future__ import annotations
 
+import collections.abc
 import ctypes as C
 import dataclasses
 from typing import Dict, List, Sequence
@@ -34,9 +35,43 @@ class SuperPoint:
     tracks: list               # track ids of the merged landmarks
 
 
+class SuperPointList(collections.abc.Sequence):
+    """The graph's super-points as a read-only sequence of SuperPoint, built
+    on access from the arrays the device returned (one Python object per
+    super-point only when it is asked for; select_neighbors reads the
+    arrays directly)."""
+
+    def __init__(self, coord, moff, members, voff, vis, processable, tids):
+        self._coord, self._moff, self._members = coord, moff, members
+        self._voff, self._vis = voff, vis
+        self._processable, self._tids = processable, tids
+
+    def __len__(self):
+        return len(self._moff) - 1
+
+    def __getitem__(self, s):
+        if isinstance(s, slice):
+            return [self[k] for k in range(*s.indices(len(self)))]
+        n = len(self)
+        if s < 0:
+            s += n
+        if not 0 <= s < n:
+            raise IndexError("super-point index out of range")
+        return SuperPoint(coord=self._coord[s].copy(),
+                          vis=[self._processable[v] for v in self._vis[self._voff[s]:self._voff[s + 1]]],
+                          tracks=[self._tids[m] for m in self._members[self._moff[s]:self._moff[s + 1]]])
+
+    def __eq__(self, other):
+        if isinstance(other, (list, tuple, SuperPointList)):
+            return len(self) == len(other) and all(
+                np.array_equal(a.coord, b.coord) and list(a.vis) == list(b.vis)
+                and list(a.tracks) == list(b.tracks) for a, b in zip(self, other))
+        return NotImplemented
+
+
 @dataclasses.dataclass
 class CovisibilityGraph:
-    super_points: List[SuperPoint]
+    super_points: Sequence[SuperPoint]
     edges: List[tuple]         # (shot_idx_i, shot_idx_j), i < j
     weights: List[float]       # shared super-points per edge
     shot_order: list           # index -> shot id
@@ -118,10 +153,7 @@ def build_covisibility_graph(coords, observers: Sequence[Sequence], processable:
         C.byref(scale)))
     ms = C.c_float(0)
     _capi.check(h.lib.pm_covis_last_ms(h.h, C.byref(ms)))
-    sps = [SuperPoint(coord=sp_coord[s].copy(),
-                      vis=[processable[v] for v in vis[voff[s]:voff[s + 1]]],
-                      tracks=[tids[m] for m in members[moff[s]:moff[s + 1]]])
-           for s in range(nsp)]
+    sps = SuperPointList(sp_coord[:nsp], moff, members, voff, vis, processable, tids)
     return CovisibilityGraph(super_points=sps, edges=[tuple(e) for e in edges[:ne].tolist()],
                              weights=[float(w) for w in weights[:ne]],
                              shot_order=processable, scale=scale.value,
@@ -143,11 +175,19 @@ def select_neighbors(origins, super_points: Sequence[SuperPoint], processable: S
     if len(org) != len(processable):
         raise ValueError("one camera centre per processable shot is needed")
     nsp = len(super_points)
-    coords = np.ascontiguousarray(
-        np.asarray([sp.coord for sp in super_points], np.float64).reshape(-1, 3))
-    voff, vis = _csr([sp.vis for sp in super_points], index)
-    ntr = np.ascontiguousarray(np.asarray([len(sp.tracks) for sp in super_points],
-                                          np.int32).reshape(-1))
+    if isinstance(super_points, SuperPointList) and list(super_points._processable) == processable:
+        # a graph's own super-points: its arrays, visibility already as
+        # indices into the same shot order
+        coords = np.ascontiguousarray(super_points._coord, np.float64).reshape(-1, 3)
+        voff = np.ascontiguousarray(super_points._voff, np.int64)
+        vis = np.ascontiguousarray(super_points._vis[:max(int(voff[-1]), 1)], np.int32)
+        ntr = np.ascontiguousarray(np.diff(super_points._moff), np.int32)
+    else:
+        coords = np.ascontiguousarray(
+            np.asarray([sp.coord for sp in super_points], np.float64).reshape(-1, 3))
+        voff, vis = _csr([sp.vis for sp in super_points], index)
+        ntr = np.ascontiguousarray(np.asarray([len(sp.tracks) for sp in super_points],
+                                              np.int32).reshape(-1))
     h = _Handle(device)
     nb, na = C.c_int64(0), C.c_int64(0)
     _capi.check(h.lib.pm_covis_select_neighbors(
