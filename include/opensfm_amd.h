@@ -169,8 +169,11 @@ int pm_estimator_score_hypotheses_fast(pm_estimator* est, int n,
  * (device memory, h x w cells of `stride` floats: t, sx, sy, c) at 1/scale:
  * each pixel blends the map bilinearly, and where the confidence logit
  * c >= 0 starts from depth exp(log_min + t (log_max - log_min)) and normal
- * (sx, sy, -1) instead of a random plane.  The map must stay valid until
- * the next compute; NULL restores random initialisation. */
+ * (sx, sy, -1) instead of a random plane (and takes every exploration
+ * sample from the first iteration on).  The map is read by the
+ * initialisation and the first red/black iteration, so it must stay valid
+ * through every compute made while it is set; NULL restores random
+ * initialisation. */
 int pm_estimator_initializer_input(pm_estimator* est, int scale, int nplanes,
                                    void* out, int out_w, int out_h);
 int pm_estimator_set_init_map(pm_estimator* est, const float* map, int w,
