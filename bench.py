@@ -136,12 +136,13 @@ def cpu_baseline(scene, threads, target_s=12.0):
     }
 
 
-def batch_bench(n_views, steps=3, seed=4000):
+def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
     """Batched street workload: n_views reference views of sizes drawn from
     {640x480, 1024x768, 1600x1200, 2048x1536}, 4-12 neighbours each, 3
     iterations in one captured hipGraph, tile and patch {5, 7} autotuned,
     then the cleaner with 2 consistent views.  Compared with one
-    DepthmapEstimator run per view on the same GPU."""
+    DepthmapEstimator run per view on the same GPU; the cleaner also with the
+    oracle's one-thread Clean on the first 4 views' depthmaps."""
     from opensfm_amd import DepthmapBatch, DepthmapEstimator, scenes
     from opensfm_amd._capi import PM_MODE_PATCHMATCH, TILE_NAMES
     log(f"batched workload: rendering {n_views} street views")
@@ -181,6 +182,25 @@ def batch_bench(n_views, steps=3, seed=4000):
         m = c > 0
         kept += int(m.sum())
         kept_good += int((m & (np.abs(c - gt) < 0.01 * gt)).sum())
+    clean_cpu = None
+    if cpu_baseline:
+        # DepthmapCleaner::Clean (depthmap.cc:524-555) by the oracle, one
+        # thread, on the same packed depthmaps; the GPU's must be bit-equal
+        import oracle
+        depth = {ref: b.result(k)[0] for k, (ref, _) in enumerate(sc.jobs)}
+        cpu_s, cpu_px, same = 0.0, 0, True
+        for k, (ref, nb) in enumerate(sc.jobs[:4]):
+            if not all(v in depth for v in nb):
+                continue
+            views = [(sc.views[v].K, sc.views[v].R, sc.views[v].t, depth[v]) for v in [ref] + list(nb)]
+            t0 = time.perf_counter()
+            want = oracle.clean(views, 0.01, 2)
+            cpu_s += time.perf_counter() - t0
+            cpu_px += want.size
+            same = same and bool(np.array_equal(want, b.clean_result(k)))
+        if cpu_px:
+            clean_cpu = {"views": 4, "mpix_s": round(cpu_px / cpu_s / 1e6, 3), "cores": 1,
+                         "kind": "port", "bit_exact": same}
     # one estimator per view, device-resident runs, same settings
     per_view_s = 0.0
     for k, (ref, nb) in enumerate(sc.jobs):
@@ -205,7 +225,9 @@ def batch_bench(n_views, steps=3, seed=4000):
         "accuracy_within_1pct": round(float(np.mean(accs)), 4),
         "mean_score": round(float(np.mean(scores)), 4),
         "clean_ms": round(clean_ms, 3),
+        "clean_mpix_s": round(npx / (clean_ms / 1e3) / 1e6, 1) if clean_ms > 0 else None,
         "clean_kept_within_1pct": round(kept_good / max(1, kept), 4),
+        "clean_cpu_baseline": clean_cpu,
     }
 
 
@@ -572,7 +594,7 @@ def main():
             "gpu_at_equal_crop_accuracy": match,
         }
     if args.batch_views > 0:
-        out["batched"] = batch_bench(args.batch_views)
+        out["batched"] = batch_bench(args.batch_views, cpu_baseline=not args.no_cpu_baseline)
         log(f"batched: {out['batched']}")
     if not args.no_fusion:
         fv = fusion_views(scene)
