@@ -246,6 +246,38 @@ def fusion_views(scene, seed=77):
     return views
 
 
+def gpu_cleaner_bench(views, cpu_baseline=True):
+    """GPUDepthmapCleaner (space carving + backward check, cleaner.cc:56-179)
+    of view 0 against the other 8 views of the 1600x1200 scene (noisy
+    ground-truth depths), beside the oracle's float32 restatement of the
+    same clean on the host (one thread, numpy); the maps must be equal."""
+    from opensfm_amd import GPUDepthmapCleaner
+    c = GPUDepthmapCleaner()
+    for v in views:
+        c.add_view(v[0], v[1], v[2], v[3])
+    nb = list(range(1, len(views)))
+    c.clean(0, nb)  # warm-up
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        got = c.clean(0, nb)
+    gpu_s = (time.perf_counter() - t0) / reps
+    mpix = views[0][3].size / 1e6
+    out = {"workload": f"view 0 of {len(views)} views {views[0][3].shape[1]}x{views[0][3].shape[0]} "
+                       f"against {len(nb)} neighbours, defaults",
+           "gpu_wall_ms": round(gpu_s * 1e3, 3), "mpix_s": round(mpix / gpu_s, 1),
+           "kept": int((got > 0).sum())}
+    if cpu_baseline:
+        import oracle
+        t0 = time.perf_counter()
+        want = oracle.gpu_clean([tuple(v[:4]) for v in views], 0, nb)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"mpix_s": round(mpix / cpu_s, 3), "cores": 1, "kind": "port",
+                               "bit_exact": bool(np.array_equal(got, want))}
+        out["gpu_over_cpu"] = round(cpu_s / gpu_s, 1)
+    return out
+
+
 def fusion_bench(views):
     """The steps after estimation: DepthmapFuser over all views, 4 consistent
     views, margin 10, with and without outlier removal (k 12); then
@@ -381,7 +413,7 @@ def main():
     ap.add_argument("--learned-views", type=int, default=4, metavar="N",
                     help="also time the learned initialiser on N nadir views (0: skip)")
     ap.add_argument("--no-fusion", action="store_true",
-                    help="skip the fusion / pruning section")
+                    help="skip the GPU cleaner / fusion / pruning sections")
     ap.add_argument("--no-neighbors", action="store_true",
                     help="skip the neighbour-selection section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -598,6 +630,8 @@ def main():
         log(f"batched: {out['batched']}")
     if not args.no_fusion:
         fv = fusion_views(scene)
+        out["gpu_cleaner"] = gpu_cleaner_bench(fv, cpu_baseline=not args.no_cpu_baseline)
+        log(f"gpu cleaner: {out['gpu_cleaner']}")
         out["fusion"] = fusion_bench(fv)
         if not args.no_cpu_baseline:
             out["fusion"]["cpu_baseline"] = fusion_cpu_baseline(fv)
