@@ -143,7 +143,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
     then the cleaner with 2 consistent views.  Compared with one
     DepthmapEstimator run per view on the same GPU; the cleaner also with the
     oracle's one-thread Clean on the first 4 views' depthmaps."""
-    from opensfm_amd import DepthmapBatch, DepthmapEstimator, scenes
+    from opensfm_amd import DepthmapBatch, DepthmapClusterEstimator, DepthmapEstimator, scenes
     from opensfm_amd._capi import PM_MODE_PATCHMATCH, TILE_NAMES
     log(f"batched workload: rendering {n_views} street views")
     t0 = time.perf_counter()
@@ -216,6 +216,23 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         t0 = time.perf_counter()
         e.run(PM_MODE_PATCHMATCH)
         per_view_s += time.perf_counter() - t0
+    # the same jobs through the cluster estimator's surface (dense.py's call
+    # sequence): host images in, [depth, normal, cost, confidence] out
+    ce = DepthmapClusterEstimator()
+    ce.set_max_iterations(3)
+    ce.set_patch_size(patch)
+    for ref, nb in sc.jobs:
+        R = sc.views[ref]
+        ce.begin_ref_view(R.K, R.R, R.t, R.image, sc.min_depth, sc.max_depth)
+        for v in nb:
+            ce.add_source_view(sc.views[v].K, sc.views[v].R, sc.views[v].t, sc.views[v].image)
+    ce.run()
+    t0 = time.perf_counter()
+    cres = ce.run()
+    cluster_s = time.perf_counter() - t0
+    cluster_acc = float(np.mean([scenes.depth_accuracy(r[0], sc.views[ref].depth, sc.min_depth,
+                                                       sc.max_depth)
+                                 for r, (ref, _) in zip(cres, sc.jobs)]))
     return {
         "views": n_views, "mpix": round(npx / 1e6, 2), "iterations": 3,
         "patch": patch, "tile": TILE_NAMES[tile], "autotune_s": round(tune_s, 2),
@@ -228,6 +245,9 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         "clean_mpix_s": round(npx / (clean_ms / 1e3) / 1e6, 1) if clean_ms > 0 else None,
         "clean_kept_within_1pct": round(kept_good / max(1, kept), 4),
         "clean_cpu_baseline": clean_cpu,
+        "cluster_estimator": {"wall_ms": round(cluster_s * 1e3, 1),
+                              "mpix_s": round(npx / cluster_s / 1e6, 3),
+                              "accuracy_within_1pct": round(cluster_acc, 4)},
     }
 
 

@@ -418,10 +418,8 @@ int pm_batch_add_view(pm_batch* b, const double* K, const double* R,
     const size_t np = (size_t)width * height;
     v->img.ensure(np);
     v->mask.ensure(np);
-    HIP_CHECK(hipMemcpy2DAsync(v->img.p, width, image, stride, width, height,
-                               hipMemcpyHostToDevice, b->stream));
-    HIP_CHECK(hipMemcpy2DAsync(v->mask.p, width, mask, stride, width, height,
-                               hipMemcpyHostToDevice, b->stream));
+    upload_image(v->img.p, image, width, height, stride, b->stream);
+    upload_image(v->mask.p, mask, width, height, stride, b->stream);
     v->quad.ensure((size_t)(width + 1) * (height + 1) * sizeof(uint32_t));
     v->quadc.ensure((size_t)(width + 1) * (height + 1) * 8);
     HIP_CHECK(pm::launch_build_quad(v->img.as<uint8_t>(), v->quad.as<uint32_t>(),

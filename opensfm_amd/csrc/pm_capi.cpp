@@ -561,16 +561,14 @@ int pm_estimator_add_view(pm_estimator* e, const double* K, const double* R,
     DeviceGuard dg(e->device);
     const size_t np = (size_t)width * height;
     v->img.ensure(np);
-    HIP_CHECK(hipMemcpy2DAsync(v->img.p, width, image, stride, width, height,
-                               hipMemcpyHostToDevice, e->stream));
+    upload_image(v->img.p, image, width, height, stride, e->stream);
     v->quad.ensure((size_t)(width + 1) * (height + 1) * sizeof(uint32_t));
     v->quadc.ensure((size_t)(width + 1) * (height + 1) * 8);
     HIP_CHECK(pm::launch_build_quad(v->img.as<uint8_t>(), v->quad.as<uint32_t>(),
                                     v->quadc.as<uint32_t>(), width, height, e->stream));
     if (e->views.empty()) {
       e->mask0.ensure(np);
-      HIP_CHECK(hipMemcpy2DAsync(e->mask0.p, width, mask, stride, width, height,
-                                 hipMemcpyHostToDevice, e->stream));
+      upload_image(e->mask0.p, mask, width, height, stride, e->stream);
     }
     HIP_CHECK(hipStreamSynchronize(e->stream));
     e->views.push_back(std::move(v));

@@ -137,6 +137,25 @@ struct DevBuf {
   }
 };
 
+// Host-to-device upload of a width x height u8 image with row stride
+// `stride` on `stream`.  A pitched copy from pageable memory goes row by row
+// (about 8 ms for 480 rows), so contiguous rows go as one transfer and
+// strided ones are packed on the host first.  The caller synchronises the
+// stream before the host image may change.
+inline void upload_image(void* dst, const uint8_t* src, int width, int height, int stride,
+                         hipStream_t stream) {
+  const size_t n = (size_t)width * height;
+  if (stride == width) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream));
+    return;
+  }
+  std::vector<uint8_t> packed(n);
+  for (int y = 0; y < height; ++y)
+    memcpy(packed.data() + (size_t)y * width, src + (size_t)y * stride, (size_t)width);
+  HIP_CHECK(hipMemcpyAsync(dst, packed.data(), n, hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));  // before `packed` goes away
+}
+
 struct DeviceGuard {
   int prev = -1;
   explicit DeviceGuard(int dev) {
