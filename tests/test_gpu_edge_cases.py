@@ -12,7 +12,9 @@ the CPU oracle (oracle/), through the C ABI.
     its invariants;
   * a reference exactly one patch in size (7x7) gives the oracle's sweep; one
     pixel less raises ValueError, as does a compute without a neighbour view
-    (the reference would score every plane -1 there; the engine refuses).
+    (the reference would score every plane -1 there; the engine refuses);
+  * images handed over with a row stride above their width give the same
+    maps as contiguous ones.
 """
 import dataclasses
 
@@ -110,3 +112,40 @@ def test_one_patch_reference_and_size_errors(ragged):
         _setup(DepthmapEstimator(), small).compute_plane_sweep()
     with pytest.raises(ValueError):
         _setup(DepthmapEstimator(), [v0]).compute_patch_match()
+
+
+def test_strided_image_upload(ragged):
+    """pm_estimator_add_view with a row stride above the width (rows packed on
+    the host before the upload) gives the same maps as contiguous rows."""
+    import ctypes as C
+    from opensfm_amd import _capi
+    print("add_view with stride > width through the C ABI", flush=True)
+    views = ragged.views
+    want = _setup(DepthmapEstimator(), views, planes=24).compute_plane_sweep()
+    est = DepthmapEstimator()
+    for v in views:
+        h, w = v.image.shape
+        pad = 13
+        img = np.zeros((h, w + pad), np.uint8)
+        img[:, :w] = v.image
+        img[:, w:] = 200                   # padding that must not be read
+        mask = np.zeros((h, w + pad), np.uint8)
+        mask[:, :w] = v.mask
+        K = np.ascontiguousarray(v.K, np.float64)
+        R = np.ascontiguousarray(v.R, np.float64)
+        t = np.ascontiguousarray(v.t, np.float64).reshape(3)
+        _capi.check(est._lib.pm_estimator_add_view(
+            est._h, _capi.ptr(K, C.c_double), _capi.ptr(R, C.c_double),
+            _capi.ptr(t, C.c_double), _capi.ptr(img, C.c_uint8), _capi.ptr(mask, C.c_uint8),
+            w, h, w + pad))
+        if est._shape is None:
+            est._shape = (h, w)
+    est.set_depth_range(2.0, 20.0, 24)
+    est.set_patch_size(7)
+    est.set_min_patch_sd(1.0)
+    got = est.compute_plane_sweep()
+    _equal_maps(got, want)
+    est.set_patchmatch_iterations(2)
+    est.set_seed(3)
+    e2 = _setup(DepthmapEstimator(), views, planes=24, iters=2, seed=3)
+    _equal_maps(est.compute_patch_match(), e2.compute_patch_match())
