@@ -628,7 +628,12 @@ __global__ __launch_bounds__(256) void k_init_input(PmArgs a, int nplanes,
 // pixels hold similar planes, a wave's gathers stay within a few L1 lines,
 // and building the footprint box costs more than the LDS reads save.
 
-constexpr float kExploreBelow = 0.95f;
+// Pixels scoring this or more skip exploration.  Measured (single view at 5
+// iterations / 16-view street batch at 3): 0.95 gives 34.9 / 119.3 Mpix/s at
+// accuracy 0.9609 / 0.7091, 0.97 34.8 / 117.8 at 0.9610 / 0.7134, 0.99
+// 34.2 / 115.7 at 0.9612 / 0.7194, exploring everywhere 31.5 / 111.6 at
+// 0.9612 / 0.7241.
+constexpr float kExploreBelow = 0.97f;
 
 // Tile (tile_x, tile_y) of TX x TY lanes covers columns [2 TX tile_x,
 // 2 TX (tile_x + 1)) and rows [TY tile_y, TY (tile_y + 1)); each lane takes
@@ -737,8 +742,7 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
       // right before their normals are); the others keep the current depth
       // with a normal drawn uniformly over the cone of half-angle 87 degrees
       // around the optical axis.  Pixels already scoring kExploreBelow or
-      // more skip them (measured on the 1600x1200 scene: same accuracy,
-      // 13 % less time than exploring everywhere).  In the first iteration
+      // more skip them.  In the first iteration
       // pixels that started from a random plane skip samples 0 and 1: their
       // neighbours' points are random too (measured: the 1600x1200 scene
       // at 5 iterations 7 % faster with accuracy 0.9609 against 0.9605; the
