@@ -459,7 +459,7 @@ def main():
         log(f"CPU baseline {cpu['value']:.4f} Mpix/s ({cpu['wall_s']:.1f} s)")
 
     from opensfm_amd import DepthmapEstimator, scenes
-    from opensfm_amd._capi import PM_MODE_PATCHMATCH, PM_MODE_PLANE_SWEEP
+    from opensfm_amd._capi import PM_MODE_PATCHMATCH, PM_MODE_PATCHMATCH_SAMPLE, PM_MODE_PLANE_SWEEP
 
     if args.copy_calibration:
         import torch
@@ -531,6 +531,20 @@ def main():
                                                             w["min_depth"], w["max_depth"]), 4)}
     log(f"matched pixel updates: {matched_updates}")
 
+    # ComputePatchMatchSample (depthmap.cc:215-227): each candidate scored
+    # against one view, the same iterations
+    est.run(PM_MODE_PATCHMATCH_SAMPLE)
+    t5 = time.perf_counter()
+    est.run(PM_MODE_PATCHMATCH_SAMPLE)
+    smp_s = time.perf_counter() - t5
+    smp_depth = est.download()[0]
+    sample_mode = {
+        "iterations": w["iterations"],
+        "mpix_s": round(w["width"] * w["height"] / smp_s / 1e6, 3),
+        "accuracy_within_1pct": round(scenes.depth_accuracy(smp_depth, scene.ref.depth,
+                                                            w["min_depth"], w["max_depth"]), 4)}
+    log(f"sample mode: {sample_mode}")
+
     npx = w["width"] * w["height"]
     ms_per_step = elapsed / args.steps * 1e3
     value = npx * args.steps / elapsed / 1e6
@@ -560,6 +574,7 @@ def main():
                    "patch": w["patch"], "iterations": w["iterations"]},
         "accuracy_within_1pct": round(acc, 4),
         "matched_pixel_updates": matched_updates,
+        "sample_mode": sample_mode,
         "mean_score": round(mean_score, 4),
         "pcie_inclusive_mpix_s": round(npx / pcie_s / 1e6, 3),
         "plane_sweep": sweep,
