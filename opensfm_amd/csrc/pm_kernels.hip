@@ -1061,28 +1061,34 @@ __global__ __launch_bounds__(256) void k_pm_init_batch(
 
 // Single-view red/black half-iteration, persistent and XCD-banded.  The
 // tile rows are split into 8 bands; workgroup b starts on band b % 8 -- the
-// XCD the hardware deals it to -- and pulls that band's tiles in raster
-// order from the band's queue, then moves on to the other bands' queues
-// until all are dry.  Each XCD thus works through one compact region at a
-// time, so its L2 holds the neighbour-view footprint of that region, and
-// the band switching at the end balances the XCDs' loads.  queues[0..7]
-// start at zero.
+// XCD the hardware deals it to -- and pulls that band's tiles from the
+// band's queue, then moves on to the other bands' queues until all are dry.
+// Within a band the tiles go in column blocks of kBandBlockCols tiles, each
+// block row by row, so an XCD's active tiles cover a compact region and its
+// L2 holds that region's neighbour-view footprint; the band switching at the
+// end balances the XCDs' loads.  The order changes no result (a half-
+// iteration reads only the other colour).  queues[0..7] start at zero.
+constexpr int kBandBlockCols = 20;  // 640 image columns; 12-25 measured alike
+
 template <int P, bool kSample, int Prec>
 __global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour, int iter,
                                                  unsigned* __restrict__ queues) {
   const int tiles_x = ((a.W + 1) / 2 + 15) >> 4;
-  const int ntiles = tiles_x * ((a.H + 15) >> 4);
+  const int tiles_y = (a.H + 15) >> 4;
   const int home = (int)(blockIdx.x & 7u);
   unsigned n = 0;
   for (int s = 0; s < 8; ++s) {
     const int band = (home + s) & 7;
-    const int t0 = (int)((long long)band * ntiles / 8);
-    const int t1 = (int)((long long)(band + 1) * ntiles / 8);
+    const int r0 = band * tiles_y / 8, rows = (band + 1) * tiles_y / 8 - r0;
+    const int per_block = kBandBlockCols * rows;
     for (;;) {
-      const int t = t0 + next_tile(queues + band);
-      if (t >= t1) break;
-      n += half_pixel<P, kSample, 16, 16, Prec>(a, colour, iter, t % tiles_x,
-                                                t / tiles_x);
+      const int u = next_tile(queues + band);
+      if (u >= rows * tiles_x) break;
+      const int blk = u / per_block, rem = u - blk * per_block;
+      const int bw = min(kBandBlockCols, tiles_x - blk * kBandBlockCols);
+      n += half_pixel<P, kSample, 16, 16, Prec>(a, colour, iter,
+                                                blk * kBandBlockCols + rem % bw,
+                                                r0 + rem / bw);
     }
   }
   if (a.counter) wave_count(a.counter, n);
