@@ -1112,9 +1112,15 @@ __global__ __launch_bounds__(256) void k_pm_half_batch(
       const int t = t0 + next_tile(queues + band);
       if (t >= t1) break;
       const PmJob& J = jobs[find_job(jobs, njobs, t, &PmJob::half_t0)];
+      // a job's tiles in column blocks of 512 image columns, each block row
+      // by row (compact active regions, as in k_pm_half; no result changes)
+      constexpr int B = 256 / TX;
       const int lt = t - J.half_t0;
-      n += half_pixel<P, kSample, TX, TY, Prec>(J.a, colour, iter,
-                                                lt % J.half_tx, lt / J.half_tx);
+      const int per = B * J.half_ty;
+      const int blk = lt / per, rem = lt - blk * per;
+      const int bw = min(B, J.half_tx - blk * B);
+      n += half_pixel<P, kSample, TX, TY, Prec>(J.a, colour, iter, blk * B + rem % bw,
+                                                rem / bw);
     }
   }
   if (count) wave_count(count, n);
