@@ -439,6 +439,15 @@ int pm_batch_set_seed(pm_batch* b, uint64_t seed);
 int pm_batch_set_exploration(pm_batch* b, int n);
 int pm_batch_set_post_process(pm_batch* b, int enable);
 int pm_batch_set_precision(pm_batch* b, int precision);
+/* Learned initialisation per job (not in the reference; the batched form of
+ * pm_estimator_set_init_map / _set_init_search / _set_init_spread, with the
+ * same meaning and defaults): job `job` of a batch with a map equals a
+ * pm_estimator given the same map, seed + job and settings.  The map must
+ * stay valid through every run made while it is set; NULL clears it. */
+int pm_batch_set_init_map(pm_batch* b, int job, const float* map, int w, int h, int stride,
+                          int scale);
+int pm_batch_set_init_search(pm_batch* b, int n);
+int pm_batch_set_init_spread(pm_batch* b, float sigma_log_depth, float sigma_slope);
 /* Lane layout of the red/black launches (default PM_TILE_16x16).  The maps
  * do not depend on it. */
 int pm_batch_set_tile(pm_batch* b, int tile);

@@ -146,6 +146,9 @@ SIGNATURES = {
     "pm_batch_set_patch_size": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_min_patch_sd": (C.c_int, [_vp, C.c_float]),
     "pm_batch_set_seed": (C.c_int, [_vp, C.c_uint64]),
+    "pm_batch_set_init_map": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "pm_batch_set_init_search": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_init_spread": (C.c_int, [_vp, C.c_float, C.c_float]),
     "pm_batch_set_exploration": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_tile": (C.c_int, [_vp, C.c_int]),

@@ -91,6 +91,21 @@ class DepthmapBatch:
     def set_exploration(self, n: int) -> None:
         _capi.check(self._lib.pm_batch_set_exploration(self._h, int(n)))
 
+    def set_init_map(self, job: int, map_ptr, w: int = 0, h: int = 0, stride: int = 8,
+                     scale: int = 4) -> None:
+        """Seeds job `job`'s initialisation from a coarse (t, sx, sy, c) map in
+        device memory, as DepthmapEstimator.set_init_map (None: random)."""
+        _capi.check(self._lib.pm_batch_set_init_map(
+            self._h, int(job), C.c_void_p(map_ptr) if map_ptr else None, int(w), int(h),
+            int(stride), int(scale)))
+
+    def set_init_search(self, n: int) -> None:
+        _capi.check(self._lib.pm_batch_set_init_search(self._h, int(n)))
+
+    def set_init_spread(self, sigma_log_depth: float, sigma_slope: float) -> None:
+        _capi.check(self._lib.pm_batch_set_init_spread(self._h, float(sigma_log_depth),
+                                                       float(sigma_slope)))
+
     def set_post_process(self, enable: bool) -> None:
         _capi.check(self._lib.pm_batch_set_post_process(self._h, int(bool(enable))))
 

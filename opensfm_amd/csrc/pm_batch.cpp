@@ -38,6 +38,9 @@ struct BatchJob {
   std::vector<int> nb;
   double min_depth = 0, max_depth = 0;
   size_t off = 0;  // first pixel in the flat per-pixel buffers
+  // learned initialisation map of this job (device memory; null: random)
+  const float* init_map = nullptr;
+  int init_w = 0, init_h = 0, init_stride = 0, init_scale = 1;
 };
 
 }  // namespace
@@ -52,6 +55,8 @@ struct pm_batch {
   float min_patch_variance = 5 * 5;
   uint64_t seed = 0x5EEDull;
   int n_far = 6, n_refine = 6, n_explore = 3;
+  int init_search = 1;
+  float init_sigma = 0.05f, init_sigma_slope = 0.3f;
   int post_process = 1;
   int tile = 1;
   int use_graph = 1;
@@ -213,7 +218,14 @@ void prepare(pm_batch* b, int count) {
     a.n_refine = b->n_refine;
     a.n_explore = b->n_explore;
     a.precision = b->precision;
-    a.init_search = 1;
+    a.init_map = J.init_map;
+    a.init_w = J.init_w;
+    a.init_h = J.init_h;
+    a.init_stride = J.init_stride;
+    a.init_scale = J.init_scale;
+    a.init_search = b->init_search;
+    a.init_sigma = b->init_sigma;
+    a.init_sigma_slope = b->init_sigma_slope;
     a.counter = nullptr;
     T.half_tx = ceil_div((R.width + 1) / 2, tx);
     T.half_ty = ceil_div(R.height, ty);
@@ -481,6 +493,40 @@ int pm_batch_set_patch_size(pm_batch* b, int size) {
 int pm_batch_set_min_patch_sd(pm_batch* b, float sd) {
   if (!b) return fail(PM_ERR_INVALID, "null batch");
   b->min_patch_variance = sd * sd;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_init_map(pm_batch* b, int job, const float* map, int w, int h, int stride,
+                           int scale) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (job < 0 || job >= (int)b->jobs.size()) return fail(PM_ERR_RANGE, "job index out of range");
+  if (map && (w < 1 || h < 1 || stride < 4 || scale < 1))
+    return fail(PM_ERR_INVALID, "init map needs w, h >= 1, stride >= 4, scale >= 1");
+  BatchJob& J = b->jobs[job];
+  J.init_map = map;
+  J.init_w = map ? w : 0;
+  J.init_h = map ? h : 0;
+  J.init_stride = map ? stride : 0;
+  J.init_scale = map ? scale : 1;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_init_spread(pm_batch* b, float sigma_log_depth, float sigma_slope) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (!(sigma_log_depth >= 0) || !(sigma_slope >= 0) || sigma_log_depth > 1 || sigma_slope > 10)
+    return fail(PM_ERR_INVALID, "spreads must be in [0, 1] and [0, 10]");
+  b->init_sigma = sigma_log_depth;
+  b->init_sigma_slope = sigma_slope;
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_init_search(pm_batch* b, int n) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (n < 1 || n > 15) return fail(PM_ERR_INVALID, "init search must be in [1, 15]");
+  b->init_search = n;
   b->invalidate();
   return PM_OK;
 }

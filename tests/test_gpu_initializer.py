@@ -10,14 +10,16 @@ trained network gains.
   iteration at all, and a map with no confident cell reproduces plain
   PatchMatch bit for bit;
 * the committed weights (opensfm_amd/data/initializer_unet.npz) gain over
-  random initialisation at the same iteration count on held-out nadir views.
+  random initialisation at the same iteration count on held-out nadir views;
+* a batched job seeded with a map equals the single-view estimator seeded
+  with the same map (and an unseeded job equals plain PatchMatch).
 """
 import numpy as np
 import pytest
 import torch
 
 import oracle
-from opensfm_amd import DepthmapEstimator, initializer as I, nn, scenes
+from opensfm_amd import DepthmapBatch, DepthmapEstimator, initializer as I, nn, scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -143,3 +145,46 @@ def test_trained_network_gains_over_random_init(nadir):
     print(f"  accuracy after 1 iteration: learned {np.mean(acc_l):.4f}, random {np.mean(acc_r):.4f}",
           flush=True)
     assert np.mean(acc_l) >= np.mean(acc_r) + 0.1
+
+
+def test_batched_learned_init_equals_estimator(nadir):
+    print("batched learned initialisation vs single-view estimator", flush=True)
+    init = I.LearnedInitializer(weights=I.WEIGHTS)
+    seed = 90
+    jobs = nadir.jobs[:3]
+    b = DepthmapBatch()
+    for v in nadir.views:
+        b.add_view(v.K, v.R, v.t, v.image, v.mask)
+    for ref, nb in jobs:
+        b.add_job(ref, nb, nadir.min_depth, nadir.max_depth)
+    b.set_patchmatch_iterations(1)
+    b.set_patch_size(7)
+    b.set_min_patch_sd(1.0)
+    b.set_seed(seed)
+    b.set_init_search(3)
+    maps = {}
+    for k in (0, 2):                      # job 1 stays unseeded
+        ref, nb = jobs[k]
+        H, W = nadir.views[ref].image.shape
+        out, ow, oh = init.predict(_est(nadir, ref, nb, seed=seed + k), W, H)
+        torch.cuda.synchronize()
+        maps[k] = (out, ow, oh)
+        b.set_init_map(k, out.data_ptr(), ow, oh, nn.OUT_CHANNELS, I.SCALE)
+    b.run()
+    for k, (ref, nb) in enumerate(jobs):
+        e = _est(nadir, ref, nb, seed=seed + k)
+        e.set_patchmatch_iterations(1)
+        e.set_init_search(3)
+        if k in maps:
+            out, ow, oh = maps[k]
+            e.set_init_map(out.data_ptr(), ow, oh, nn.OUT_CHANNELS, I.SCALE)
+        want = e.compute_patch_match()
+        got = b.result(k)
+        assert all(np.array_equal(x, y) for x, y in zip(got, want)), f"job {k}"
+    acc = [scenes.depth_accuracy(b.result(k)[0], nadir.views[r].depth, nadir.min_depth,
+                                 nadir.max_depth) for k, (r, _) in enumerate(jobs)]
+    print(f"  accuracy after 1 iteration: seeded {acc[0]:.4f} {acc[2]:.4f}, random {acc[1]:.4f}",
+          flush=True)
+    assert min(acc[0], acc[2]) > acc[1]
+    with pytest.raises(IndexError):
+        b.set_init_map(7, None)
