@@ -417,6 +417,64 @@ def learned_bench(width=2000, height=1504, views=4, seed=5000):
     base = res["plain_5"]["mpix_s"]
     out["learned_1_over_plain_5"] = round(res["learned_1"]["mpix_s"] / base, 2)
     out["learned_2_over_plain_5"] = round(res["learned_2"]["mpix_s"] / base, 2)
+    out["batched"] = learned_batch_bench(init, sc, views)
+    return out
+
+
+def learned_batch_bench(init, sc, views):
+    """The same reference views as one DepthmapBatch: plain (random planes,
+    5 and 2 iterations) against learned (network input + network per view,
+    seeded batch run of 1 and 2 iterations, all inside the timed region)."""
+    import torch
+    from opensfm_amd import DepthmapBatch, initializer as I, nn, scenes
+    jobs = sc.jobs[:views]
+    npx = sum(sc.views[r].image.size for r, _ in jobs)
+    b = DepthmapBatch()
+    for v in sc.views:
+        b.add_view(v.K, v.R, v.t, v.image, v.mask)
+    for r, nb in jobs:
+        b.add_job(r, nb, sc.min_depth, sc.max_depth)
+    b.set_patch_size(7)
+    b.set_min_patch_sd(1.0)
+    b.set_seed(7000)
+    ests = [I.make_estimator(I.scene_views(sc, r, nb), sc.min_depth, sc.max_depth)
+            for r, nb in jobs]
+
+    def acc():
+        return float(np.mean([scenes.depth_accuracy(b.result(k)[0], sc.views[r].depth,
+                                                    sc.min_depth, sc.max_depth)
+                              for k, (r, _) in enumerate(jobs)]))
+
+    out = {}
+    for mode, learned, iters in (("plain_5", False, 5), ("plain_2", False, 2),
+                                 ("learned_1", True, 1), ("learned_2", True, 2)):
+        b.set_patchmatch_iterations(iters)
+        for k in range(len(jobs)):
+            b.set_init_map(k, None)
+        b.run()  # warm (and, for learned, the coarse views and the network below)
+        if learned:
+            for e, (r, _) in zip(ests, jobs):
+                H, W = sc.views[r].image.shape
+                init.predict(e, W, H)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        keep = []
+        if learned:
+            for k, (e, (r, _)) in enumerate(zip(ests, jobs)):
+                H, W = sc.views[r].image.shape
+                m, ow, oh = init.predict(e, W, H)
+                keep.append(m)
+                b.set_init_map(k, m.data_ptr(), ow, oh, nn.OUT_CHANNELS, I.SCALE)
+            torch.cuda.synchronize()
+        b.run()
+        dt = time.perf_counter() - t0
+        out[mode] = {"acc": round(acc(), 4), "ms": round(dt * 1e3, 2),
+                     "mpix_s": round(npx / dt / 1e6, 2)}
+        for k in range(len(jobs)):
+            b.set_init_map(k, None)
+        del keep
+    out["learned_1_over_plain_5"] = round(out["learned_1"]["mpix_s"] / out["plain_5"]["mpix_s"], 2)
+    out["learned_2_over_plain_5"] = round(out["learned_2"]["mpix_s"] / out["plain_5"]["mpix_s"], 2)
     return out
 
 
