@@ -44,7 +44,10 @@ extern "C" {
 
 /* sample precision of the PatchMatch scorer (the plane sweep and
  * pm_estimator_score_hypotheses are always the reference's FP32) */
-#define PM_PRECISION_FP32 0 /* u8 texels, FP32 bilinear */
+#define PM_PRECISION_FP32 0 /* FP32 samples: random-plane launches blend the
+                              u8 2x2 quads, converged launches the same
+                              quads stored as FP16 bilinear coefficients
+                              (integers <= 510, exact), FP32 arithmetic */
 #define PM_PRECISION_FP16 1 /* FP16 texels and weights, FP32 dot products */
 #define PM_PRECISION_BF16 2 /* BF16 texels and weights, FP32 dot products */
 
@@ -108,7 +111,7 @@ int pm_estimator_set_refinement_steps(pm_estimator* est, int n);
  * neighbours at distance 1 and 3, further samples keep the current depth
  * with a normal drawn within 87 degrees of the optical axis.  Scored after
  * propagation and before the refinements, only at pixels whose best score
- * is below 0.95.  They reach strongly slanted surfaces that the reference
+ * is below 0.97.  They reach strongly slanted surfaces that the reference
  * reaches by chaining refinements along its sequential sweep. */
 int pm_estimator_set_exploration(pm_estimator* est, int n);
 /* Sample precision of PatchMatch (PM_PRECISION_*, default FP32; FP16 and
