@@ -216,6 +216,30 @@ int pm_cleaner_clean(pm_cleaner* cl, float* clean_depth);
 /* HIP-event time of the last clean kernel, milliseconds. */
 int pm_cleaner_last_ms(pm_cleaner* cl, float* ms);
 
+/* Maximum number of views of pm_cleaner_clean_noalloc. */
+#define PM_NOALLOC_MAX_VIEWS 64
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same clean as pm_cleaner_clean (DepthmapCleaner::
+ * Clean's arithmetic, bit for bit) over n_views depthmaps already in device
+ * memory, written to the caller's device buffer, with no device allocation.
+ * K, R (n_views x 9, row-major) and t (n_views x 3) are host arrays as in
+ * pm_cleaner_add_view; depths[v] is a device pointer to view v's
+ * heights[v] x widths[v] f32 map; view 0 is cleaned into clean_depth
+ * (device, heights[0] x widths[0] f32).  stream is a hipStream_t (NULL: the
+ * default stream); the call returns after the clean has finished on it.
+ * The per-view table lives in a fixed device array of this library (about
+ * 12 KiB per device, held until the process ends); more than
+ * PM_NOALLOC_MAX_VIEWS views is PM_ERR_RANGE.  Every call on one device
+ * shares that array, so calls that overlap on one device race. */
+int pm_cleaner_clean_noalloc(int device, int n_views, const double* K,
+                             const double* R, const double* t,
+                             const float* const* depths,
+                             const int32_t* widths, const int32_t* heights,
+                             float same_depth_threshold,
+                             int min_consistent_views, float* clean_depth,
+                             void* stream);
+
 /* ---- GPUDepthmapCleaner ----------------------------------------------- */
 
 /* The live pipeline's cleaner (opensfm/src/lib/dense/cleaner.h:16-63,

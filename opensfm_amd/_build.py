@@ -18,7 +18,7 @@ LIB = os.path.join(LIBDIR, "libopensfm_amd.so")
 OBJDIR = os.path.join(LIBDIR, "obj")
 SOURCES = ["pm_kernels.hip", "pm_capi.cpp", "pm_batch.cpp", "nn_kernels.hip", "nn_capi.cpp",
            "fuse_kernels.hip", "fuse_capi.cpp", "covis_kernels.hip", "covis_capi.cpp",
-           "lz4_codec.cpp"]
+           "lz4_codec.cpp", "pm_noalloc.hip"]
 ARCH = os.environ.get("OPENSFM_AMD_ARCH", "gfx950")
 
 # -ffp-contract=off: the exact scorer and the host-side homography/weight

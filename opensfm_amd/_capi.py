@@ -19,6 +19,8 @@ PM_ERR_INVALID = -1
 PM_ERR_RUNTIME = -2
 PM_ERR_RANGE = -3
 
+PM_NOALLOC_MAX_VIEWS = 64  # include/opensfm_amd.h
+
 PM_MODE_PLANE_SWEEP = 0
 PM_MODE_PATCHMATCH = 1
 PM_MODE_PATCHMATCH_SAMPLE = 2
@@ -88,6 +90,8 @@ SIGNATURES = {
     "pm_cleaner_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _fp, C.c_int, C.c_int]),
     "pm_cleaner_clean": (C.c_int, [_vp, _fp]),
     "pm_cleaner_last_ms": (C.c_int, [_vp, _fp]),
+    "pm_cleaner_clean_noalloc": (C.c_int, [C.c_int, C.c_int, _dp, _dp, _dp, C.POINTER(_vp),
+                                           _ip, _ip, C.c_float, C.c_int, _vp, _vp]),
     "pm_gpu_cleaner_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pm_gpu_cleaner_destroy": (C.c_int, [_vp]),
     "pm_gpu_cleaner_set_same_depth_threshold": (C.c_int, [_vp, C.c_float]),
