@@ -120,6 +120,17 @@ int pm_estimator_set_exploration(pm_estimator* est, int n);
  * in FP32, and the NCC sums and plane/depth state stay FP32.  Not in the
  * reference (all FP32, depthmap.cc:441-478). */
 int pm_estimator_set_precision(pm_estimator* est, int precision);
+/* Median-consistent re-selection (default 1; 0 = the reference's search).
+ * Not in the reference.  After every red/black iteration but the first, and
+ * after the last, each live pixel whose depth PostProcess (depthmap.cc:
+ * 489-502) would reject -- more than 5 % from the median of its 5x5 window --
+ * re-selects its plane among the planes of that window's other pixels whose
+ * depth at this pixel passes the same test: best NCC over the views, then
+ * the reference's 6 refinements around it, each kept only while it passes
+ * the test; the new plane is kept when its NCC is at least 0.55.  Such
+ * pixels hold planes that fit their patch better than the surrounding
+ * surface does (next to occlusion edges, planes bridging both sides). */
+int pm_estimator_set_consistency(pm_estimator* est, int enable);
 /* Run PostProcess (5x5 median, depthmap.cc:489-502) after PatchMatch
  * (default 1, as the reference always does; 0 returns the raw depth). */
 int pm_estimator_set_post_process(pm_estimator* est, int enable);
@@ -464,6 +475,8 @@ int pm_batch_set_patch_size(pm_batch* b, int size);
 int pm_batch_set_min_patch_sd(pm_batch* b, float sd);
 int pm_batch_set_seed(pm_batch* b, uint64_t seed);
 int pm_batch_set_exploration(pm_batch* b, int n);
+/* pm_estimator_set_consistency for every job (default 1). */
+int pm_batch_set_consistency(pm_batch* b, int enable);
 int pm_batch_set_post_process(pm_batch* b, int enable);
 int pm_batch_set_precision(pm_batch* b, int precision);
 /* Learned initialisation per job (not in the reference; the batched form of
@@ -500,10 +513,13 @@ int pm_batch_clean(pm_batch* b, float same_depth_threshold,
                    int min_consistent_views);
 int pm_batch_download_clean(pm_batch* b, int job, float* depth);
 /* Job `job`'s result in the layout of the cluster estimator's DepthmapResult
- * (estimator.h:33-38, depthmap_bind.h:53-73): depth H*W, unit camera-frame
- * normal H*W*3 (plane / |plane|, 0 without depth), cost H*W (1 - NCC score;
- * 2 without depth), confidence H*W (1 / (1 + exp((cost - 0.5) * 5)) where
- * depth > 0, else 0; estimator.cc:1022-1040). */
+ * (estimator.h:33-38, depthmap_bind.h:53-73) with its readback rules
+ * (estimator.cc:960-993): depth H*W (0 where the plane is not finite or the
+ * depth lies outside the job's [min_depth, max_depth]), unit camera-frame
+ * normal H*W*3 (plane / |plane|, 0 without depth), cost H*W (1 - NCC score
+ * wherever finite, whatever the depth; 0 otherwise), confidence H*W
+ * (1 / (1 + exp((cost - 0.5) * 5)) where depth > 0, else 0;
+ * estimator.cc:1022-1040). */
 int pm_batch_download_cluster(pm_batch* b, int job, float* depth, float* normal,
                               float* cost, float* confidence);
 /* Last run: scorings, launches = red/black launches, total_ms = dominant_ms

@@ -72,6 +72,7 @@ SIGNATURES = {
     "pm_estimator_set_refinement_steps": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_exploration": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_precision": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_consistency": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_compute": (C.c_int, [_vp, C.c_int, _fp, _fp, _fp, _ip]),
     "pm_estimator_run": (C.c_int, [_vp, C.c_int]),
@@ -154,6 +155,7 @@ SIGNATURES = {
     "pm_batch_set_init_search": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_init_spread": (C.c_int, [_vp, C.c_float, C.c_float]),
     "pm_batch_set_exploration": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_consistency": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_tile": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_precision": (C.c_int, [_vp, C.c_int]),

@@ -106,6 +106,10 @@ class DepthmapBatch:
         _capi.check(self._lib.pm_batch_set_init_spread(self._h, float(sigma_log_depth),
                                                        float(sigma_slope)))
 
+    def set_consistency(self, enable: bool) -> None:
+        """Median-consistent re-selection for every job (default on)."""
+        _capi.check(self._lib.pm_batch_set_consistency(self._h, int(bool(enable))))
+
     def set_post_process(self, enable: bool) -> None:
         _capi.check(self._lib.pm_batch_set_post_process(self._h, int(bool(enable))))
 

@@ -58,6 +58,7 @@ struct pm_batch {
   int init_search = 1;
   float init_sigma = 0.05f, init_sigma_slope = 0.3f;
   int post_process = 1;
+  int consistency = 1;
   int tile = 1;
   int use_graph = 1;
   int precision = PM_PRECISION_FP32;
@@ -66,6 +67,7 @@ struct pm_batch {
   // device state of the prepared configuration
   DevBuf viewtab, jobtab, wlut, state, score, nghbr;
   DevBuf out_depth, out_plane, out_score, out_nghbr, queue, counter;
+  DevBuf cons_count, cons_idx, cons_med, cons_job;  // median-consistent re-selection list
   DevBuf cluster_out;  // staging of pm_batch_download_cluster
   DevBuf cl_views, cl_jobs, cl_out;
   int wlut_patch = -1;
@@ -181,6 +183,12 @@ void prepare(pm_batch* b, int count) {
   b->out_plane.ensure(px * 3 * sizeof(float));
   b->out_score.ensure(px * sizeof(float));
   b->out_nghbr.ensure(px * sizeof(int));
+  if (b->consistency) {
+    b->cons_count.ensure(sizeof(unsigned));
+    b->cons_idx.ensure(px * sizeof(unsigned));
+    b->cons_med.ensure(px * sizeof(float));
+    b->cons_job.ensure(px * sizeof(int));
+  }
   // one init queue, then 8 band queues per red/black launch
   b->queue.ensure((1 + 16 * (size_t)std::max(b->iterations, 1)) * sizeof(unsigned));
   b->counter.ensure(2 * sizeof(unsigned long long));
@@ -250,7 +258,8 @@ void prepare(pm_batch* b, int count) {
 
 // Enqueues one whole run on b->stream.  Only kernel launches (no memsets or
 // host work), so that it can be captured in a graph.  half_ev, if given,
-// receives 2 * iterations + 1 events around the red/black launches.
+// receives 4 * iterations events: one before and one after each red/black
+// launch.
 void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   const bool sample = mode == PM_MODE_PATCHMATCH_SAMPLE;
   const int nj = b->prepared_jobs;
@@ -270,16 +279,28 @@ void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   const int first_prec = b->precision == PM_PRECISION_FP32 ? pm::kPrecFp32Quads : b->precision;
   HIP_CHECK(pm::launch_pm_init_batch(jt, nj, b->pix_total, b->patch_size,
                                      first_prec, sample, q, cnt, b->stream));
-  if (half_ev) HIP_CHECK(hipEventRecord(half_ev[0], b->stream));
   int l = 0;
   for (int it = 0; it < b->iterations; ++it) {
     for (int colour = 0; colour < 2; ++colour) {
+      if (half_ev) HIP_CHECK(hipEventRecord(half_ev[2 * l], b->stream));
       HIP_CHECK(pm::launch_pm_half_batch(jt, nj, b->half_total, b->patch_size,
                                          it == 0 ? first_prec : b->precision, sample,
                                          b->tile, colour, it,
                                          q + 1 + 8 * l, cnt + 1, b->stream));
+      if (half_ev) HIP_CHECK(hipEventRecord(half_ev[2 * l + 1], b->stream));
       ++l;
-      if (half_ev) HIP_CHECK(hipEventRecord(half_ev[l], b->stream));
+    }
+    // median-consistent re-selection, as the single-view estimator does it
+    // (after every iteration but the first, and after the last)
+    if (b->consistency && (it >= 1 || it + 1 == b->iterations)) {
+      const bool last = it + 1 == b->iterations;
+      const pm::ConsistOut res{b->out_plane.as<float>(), b->out_depth.as<float>(),
+                               b->out_score.as<float>(), b->out_nghbr.as<int>(),
+                               pm::kConsistMinScore, last ? pm::kConsistFinalSupport : 0};
+      HIP_CHECK(pm::launch_pm_consist_batch(
+          jt, nj, b->pix_total, b->patch_size, b->precision, sample,
+          b->cons_count.as<unsigned>(), b->cons_idx.as<unsigned>(), b->cons_med.as<float>(),
+          b->cons_job.as<int>(), res, (unsigned)it, cnt, b->stream));
     }
   }
   HIP_CHECK(pm::launch_pack_batch(jt, nj, b->pix_total, b->post_process, b->stream));
@@ -345,9 +366,9 @@ int run(pm_batch* b, int mode) {
   return PM_OK;
 }
 
-// Times init + one red/black iteration over the first jobs (up to about
-// 8 Mpx) for one (patch, tile) candidate; milliseconds per Mpx of the red/
-// black launches.
+// Times the second red/black iteration (converged planes, the sample source
+// most of a run reads) of a 2-iteration run over the first jobs (up to about
+// 8 Mpx) for one (patch, tile) candidate; milliseconds per Mpx.
 float probe(pm_batch* b, int mode) {
   size_t px = 0;
   int count = 0;
@@ -357,15 +378,19 @@ float probe(pm_batch* b, int mode) {
     ++count;
   }
   const int iters = b->iterations;
-  b->iterations = 1;
-  std::vector<hipEvent_t> ev(3, nullptr);
+  b->iterations = 2;
+  std::vector<hipEvent_t> ev(8, nullptr);
   float ms = 0;
   try {
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
     prepare(b, count);
     enqueue(b, mode, ev.data());
-    HIP_CHECK(hipEventSynchronize(ev[2]));
-    HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[2]));
+    HIP_CHECK(hipEventSynchronize(ev[7]));
+    for (int l = 2; l < 4; ++l) {
+      float t = 0;
+      HIP_CHECK(hipEventElapsedTime(&t, ev[2 * l], ev[2 * l + 1]));
+      ms += t;
+    }
   } catch (...) {
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
@@ -546,6 +571,13 @@ int pm_batch_set_exploration(pm_batch* b, int n) {
   return PM_OK;
 }
 
+int pm_batch_set_consistency(pm_batch* b, int enable) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  b->consistency = enable ? 1 : 0;
+  b->invalidate();
+  return PM_OK;
+}
+
 int pm_batch_set_post_process(pm_batch* b, int enable) {
   if (!b) return fail(PM_ERR_INVALID, "null batch");
   b->post_process = enable ? 1 : 0;
@@ -677,16 +709,17 @@ int pm_batch_download_cluster(pm_batch* b, int job, float* depth, float* normal,
     const BatchJob& J = b->jobs[job];
     const PoolView& R = *b->views[J.ref];
     const size_t np = (size_t)R.width * R.height;
-    b->cluster_out.ensure(np * 5 * sizeof(float));
+    b->cluster_out.ensure(np * 6 * sizeof(float));
     float* nrm = b->cluster_out.as<float>();
     float* cst = nrm + 3 * np;
     float* cnf = cst + np;
+    float* dep = cnf + np;
     HIP_CHECK(pm::launch_cluster_out(b->out_depth.as<float>() + J.off,
                                      b->out_plane.as<float>() + 3 * J.off,
-                                     b->out_score.as<float>() + J.off, (int)np, nrm, cst, cnf,
+                                     b->out_score.as<float>() + J.off, (int)np,
+                                     (float)J.min_depth, (float)J.max_depth, dep, nrm, cst, cnf,
                                      b->stream));
-    HIP_CHECK(hipMemcpyAsync(depth, b->out_depth.as<float>() + J.off, np * sizeof(float),
-                             hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipMemcpyAsync(depth, dep, np * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipMemcpyAsync(normal, nrm, 3 * np * sizeof(float), hipMemcpyDeviceToHost,
                              b->stream));
     HIP_CHECK(hipMemcpyAsync(cost, cst, np * sizeof(float), hipMemcpyDeviceToHost, b->stream));

@@ -53,10 +53,12 @@ struct pm_estimator {
   int n_explore = 3;
   int precision = PM_PRECISION_FP32;
   int post_process = 1;
+  int consistency = 1;
   std::vector<std::unique_ptr<HostView>> views;
   DevBuf mask0;
   // per-compute device buffers
   DevBuf viewtab, wlut, state, score, nghbr, counter, queues;
+  DevBuf cons_count, cons_idx, cons_med;  // median-consistent re-selection list
   DevBuf out_depth, out_plane, out_score, out_nghbr;
   DevBuf sweep_H, sweep_planes;
   bool have_result = false;
@@ -213,6 +215,31 @@ pm::PmArgs prepare(pm_estimator* e) {
   return a;
 }
 
+// One median-consistent re-selection pass over the estimator's maps.
+hipError_t consistency_pass(pm_estimator* e, const pm::PmArgs& a0, bool sample, unsigned salt,
+                            bool final_pass) {
+  pm::PmArgs a = a0;
+  const size_t np = (size_t)a.W * a.H;
+  e->cons_count.ensure(sizeof(unsigned));
+  e->cons_idx.ensure(np * sizeof(unsigned));
+  e->cons_med.ensure(np * sizeof(float));
+  // the output maps hold the listed pixels' new states until the pack
+  e->out_depth.ensure(np * sizeof(float));
+  e->out_plane.ensure(np * 3 * sizeof(float));
+  e->out_score.ensure(np * sizeof(float));
+  e->out_nghbr.ensure(np * sizeof(int));
+  a.counter = e->counter.as<unsigned long long>();  // slot 0: not the dominant kernel
+  // the last pass keeps only re-selections that agree with enough of the
+  // window; earlier ones keep all, so that the following iterations can
+  // improve them
+  const pm::ConsistOut res{e->out_plane.as<float>(), e->out_depth.as<float>(),
+                           e->out_score.as<float>(), e->out_nghbr.as<int>(),
+                           pm::kConsistMinScore, final_pass ? pm::kConsistFinalSupport : 0};
+  return pm::launch_pm_consist(a, sample, e->cons_count.as<unsigned>(),
+                               e->cons_idx.as<unsigned>(), e->cons_med.as<float>(), res, salt,
+                               e->stream);
+}
+
 int run(pm_estimator* e, int mode) {
   int rc = validate_for_compute(e);
   if (rc) return rc;
@@ -274,9 +301,10 @@ int run(pm_estimator* e, int mode) {
     HIP_CHECK(pm::launch_pm_init(ai, sample, e->stream));
     pm::PmArgs ah = a;  // the red/black launches count into slot 1
     ah.counter = a.counter + 1;
-    // one event between consecutive launches; no host wait inside the loop
+    // an event before and after each red/black launch (the consistency
+    // passes run between them); no host wait inside the loop
     const int nl = 2 * e->iterations;
-    while ((int)e->half_ev.size() < nl + 1) {
+    while ((int)e->half_ev.size() < 2 * nl) {
       hipEvent_t ev;
       HIP_CHECK(hipEventCreate(&ev));
       e->half_ev.push_back(ev);
@@ -284,19 +312,24 @@ int run(pm_estimator* e, int mode) {
     // 8 band queues per red/black launch, cleared once per compute
     e->queues.ensure((size_t)8 * std::max(nl, 1) * sizeof(unsigned));
     HIP_CHECK(pm::launch_fill_u32(e->queues.as<unsigned>(), 8 * nl, 0u, e->stream));
-    HIP_CHECK(hipEventRecord(e->half_ev[0], e->stream));
     for (int it = 0; it < e->iterations; ++it) {
       ah.precision = (quads_first && it == 0) ? pm::kPrecFp32Quads : a.precision;
       for (int colour = 0; colour < 2; ++colour) {
+        HIP_CHECK(hipEventRecord(e->half_ev[2 * launches], e->stream));
         HIP_CHECK(pm::launch_pm_half(ah, sample, colour, it,
                                      e->queues.as<unsigned>() + 8 * launches, e->stream));
-        HIP_CHECK(hipEventRecord(e->half_ev[++launches], e->stream));
+        HIP_CHECK(hipEventRecord(e->half_ev[2 * launches + 1], e->stream));
+        ++launches;
       }
+      // median-consistent re-selection after every iteration but the
+      // first (whose planes are still mostly random), and after the last
+      if (e->consistency && (it >= 1 || it + 1 == e->iterations))
+        HIP_CHECK(consistency_pass(e, ah, sample, (unsigned)it, it + 1 == e->iterations));
     }
-    HIP_CHECK(hipEventSynchronize(e->half_ev[nl]));
+    if (nl > 0) HIP_CHECK(hipEventSynchronize(e->half_ev[2 * nl - 1]));
     for (int l = 0; l < nl; ++l) {
       float ms = 0;
-      HIP_CHECK(hipEventElapsedTime(&ms, e->half_ev[l], e->half_ev[l + 1]));
+      HIP_CHECK(hipEventElapsedTime(&ms, e->half_ev[2 * l], e->half_ev[2 * l + 1]));
       dom_ms += ms;
     }
   }
@@ -641,6 +674,12 @@ int pm_estimator_set_far_probes(pm_estimator* e, int n) {
   if (!e) return fail(PM_ERR_INVALID, "null estimator");
   if (n < 0 || n > 64) return fail(PM_ERR_INVALID, "far probes must be in [0, 64]");
   e->n_far = n;
+  return PM_OK;
+}
+
+int pm_estimator_set_consistency(pm_estimator* e, int enable) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  e->consistency = enable ? 1 : 0;
   return PM_OK;
 }
 

@@ -39,6 +39,18 @@ hipError_t launch_downsample_u8(const uint8_t* in, int W, int H, int sc,
 hipError_t launch_init_input(const PmArgs& a, int nplanes, float inv_min,
                              float inv_step, void* out, int ow, int oh,
                              hipStream_t s);
+// Median-consistent re-selection of a's maps, in place: count (1 unsigned)
+// and list_idx / list_med (W*H entries each) are scratch, res holds W*H
+// entries.  salt varies the refinement draws between passes.
+hipError_t launch_pm_consist(const PmArgs& a, bool sample, unsigned* count, unsigned* list_idx,
+                             float* list_med, const ConsistOut& res, unsigned salt,
+                             hipStream_t s);
+// The same over every job of a batch (list_job: W*H entries over all jobs).
+hipError_t launch_pm_consist_batch(const PmJob* jobs, int njobs, int pix_total, int patch,
+                                   int precision, bool sample, unsigned* count,
+                                   unsigned* list_idx, float* list_med, int* list_job,
+                                   const ConsistOut& res, unsigned salt,
+                                   unsigned long long* counter, hipStream_t s);
 hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
                        float* score, int* nghbr, hipStream_t s);
 // Batched launches over the jobs of a PmJob table (see pm_types.h).  `tile`
@@ -56,9 +68,11 @@ hipError_t launch_pm_half_batch(const PmJob* jobs, int njobs, int total,
                                 unsigned long long* count, hipStream_t s);
 hipError_t launch_pack_batch(const PmJob* jobs, int njobs, int total,
                              int median, hipStream_t s);
-// Cluster-estimator outputs (unit normal, cost, confidence) of n pixels.
+// Cluster-estimator outputs (depth within [dmin, dmax], unit normal, cost,
+// confidence) of n pixels.
 hipError_t launch_cluster_out(const float* depth, const float* plane, const float* score, int n,
-                              float* normal, float* cost, float* conf, hipStream_t s);
+                              float dmin, float dmax, float* out_depth, float* normal,
+                              float* cost, float* conf, hipStream_t s);
 hipError_t launch_clean_batch(const ClJob* jobs, int njobs, int total,
                               const ClView* views, float thr, int min_views,
                               hipStream_t s);

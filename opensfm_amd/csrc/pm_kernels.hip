@@ -54,6 +54,21 @@ static inline dim3 tile_grid(int tiles_x, int tiles_y) {
   return dim3(tiles_x * tiles_y);
 }
 
+// Index of the job whose tiles (first index at offset `field` of PmJob or
+// ClJob) contain tile t; t0 of job 0 is 0.
+template <typename J>
+__device__ __forceinline__ int find_job(const J* __restrict__ jobs, int njobs,
+                                        int t, int J::*first) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].*first <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+
 // ---------------------------------------------------------------------------
 // Zero-bordered quad texture from a u8 image (see pm_device.h).
 
@@ -871,6 +886,35 @@ __device__ __forceinline__ void cswap(float& x, float& y) {
   y = hi;
 }
 
+// Median of the 5x5 window of depths around (i, j) with replicate border
+// (cv::medianBlur(depth, 5) of a float map).
+__device__ __forceinline__ float median5_depth(const float4* __restrict__ st, int W, int H,
+                                               int i, int j) {
+  float win[25];
+#pragma unroll
+  for (int u = 0; u < 5; ++u) {
+    const int ii = min(max(i + u - 2, 0), H - 1);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const int jj = min(max(j + v - 2, 0), W - 1);
+      win[u * 5 + v] = st[(size_t)ii * W + jj].w;
+    }
+  }
+  // odd-even transposition sort (25 rounds) -> exact median win[12]
+#pragma unroll
+  for (int r = 0; r < 25; ++r) {
+#pragma unroll
+    for (int k = (r & 1); k + 1 < 25; k += 2) cswap(win[k], win[k + 1]);
+  }
+  return win[12];
+}
+
+// PostProcess's rejection test (depthmap.cc:497-498): true when depth d is
+// kept against window median m.
+__device__ __forceinline__ bool median_keeps(float d, float m) {
+  return d != 0.0f && !(__fdiv_rn(fabsf(d - m), d) > 0.05);
+}
+
 __device__ __forceinline__ void pack_pixel(const PmArgs& a, int median, int i,
                                            int j, float* __restrict__ out_depth,
                                            float* __restrict__ out_plane,
@@ -881,26 +925,7 @@ __device__ __forceinline__ void pack_pixel(const PmArgs& a, int median, int i,
   const float4* st = reinterpret_cast<const float4*>(a.state);
   const float4 s = st[idx];
   float d = s.w;
-  if (median) {
-    float win[25];
-#pragma unroll
-    for (int u = 0; u < 5; ++u) {
-      const int ii = min(max(i + u - 2, 0), a.H - 1);
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        const int jj = min(max(j + v - 2, 0), a.W - 1);
-        win[u * 5 + v] = st[(size_t)ii * a.W + jj].w;
-      }
-    }
-    // odd-even transposition sort (25 rounds) -> exact median win[12]
-#pragma unroll
-    for (int r = 0; r < 25; ++r) {
-#pragma unroll
-      for (int k = (r & 1); k + 1 < 25; k += 2) cswap(win[k], win[k + 1]);
-    }
-    const float m = win[12];
-    if (d == 0.0f || __fdiv_rn(fabsf(d - m), d) > 0.05) d = 0.0f;
-  }
+  if (median && !median_keeps(d, median5_depth(st, a.W, a.H, i, j))) d = 0.0f;
   out_depth[idx] = d;
   out_plane[3 * idx + 0] = s.x;
   out_plane[3 * idx + 1] = s.y;
@@ -917,6 +942,277 @@ __global__ __launch_bounds__(256) void k_pack(PmArgs a, int median,
   pack_pixel(a, median, blockIdx.y * 16 + (threadIdx.x >> 4),
              blockIdx.x * 16 + (threadIdx.x & 15), out_depth, out_plane,
              out_score, out_nghbr);
+}
+
+// ---------------------------------------------------------------------------
+// Median-consistent re-selection (not in the reference).  A live pixel whose
+// depth PostProcess would reject (more than 5 % from the median of its 5x5
+// window, depthmap.cc:489-502) re-selects its plane among the planes of that
+// window's other pixels whose depth at this pixel passes the same test
+// against the median: the best NCC wins (strict >, window order), then the
+// reference's refinement schedule around it, each step kept only while it
+// passes the test too.  Such pixels hold a plane that fits the patch better
+// than the surrounding surface does -- next to occlusion edges, a plane
+// bridging the two sides -- which PostProcess would zero.  The re-selection
+// is kept when its NCC reaches res.min_score and its depth lies within 1 %
+// of the depths of at least res.min_support of the window's other pixels; otherwise, or without a consistent candidate, the pixel keeps its
+// state (and PostProcess zeroes it).
+//
+// Three launches: k_consist_mark lists the rejected pixels (with their
+// window medians), k_consist_select computes each listed pixel's new state
+// from the unmodified maps into res_*, one lane per listed pixel, and
+// k_consist_apply writes them back, so the result does not depend on the
+// order of lanes.
+
+// Appends pixel (i, j) of the view described by `a` to the list when
+// PostProcess would reject its depth (all lanes of the wave must call this).
+__device__ __forceinline__ void consist_mark_pixel(const PmArgs& a, int i, int j, int job,
+                                                   unsigned* __restrict__ count,
+                                                   unsigned* __restrict__ list_idx,
+                                                   float* __restrict__ list_med,
+                                                   int* __restrict__ list_job) {
+  const float4* st = reinterpret_cast<const float4*>(a.state);
+  bool want = false;
+  float m = 0.0f;
+  size_t idx = 0;
+  if (i < a.H && j < a.W && is_interior(a, i, j)) {
+    idx = (size_t)i * a.W + j;
+    const float d = st[idx].w;
+    if (d != 0.0f) {
+      m = median5_depth(st, a.W, a.H, i, j);
+      want = m != 0.0f && !median_keeps(d, m);
+    }
+  }
+  const unsigned long long mask = __ballot(want);
+  if (mask == 0) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(mask));
+  base = __shfl(base, leader);
+  if (want) {
+    const unsigned k = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+    list_idx[k] = (unsigned)idx;
+    list_med[k] = m;
+    if (list_job) list_job[k] = job;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_consist_mark(PmArgs a, unsigned* __restrict__ count,
+                                                      unsigned* __restrict__ list_idx,
+                                                      float* __restrict__ list_med) {
+  consist_mark_pixel(a, blockIdx.y * 16 + (threadIdx.x >> 4), blockIdx.x * 16 + (threadIdx.x & 15),
+                     0, count, list_idx, list_med, nullptr);
+}
+
+__global__ __launch_bounds__(256) void k_consist_mark_batch(const PmJob* __restrict__ jobs,
+                                                            int njobs,
+                                                            unsigned* __restrict__ count,
+                                                            unsigned* __restrict__ list_idx,
+                                                            float* __restrict__ list_med,
+                                                            int* __restrict__ list_job) {
+  const int t = blockIdx.x;
+  const int k = find_job(jobs, njobs, t, &PmJob::pix_t0);
+  const PmJob& J = jobs[k];
+  const int lt = t - J.pix_t0;
+  consist_mark_pixel(J.a, (lt / J.pix_tx) * 16 + (threadIdx.x >> 4),
+                     (lt % J.pix_tx) * 16 + (threadIdx.x & 15), k, count, list_idx, list_med,
+                     list_job);
+}
+
+template <int P, bool kSample, int Prec>
+__device__ __forceinline__ unsigned consist_select(const PmArgs& a, size_t idx, float m,
+                                                   const ConsistOut& res, unsigned e,
+                                                   unsigned salt) {
+  const int i = (int)(idx / (size_t)a.W), j = (int)(idx % (size_t)a.W);
+  const float4* st = reinterpret_cast<const float4*>(a.state);
+  float4 cur = st[idx];
+  float best_s = a.score[idx];
+  int best_n = a.nghbr[idx];
+  unsigned tries = 0;
+  RefPatch<P> rp;
+  if (load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) {
+    const PixelRay ray = pixel_ray(a, i, j);
+    float bs = -2.0f;
+    int bn = 0;
+    float4 bp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float px[4] = {0.0f, 0.0f, 0.0f, 0.0f}, py[4] = {0.0f, 0.0f, 0.0f, 0.0f},
+          pz[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // the last scored planes
+    int nseen = 0;
+    for (int w = 0; w < 25; ++w) {
+      if (w == 12) continue;
+      const int ii = min(max(i + w / 5 - 2, 0), a.H - 1);
+      const int jj = min(max(j + w % 5 - 2, 0), a.W - 1);
+      const size_t q = (size_t)ii * a.W + jj;
+      const float4 c = st[q];
+      bool go = c.w != 0.0f;
+      const PlaneAtPixel pp = plane_at_pixel(a, ray, c.x, c.y, c.z);
+      go = go && median_keeps(__frcp_rn(fmaxf(1e-6f, -pp.gq)), m);
+      bool dup = false;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        dup |= k < nseen && px[k] == c.x && py[k] == c.y && pz[k] == c.z;
+      go = go && !dup;
+      int nb = 0;
+      if constexpr (kSample) nb = a.nghbr[q];
+      float sc = -1.0f, d2 = 0.0f;
+      if constexpr (kSample) {
+        sc = score_plane_view<P, Prec>(a, rp, ray, i, j, c.x, c.y, c.z, nb, go, &d2);
+      } else if (go) {
+        sc = score_plane_all<P, Prec>(a, rp, ray, i, j, c.x, c.y, c.z, &nb, &d2);
+      }
+      if (go) {
+        tries += kSample ? 1u : (unsigned)(a.nviews - 1);
+        const int slot = nseen & 3;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (k == slot) {
+            px[k] = c.x;
+            py[k] = c.y;
+            pz[k] = c.z;
+          }
+        }
+        ++nseen;
+        if (sc > bs) {
+          bs = sc;
+          bn = nb;
+          bp = make_float4(c.x, c.y, c.z, d2);
+        }
+      }
+    }
+    if (bs > -2.0f) {
+      // the reference's refinement schedule (depthmap.cc:333-360) around the
+      // chosen plane
+      float depth_range = 0.02f, normal_range = 0.5f;
+      for (int k = 0; k < a.n_refine; ++k) {
+        if (k > 0 && k % 6 == 0) {
+          depth_range = 0.02f;
+          normal_range = 0.5f;
+        }
+        const u4 r = philox4x32_10(a.seed_lo ^ (uint32_t)idx, a.seed_hi,
+                                   u4{0xFFFFFFFEu - salt, (uint32_t)k, 4u, 0u});
+        float g0, g1, g2, g3;
+        box_muller(r.x, r.y, &g0, &g1);
+        box_muller(r.z, r.w, &g2, &g3);
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        bool go = bp.z != 0.0f;
+        if (go) {
+          const float depth = bp.w * expf(depth_range * g0);
+          const float nx = -bp.x / bp.z + normal_range * g1;
+          const float ny = -bp.y / bp.z + normal_range * g2;
+          plane_from_depth_normal(ray, depth, nx, ny, -1.0f, v);
+          depth_range *= 0.3f;
+          normal_range *= 0.8f;
+        }
+        const PlaneAtPixel pp = plane_at_pixel(a, ray, v[0], v[1], v[2]);
+        go = go && median_keeps(__frcp_rn(fmaxf(1e-6f, -pp.gq)), m);
+        float sc = -1.0f, d2 = 0.0f;
+        int nb = bn;
+        if constexpr (kSample) {
+          sc = score_plane_view<P, Prec>(a, rp, ray, i, j, v[0], v[1], v[2], bn, go, &d2);
+        } else if (go) {
+          sc = score_plane_all<P, Prec>(a, rp, ray, i, j, v[0], v[1], v[2], &nb, &d2);
+        }
+        if (go) {
+          tries += kSample ? 1u : (unsigned)(a.nviews - 1);
+          if (sc > bs) {
+            bs = sc;
+            bn = nb;
+            bp = make_float4(v[0], v[1], v[2], d2);
+          }
+        }
+      }
+      // support: window pixels whose depth is within 1 % of the new depth
+      int support = 0;
+      for (int w = 0; w < 25; ++w) {
+        if (w == 12) continue;
+        const int ii = min(max(i + w / 5 - 2, 0), a.H - 1);
+        const int jj = min(max(j + w % 5 - 2, 0), a.W - 1);
+        const float dq = st[(size_t)ii * a.W + jj].w;
+        support += (dq > 0.0f && fabsf(dq - bp.w) < 0.01f * bp.w) ? 1 : 0;
+      }
+      if (bs >= res.min_score && support >= res.min_support) {
+        cur = bp;
+        best_s = bs;
+        best_n = bn;
+      }
+    }
+  }
+  res.plane[3 * (size_t)e] = cur.x;
+  res.plane[3 * (size_t)e + 1] = cur.y;
+  res.plane[3 * (size_t)e + 2] = cur.z;
+  res.depth[e] = cur.w;
+  res.score[e] = best_s;
+  res.nghbr[e] = best_n;
+  return tries;
+}
+
+template <int P, bool kSample, int Prec>
+__global__ __launch_bounds__(256) void k_consist_select(PmArgs a, const unsigned* __restrict__ count,
+                                                        const unsigned* __restrict__ list_idx,
+                                                        const float* __restrict__ list_med,
+                                                        ConsistOut res, unsigned salt) {
+  const unsigned n = *count;
+  unsigned tries = 0;
+  for (unsigned e = blockIdx.x * 256u + threadIdx.x; e < n; e += gridDim.x * 256u)
+    tries += consist_select<P, kSample, Prec>(a, list_idx[e], list_med[e], res, e, salt);
+  if (a.counter) wave_count(a.counter, tries);
+}
+
+template <int P, bool kSample, int Prec>
+__global__ __launch_bounds__(256) void k_consist_select_batch(
+    const PmJob* __restrict__ jobs, const unsigned* __restrict__ count,
+    const unsigned* __restrict__ list_idx, const float* __restrict__ list_med,
+    const int* __restrict__ list_job, ConsistOut res, unsigned salt,
+    unsigned long long* __restrict__ counter) {
+  // The views' texture descriptors must be wave-uniform, so the wave takes
+  // the jobs of its entries one at a time (entries of one job are mostly
+  // contiguous in the list).
+  const unsigned n = *count;
+  unsigned tries = 0;
+  for (unsigned base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    const unsigned e = base + threadIdx.x;
+    bool pend = e < n;
+    const int job = pend ? list_job[e] : -1;
+    for (;;) {
+      const unsigned long long m = __ballot(pend);
+      if (m == 0) break;
+      const int ju = __builtin_amdgcn_readfirstlane(__shfl(job, __ffsll((long long)m) - 1));
+      if (pend && job == ju) {
+        tries += consist_select<P, kSample, Prec>(jobs[ju].a, list_idx[e], list_med[e], res, e,
+                                                  salt);
+        pend = false;
+      }
+    }
+  }
+  if (counter) wave_count(counter, tries);
+}
+
+__device__ __forceinline__ void consist_apply(const PmArgs& a, unsigned idx, const ConsistOut& res,
+                                              unsigned e) {
+  reinterpret_cast<float4*>(a.state)[idx] =
+      make_float4(res.plane[3 * (size_t)e], res.plane[3 * (size_t)e + 1],
+                  res.plane[3 * (size_t)e + 2], res.depth[e]);
+  a.score[idx] = res.score[e];
+  a.nghbr[idx] = res.nghbr[e];
+}
+
+__global__ __launch_bounds__(256) void k_consist_apply(PmArgs a, const unsigned* __restrict__ count,
+                                                       const unsigned* __restrict__ list_idx,
+                                                       ConsistOut res) {
+  const unsigned n = *count;
+  for (unsigned e = blockIdx.x * 256u + threadIdx.x; e < n; e += gridDim.x * 256u)
+    consist_apply(a, list_idx[e], res, e);
+}
+
+__global__ __launch_bounds__(256) void k_consist_apply_batch(const PmJob* __restrict__ jobs,
+                                                             const unsigned* __restrict__ count,
+                                                             const unsigned* __restrict__ list_idx,
+                                                             const int* __restrict__ list_job,
+                                                             ConsistOut res) {
+  const unsigned n = *count;
+  for (unsigned e = blockIdx.x * 256u + threadIdx.x; e < n; e += gridDim.x * 256u)
+    consist_apply(jobs[list_job[e]].a, list_idx[e], res, e);
 }
 
 // ---------------------------------------------------------------------------
@@ -1019,20 +1315,6 @@ __global__ __launch_bounds__(256) void k_clean(const ClView* __restrict__ views,
 __global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = v;
-}
-
-// Index of the job whose tiles (first index at offset `field` of PmJob or
-// ClJob) contain tile t; t0 of job 0 is 0.
-template <typename J>
-__device__ __forceinline__ int find_job(const J* __restrict__ jobs, int njobs,
-                                        int t, int J::*first) {
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].*first <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
 }
 
 // Next tile of the workgroup from the queue (wave-uniform result).
@@ -1541,6 +1823,55 @@ hipError_t launch_score_hyp_fast(const PmArgs& a, int view, int n,
   return hipGetLastError();
 }
 
+hipError_t launch_pm_consist(const PmArgs& a, bool sample, unsigned* count, unsigned* list_idx,
+                             float* list_med, const ConsistOut& res, unsigned salt,
+                             hipStream_t s) {
+  const int P_ = 2 * a.hpz + 1;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, 1, 0u);
+  hipLaunchKernelGGL(k_consist_mark, grid16(a.W, a.H), dim3(256), 0, s, a, count, list_idx,
+                     list_med);
+  // a fixed grid striding over the list (its length is on the device)
+  const dim3 grid(1024);
+  if (sample) {
+    PM_DISPATCH_PREC(a.precision, P_,
+                     hipLaunchKernelGGL((k_consist_select<P, true, Prec>), grid, dim3(256), 0, s,
+                                        a, count, list_idx, list_med, res, salt));
+  } else {
+    PM_DISPATCH_PREC(a.precision, P_,
+                     hipLaunchKernelGGL((k_consist_select<P, false, Prec>), grid, dim3(256), 0, s,
+                                        a, count, list_idx, list_med, res, salt));
+  }
+  hipLaunchKernelGGL(k_consist_apply, grid, dim3(256), 0, s, a, count, list_idx, res);
+  return hipGetLastError();
+}
+
+hipError_t launch_pm_consist_batch(const PmJob* jobs, int njobs, int pix_total, int patch,
+                                   int precision, bool sample, unsigned* count,
+                                   unsigned* list_idx, float* list_med, int* list_job,
+                                   const ConsistOut& res, unsigned salt,
+                                   unsigned long long* counter, hipStream_t s) {
+  const int P_ = patch;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, 1, 0u);
+  if (pix_total <= 0) return hipGetLastError();
+  hipLaunchKernelGGL(k_consist_mark_batch, dim3(pix_total), dim3(256), 0, s, jobs, njobs, count,
+                     list_idx, list_med, list_job);
+  const dim3 grid(2048);
+  if (sample) {
+    PM_DISPATCH_PREC(precision, P_,
+                     hipLaunchKernelGGL((k_consist_select_batch<P, true, Prec>), grid, dim3(256),
+                                        0, s, jobs, count, list_idx, list_med, list_job, res,
+                                        salt, counter));
+  } else {
+    PM_DISPATCH_PREC(precision, P_,
+                     hipLaunchKernelGGL((k_consist_select_batch<P, false, Prec>), grid, dim3(256),
+                                        0, s, jobs, count, list_idx, list_med, list_job, res,
+                                        salt, counter));
+  }
+  hipLaunchKernelGGL(k_consist_apply_batch, grid, dim3(256), 0, s, jobs, count, list_idx, list_job,
+                     res);
+  return hipGetLastError();
+}
+
 hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
                        float* score, int* nghbr, hipStream_t s) {
   hipLaunchKernelGGL(k_pack, grid16(a.W, a.H), dim3(256), 0, s, a, median,
@@ -1670,35 +2001,46 @@ hipError_t launch_pack_batch(const PmJob* jobs, int njobs, int total,
 }
 
 // The cluster estimator's result layout (DepthmapResult, estimator.h:33-38)
-// from a PatchMatch result: unit normal = plane / |plane| (camera frame,
-// facing the camera), cost = 1 - score (the NCC cost, opencl_kernels.h:460;
-// 2 where there is no depth), confidence = 1 / (1 + exp((cost - 0.5) * 5))
-// where depth > 0, else 0 (estimator.cc:1022-1040).
+// from a PatchMatch result, with the readback rules of estimator.cc:960-993:
+// a pixel with a non-finite plane gets depth 0, normal 0 and cost 0; a
+// depth outside [dmin, dmax] is zeroed with its normal; every other pixel
+// keeps its depth and the unit normal plane / |plane| (camera frame, facing
+// the camera; 0 when |plane| <= 1e-6).  cost = 1 - score (the NCC cost,
+// opencl_kernels.h:460) wherever it is finite, else 0, whatever the depth;
+// confidence = 1 / (1 + exp((cost - 0.5) * 5)) where depth > 0, else 0
+// (estimator.cc:1022-1040).
 __global__ __launch_bounds__(256) void k_cluster_out(const float* __restrict__ depth,
                                                      const float* __restrict__ plane,
                                                      const float* __restrict__ score, int n,
+                                                     float dmin, float dmax,
+                                                     float* __restrict__ out_depth,
                                                      float* __restrict__ normal,
                                                      float* __restrict__ cost,
                                                      float* __restrict__ conf) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const float d = depth[i];
   const float p0 = plane[3 * (size_t)i], p1 = plane[3 * (size_t)i + 1], p2 = plane[3 * (size_t)i + 2];
+  const bool finite = __builtin_isfinite(p0) && __builtin_isfinite(p1) && __builtin_isfinite(p2);
+  float d = depth[i];
+  if (!finite || !__builtin_isfinite(d) || !(d > 0.0f && d >= dmin && d <= dmax)) d = 0.0f;
   const float nrm = sqrtf(p0 * p0 + p1 * p1 + p2 * p2);
-  const bool ok = d > 0.0f && nrm > 0.0f;
-  normal[3 * (size_t)i] = ok ? p0 / nrm : 0.0f;
-  normal[3 * (size_t)i + 1] = ok ? p1 / nrm : 0.0f;
-  normal[3 * (size_t)i + 2] = ok ? p2 / nrm : 0.0f;
-  const float c = d > 0.0f ? 1.0f - score[i] : 2.0f;
-  cost[i] = c;
-  conf[i] = d > 0.0f ? 1.0f / (1.0f + expf((c - 0.5f) * 5.0f)) : 0.0f;
+  const bool unit = d > 0.0f && nrm > 1e-6f;
+  out_depth[i] = d;
+  normal[3 * (size_t)i] = unit ? p0 / nrm : 0.0f;
+  normal[3 * (size_t)i + 1] = unit ? p1 / nrm : 0.0f;
+  normal[3 * (size_t)i + 2] = unit ? p2 / nrm : 0.0f;
+  const float c = 1.0f - score[i];
+  const float cf = finite && __builtin_isfinite(c) ? c : 0.0f;
+  cost[i] = cf;
+  conf[i] = d > 0.0f ? 1.0f / (1.0f + expf((cf - 0.5f) * 5.0f)) : 0.0f;
 }
 
 hipError_t launch_cluster_out(const float* depth, const float* plane, const float* score, int n,
-                              float* normal, float* cost, float* conf, hipStream_t s) {
+                              float dmin, float dmax, float* out_depth, float* normal,
+                              float* cost, float* conf, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_cluster_out, dim3((n + 255) / 256), dim3(256), 0, s, depth, plane, score,
-                     n, normal, cost, conf);
+                     n, dmin, dmax, out_depth, normal, cost, conf);
   return hipGetLastError();
 }
 

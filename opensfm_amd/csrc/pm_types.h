@@ -16,6 +16,14 @@ constexpr int kMaxViews = 64;  // reference view + up to 63 neighbours
 // shorter blend wins.
 constexpr int kPrecFp32Quads = 3;
 
+// Median-consistent re-selection: minimum NCC of a re-selected plane, and
+// the minimum number of the 24 other pixels of the 5x5 window whose depth
+// lies within 1 % of the re-selected depth in the pass after the last
+// iteration (measured on the 1600x1200 and 320x240 scenes: below 9 of 24
+// fewer than one re-selection in ten is within 1 % of the true depth).
+constexpr float kConsistMinScore = 0.5f;
+constexpr int kConsistFinalSupport = 10;
+
 // One view as the kernels see it.  For neighbour view v (v >= 1) the plane-
 // induced homography of plane v_pi (in the reference camera frame) is
 //   H_v = K_v (Q_v + a_v v_pi^T) K_0^-1 = M_v + b_v (K_0^-T v_pi)^T
@@ -63,6 +71,20 @@ struct PmArgs {
   int init_search;     // depth hypotheses tried at a seeded pixel (>= 1)
   float init_sigma, init_sigma_slope;  // per-pixel spread around the map
   unsigned long long* counter;  // scorings executed (nullptr: not counted)
+};
+
+// One median-consistent re-selection pass: the new states of the listed
+// pixels, one entry per listed pixel (plane, 3 floats; depth; score; view),
+// and its acceptance rule -- a re-selected plane is kept when its NCC is at
+// least min_score and its depth lies within 1 % of the depths of at least
+// min_support of the 24 other pixels of the 5x5 window.
+struct ConsistOut {
+  float* plane;
+  float* depth;
+  float* score;
+  int* nghbr;
+  float min_score;
+  int min_support;
 };
 
 // One reference view of a batch: its estimator arguments plus where its

@@ -12,7 +12,7 @@ reference's bindings do (depthmap_bind.h:57-60).
 
 Additive API: ``compute_plane_sweep`` (alias of ``compute_brute_force``),
 ``set_seed``, ``set_far_probes``, ``set_refinement_steps``,
-``set_exploration``, ``set_precision``, ``set_post_process``,
+``set_exploration``, ``set_consistency``, ``set_precision``, ``set_post_process``,
 ``score_hypotheses``, ``run``/``download``, ``stats`` and the ``device``
 constructor argument.
 """
@@ -98,6 +98,11 @@ class DepthmapEstimator:
     def set_precision(self, precision) -> None:
         """PatchMatch sample precision: "fp32" (default), "fp16" or "bf16"."""
         _capi.check(self._lib.pm_estimator_set_precision(self._h, _capi.precision_code(precision)))
+
+    def set_consistency(self, enable: bool) -> None:
+        """Median-consistent re-selection between iterations (default on;
+        off = the reference's candidate search)."""
+        _capi.check(self._lib.pm_estimator_set_consistency(self._h, int(bool(enable))))
 
     def set_post_process(self, enable: bool) -> None:
         _capi.check(self._lib.pm_estimator_set_post_process(self._h, int(bool(enable))))

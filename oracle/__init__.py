@@ -541,3 +541,30 @@ def gpu_clean(views, ref_idx, neighbor_ids, same_depth_threshold=0.01,
         consistent += ok
     keep = (d > 0) & (consistent >= eff_min) & (carved < max_carved_views)
     return np.where(keep, d, f(0)).astype(f)
+
+
+# ---------------------------------------------------------------------------
+# The cluster estimator's readback (opensfm/src/lib/dense/src/estimator.cc:
+# 960-993 and its confidence, :1022-1040), restated in numpy float32 over a
+# PatchMatch result (depth, plane, score) instead of the OpenCL plane/cost
+# buffers: cost = 1 - score.
+
+def cluster_readback(depth, plane, score, depth_min, depth_max):
+    f = np.float32
+    depth = np.asarray(depth, f)
+    plane = np.asarray(plane, f)
+    score = np.asarray(score, f)
+    finite = np.isfinite(plane).all(axis=-1)
+    d = np.where(np.isfinite(depth), depth, f(0))
+    keep = finite & (d > 0) & (d >= f(depth_min)) & (d <= f(depth_max))
+    out_depth = np.where(keep, d, f(0)).astype(f)
+    p64 = plane.astype(np.float64)
+    nrm = np.sqrt((p64 * p64).sum(-1)).astype(f)
+    unit = keep & (nrm > f(1e-6))
+    normal = np.where(unit[..., None], plane / np.where(unit, nrm, f(1))[..., None], f(0)).astype(f)
+    with np.errstate(invalid="ignore"):
+        c = (f(1) - score).astype(f)
+    cost = np.where(finite & np.isfinite(c), c, f(0)).astype(f)
+    with np.errstate(over="ignore"):
+        conf = np.where(out_depth > 0, f(1) / (f(1) + np.exp((cost - f(0.5)) * f(5))), f(0))
+    return out_depth, normal, cost, conf.astype(f)

@@ -1,12 +1,13 @@
 """GPU tests of the DepthmapClusterEstimator surface (depthmap_bind.h:19-114,
 the call sequence of opensfm/dense.py:657-731): results in begin_ref_view
-order with the reference's layout; depth equal to the batched engine's for
-the same jobs; normal, cost and confidence as their definitions
-(plane / |plane|, 1 - score, the logistic of estimator.cc:1022-1040);
-accurate depths on a rendered scene."""
+order with the reference's layout; depth, normal, cost and confidence equal
+to oracle.cluster_readback (the readback rules of estimator.cc:960-993 and
+the logistic of :1022-1040) applied to the batched engine's maps for the
+same jobs; accurate depths on a rendered scene."""
 import numpy as np
 import pytest
 
+import oracle
 from opensfm_amd import DepthmapBatch, DepthmapClusterEstimator, scenes
 
 pytestmark = pytest.mark.gpu
@@ -44,16 +45,13 @@ def test_cluster_surface(cfg1_scene):
         assert cost.shape == (h, w) and conf.shape == (h, w)
         assert all(a.dtype == np.float32 for a in res[k])
         d, plane, score, _ = b.result(k)
-        assert np.array_equal(depth, d)
-        ok = d > 0
-        nl = np.linalg.norm(plane, axis=-1, keepdims=True)
-        n = plane / np.where(nl > 0, nl, 1)
-        np.testing.assert_allclose(normal[ok], n[ok], rtol=1e-6, atol=1e-6)
+        wd, wn, wc, wk = oracle.cluster_readback(d, plane, score, sc.min_depth, sc.max_depth)
+        assert np.array_equal(depth, wd)
+        ok = depth > 0
+        np.testing.assert_allclose(normal, wn, rtol=1e-6, atol=1e-6)
         assert (normal[~ok] == 0).all()
-        assert np.array_equal(cost[ok], (np.float32(1) - score)[ok])
-        assert (cost[~ok] == 2).all()
-        want = np.float32(1) / (np.float32(1) + np.exp((cost - np.float32(0.5)) * np.float32(5)))
-        np.testing.assert_allclose(conf[ok], want[ok], rtol=3e-7, atol=1e-7)
+        assert np.array_equal(cost, wc)
+        np.testing.assert_allclose(conf, wk, rtol=3e-7, atol=1e-7)
         assert (conf[~ok] == 0).all()
         assert (normal[ok][:, 2] < 0).mean() > 0.99           # facing the camera
         acc = scenes.depth_accuracy(depth, v[r].depth, sc.min_depth, sc.max_depth)
