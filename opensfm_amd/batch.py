@@ -123,16 +123,47 @@ class DepthmapBatch:
     def set_graph(self, enable: bool) -> None:
         _capi.check(self._lib.pm_batch_set_graph(self._h, int(bool(enable))))
 
-    def autotune(self, patch_sizes=(), sample: bool = False):
-        """Times every tile (and every patch size in `patch_sizes`) on the
-        first jobs and keeps the fastest; returns (patch, tile)."""
-        ps = np.ascontiguousarray(np.asarray(list(patch_sizes), np.int32).reshape(-1))
-        p, t = C.c_int32(0), C.c_int32(0)
+    def autotune(self, patch_sizes=(), sample: bool = False, accuracy=None,
+                 tolerance: float = 0.002):
+        """Chooses the lane layout by speed and, among `patch_sizes`, the
+        patch size; returns (patch, tile).
+
+        Without `accuracy`, the fastest (patch, tile) on the first jobs'
+        converged iteration wins.  Patch size changes the maps, so a caller
+        that can measure accuracy (ground truth, or a reference run) passes
+        `accuracy(batch) -> float`: each patch size then gets its fastest
+        tile and one full run, and the fastest patch size whose accuracy is
+        within `tolerance` of the most accurate one wins.  The per-candidate
+        figures are kept in `self.tune_report`."""
         mode = _capi.PM_MODE_PATCHMATCH_SAMPLE if sample else _capi.PM_MODE_PATCHMATCH
-        _capi.check(self._lib.pm_batch_autotune(
-            self._h, mode, _capi.ptr(ps, C.c_int32) if ps.size else None, int(ps.size),
-            C.byref(p), C.byref(t)))
-        return p.value, t.value
+
+        def tune(sizes):
+            ps = np.ascontiguousarray(np.asarray(list(sizes), np.int32).reshape(-1))
+            p, t = C.c_int32(0), C.c_int32(0)
+            _capi.check(self._lib.pm_batch_autotune(
+                self._h, mode, _capi.ptr(ps, C.c_int32) if ps.size else None, int(ps.size),
+                C.byref(p), C.byref(t)))
+            return p.value, t.value
+
+        if accuracy is None or len(patch_sizes) < 2:
+            self.tune_report = None
+            return tune(patch_sizes)
+        report = []
+        for p in patch_sizes:
+            _, t = tune([p])
+            self.run(sample)            # first run of this configuration (graph capture)
+            self.run(sample)
+            report.append({"patch": int(p), "tile": int(t),
+                           "device_ms": float(self.stats()["total_ms"]),
+                           "accuracy": float(accuracy(self))})
+        best = max(r["accuracy"] for r in report)
+        ok = [r for r in report if r["accuracy"] >= best - tolerance]
+        pick = min(ok, key=lambda r: r["device_ms"])
+        self.set_patch_size(pick["patch"])
+        self.set_tile(pick["tile"])
+        self.tune_report = {"candidates": report, "tolerance": tolerance,
+                            "patch": pick["patch"], "tile": pick["tile"]}
+        return pick["patch"], pick["tile"]
 
     def run(self, sample: bool = False) -> None:
         """Runs every job; the maps stay on the device."""

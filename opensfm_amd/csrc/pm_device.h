@@ -100,18 +100,29 @@ __device__ __forceinline__ float depth_of_plane_f64(const double* Kinv,
 // accumulates Sum w x, Sum w x x and Sum w in sample order; those sums do
 // not depend on the hypothesis and are formed once.
 
+//
+// The fast scorer's values are centred instead: x relative to c, the
+// weighted mean of the reference patch (load_ref_patch<P, true>), and the
+// samples y of each hypothesis relative to its first sample, so that NCC's
+// un-centred FP32 moments (Sum w y y / Sum w - mean_y^2) do not cancel on
+// bright, nearly flat patches.  With the reference's raw values such a patch
+// can score well above 1 (1.08 measured on street facades), a rounding
+// artefact the search then prefers.  NCC is invariant to both shifts.
+
 template <int P>
 struct RefPatch {
   static constexpr int N = P * P;
   float w[N];
   float wx[N];              // w[k] * x[k], the FP32 product the reference forms
-  float sumw, meanx, varx;  // exact-order FP32
+                            // (centred: w[k] * (x[k] - c))
+  float sumw, meanx, varx;  // exact-order FP32 (centred: of x - c)
   float rsumw;              // 1 / sumw for the fast path
+  float c;                  // centre of the fast path's x values (0: raw)
 };
 
 // Bilateral weight table: wlut[|dcolor| * nr2 + (dx*dx + dy*dy)], nr2 =
 // 2*hpz*hpz + 1.
-template <int P>
+template <int P, bool kCentred = false>
 __device__ __forceinline__ bool load_ref_patch(RefPatch<P>& rp,
                                                const uint8_t* __restrict__ img,
                                                int W, int i, int j,
@@ -119,6 +130,44 @@ __device__ __forceinline__ bool load_ref_patch(RefPatch<P>& rp,
   constexpr int hpz = (P - 1) / 2;
   constexpr int nr2 = 2 * hpz * hpz + 1;
   const int c = img[i * W + j];
+  if constexpr (kCentred) {
+    float sw = 0, swx = 0;
+#pragma unroll
+    for (int dy = -hpz; dy <= hpz; ++dy) {
+#pragma unroll
+      for (int dx = -hpz; dx <= hpz; ++dx) {
+        const int k = (dy + hpz) * P + (dx + hpz);
+        const int x = img[(i + dy) * W + (j + dx)];
+        const int dc = x - c;
+        const float w = wlut[(dc < 0 ? -dc : dc) * nr2 + dx * dx + dy * dy];
+        rp.w[k] = w;
+        sw += w;
+        swx = __builtin_fmaf(w, (float)x, swx);
+      }
+    }
+    const float rsw = __frcp_rn(sw);
+    const float cen = swx * rsw;
+    float sx = 0, sxx = 0;
+#pragma unroll
+    for (int dy = -hpz; dy <= hpz; ++dy) {
+#pragma unroll
+      for (int dx = -hpz; dx <= hpz; ++dx) {
+        const int k = (dy + hpz) * P + (dx + hpz);
+        const float xs = (float)img[(i + dy) * W + (j + dx)] - cen;
+        const float wx = rp.w[k] * xs;
+        rp.wx[k] = wx;
+        sx += wx;
+        sxx = __builtin_fmaf(wx, xs, sxx);
+      }
+    }
+    rp.sumw = sw;
+    rp.rsumw = rsw;
+    rp.c = cen;
+    rp.meanx = sx * rsw;
+    rp.varx = __builtin_fmaf(-rp.meanx, rp.meanx, sxx * rsw);
+    // NCCEstimator::Get returns -1 for every hypothesis when var_x < 0.1.
+    return !(rp.varx < 0.1f);
+  }
   float sumx = 0, sumxx = 0, sumw = 0;
 #pragma unroll
   for (int dy = -hpz; dy <= hpz; ++dy) {
@@ -143,6 +192,7 @@ __device__ __forceinline__ bool load_ref_patch(RefPatch<P>& rp,
   const float meanxx = __fdiv_rn(sumxx, sumw);
   rp.meanx = meanx;
   rp.varx = meanxx - meanx * meanx;
+  rp.c = 0.0f;
   // NCCEstimator::Get returns -1 for every hypothesis when var_x < 0.1.
   return !(rp.varx < 0.1);
 }
@@ -451,7 +501,7 @@ __device__ __forceinline__ float score_fast_affine(const RefPatch<P>& rp,
   const float dfdy_x = wp.dfdy_x, dfdy_y = wp.dfdy_y;
   // One patch row of gathers in flight at a time (a two-row pipeline was
   // measured no faster and costs ~20 registers).
-  float sumy = 0, sumyy = 0, sumxy = 0;
+  float sumy = 0, sumyy = 0, sumxy = 0, shift = 0;
 #pragma unroll
   for (int dy = -hpz; dy <= hpz; ++dy) {
     const float rx = __builtin_fmaf(dfdy_x, (float)dy, Hx0);
@@ -462,7 +512,9 @@ __device__ __forceinline__ float score_fast_affine(const RefPatch<P>& rp,
       const float x2 = __builtin_fmaf(dfdx_x, (float)dx, rx);
       const float y2 = __builtin_fmaf(dfdx_y, (float)dx, ry);
       const auto t = fetch_texel(x2, y2);
-      const float y = Fetch::bilerp(t, __builtin_amdgcn_fractf(x2), __builtin_amdgcn_fractf(y2));
+      const float yr = Fetch::bilerp(t, __builtin_amdgcn_fractf(x2), __builtin_amdgcn_fractf(y2));
+      if (k == 0) shift = yr;  // the first sample centres the others
+      const float y = yr - shift;
       const float wy = rp.w[k] * y;
       sumy += wy;
       sumyy = __builtin_fmaf(wy, y, sumyy);

@@ -410,7 +410,7 @@ __global__ __launch_bounds__(256) void k_score_hyp_fast(
   const int i = ij[2 * k], j = ij[2 * k + 1];
   RefPatch<P> rp;
   float s = -1.0f;
-  if (load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) {
+  if (load_ref_patch<P, true>(rp, a.ref_img, a.W, i, j, a.wlut)) {
     const PixelRay r = pixel_ray(a, i, j);
     const PlaneAtPixel p =
         plane_at_pixel(a, r, planes[3 * k], planes[3 * k + 1], planes[3 * k + 2]);
@@ -511,7 +511,7 @@ __device__ __forceinline__ unsigned init_pixel(const PmArgs& a, int tile_x,
   float s = -1.0f;
   int nb = 0;
   unsigned count = 0;
-  const bool textured = load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut);
+  const bool textured = load_ref_patch<P, true>(rp, a.ref_img, a.W, i, j, a.wlut);
   float dtmp;
   if constexpr (kSample) {
     // one random neighbour view (uni_(rng_), depthmap.cc:247-249)
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(256) void k_init_input(PmArgs a, int nplanes,
   const bool interior = inside && is_interior(a, i, j);
   RefPatch<P> rp;
   const bool textured =
-      interior && load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut);
+      interior && load_ref_patch<P, true>(rp, a.ref_img, a.W, i, j, a.wlut);
   o[0] = (__bf16)(inside ? ((float)a.ref_img[(size_t)i * a.W + j] - 128.0f) / 64.0f : 0.0f);
   const PixelRay ray = pixel_ray(a, i, j);
   for (int d = 0; d < nplanes; ++d) {
@@ -667,7 +667,7 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   float4 cur = st[idx];
   if (cur.w == 0.0f) return 0;
   RefPatch<P> rp;
-  if (!load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) return 0;
+  if (!load_ref_patch<P, true>(rp, a.ref_img, a.W, i, j, a.wlut)) return 0;
   const PixelRay ray = pixel_ray(a, i, j);
   float best_s = a.score[idx];
   int best_n = a.nghbr[idx];
@@ -1031,7 +1031,7 @@ __device__ __forceinline__ unsigned consist_select(const PmArgs& a, size_t idx, 
   int best_n = a.nghbr[idx];
   unsigned tries = 0;
   RefPatch<P> rp;
-  if (load_ref_patch<P>(rp, a.ref_img, a.W, i, j, a.wlut)) {
+  if (load_ref_patch<P, true>(rp, a.ref_img, a.W, i, j, a.wlut)) {
     const PixelRay ray = pixel_ray(a, i, j);
     float bs = -2.0f;
     int bn = 0;

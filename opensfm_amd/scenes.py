@@ -329,6 +329,16 @@ def render(geom: SceneGeometry, K, R, t, width, height):
     return image, depth
 
 
+def _render_all(geom, cams, device=None):
+    """Renders (K, R, t, width, height) cameras: on the host with a thread
+    pool, or one after another on a torch device (scenes_gpu)."""
+    if device is not None:
+        from . import scenes_gpu
+        return [scenes_gpu.render(geom, K, R, t, w, h, device=device) for K, R, t, w, h in cams]
+    with concurrent.futures.ThreadPoolExecutor(max(1, min(len(cams), _threads()))) as ex:
+        return list(ex.map(lambda c: render(geom, *c), cams))
+
+
 def random_boxes(rng, n, extent=4.0, base_fn=None):
     boxes = []
     for _ in range(n):
@@ -372,6 +382,48 @@ def scene_config2(seed=2000, width=1600, height=1200):
     amplitude 1.5 and period 6, depth [1, 40]."""
     return make_scene(width, height, 8, 20, seed, 1.0, 40.0, terrain=(1.5, 6.0),
                       name="cfg2")
+
+
+def ring_scene(n_refs, n_neighbours, width, height, n_boxes, seed, min_depth, max_depth,
+               terrain=None, corr_px=4.0, radius=10.0, spacing_deg=8.0, elevation_deg=45.0,
+               name="", device=None):
+    """The make_scene family as a batch: cameras on an arc at `spacing_deg`
+    steps, n_refs reference views in the middle of the arc, each with its
+    n_neighbours nearest cameras along the arc (half on each side) as
+    neighbours; every camera is rendered once."""
+    rng = np.random.default_rng(seed)
+    K = intrinsics(width, height)
+    spacing = corr_px * radius / K[0, 0]
+    geom = SceneGeometry(
+        boxes=random_boxes(rng, n_boxes, base_fn=terrain),
+        ground=True, terrain=terrain, tex_spacing=spacing,
+        tex_seed=int(rng.integers(1, 2 ** 62)))
+    half = (n_neighbours + 1) // 2
+    n_cams = n_refs + 2 * half
+    el = math.radians(elevation_deg)
+    poses = []
+    for k in range(n_cams):
+        a = math.radians((k - (n_cams - 1) / 2.0) * spacing_deg)
+        c = radius * np.array([math.cos(a) * math.cos(el), math.sin(a) * math.cos(el),
+                               math.sin(el)])
+        poses.append(look_at(c))
+    jobs = []
+    for r in range(half, half + n_refs):
+        order = sorted((v for v in range(n_cams) if v != r), key=lambda v: (abs(v - r), v))
+        jobs.append((r, order[:n_neighbours]))
+    rendered = _render_all(geom, [(K, R, t, width, height) for R, t in poses], device)
+    views = [View(K=K.copy(), R=R, t=t, image=img, mask=np.full_like(img, 255), depth=dep)
+             for (R, t), (img, dep) in zip(poses, rendered)]
+    return BatchScene(views=views, jobs=jobs, min_depth=min_depth, max_depth=max_depth,
+                      name=name)
+
+
+def scene_config3(seed=3000, n_refs=16, width=2048, height=1536, neighbours=10, device=None):
+    """16 reference views of 2048x1536, 10 neighbours each, from the 1600x1200
+    scene's family (20 boxes on a sinusoidal terrain of amplitude 1.5 and
+    period 6, depth [1, 40])."""
+    return ring_scene(n_refs, neighbours, width, height, 20, seed, 1.0, 40.0,
+                      terrain=(1.5, 6.0), name="cfg3", device=device)
 
 
 def normals_from_depth(K, depth):
@@ -445,7 +497,7 @@ def street_geometry(rng, length=60.0, street_width=8.0, tex_spacing=0.06):
 
 
 def street_scene(n_views, seed=4000, sizes=STREET_SIZES, neighbours=(4, 12),
-                 min_depth=1.0, max_depth=60.0, name="street"):
+                 min_depth=1.0, max_depth=60.0, name="street", device=None):
     """Cameras along a street (1 m apart, height 1.6-2.6 m, yaw within 40
     degrees of the street axis, pitched 3-12 degrees down), each with an
     image size drawn uniformly from `sizes`.  Every camera is a job whose
@@ -470,15 +522,10 @@ def street_scene(n_views, seed=4000, sizes=STREET_SIZES, neighbours=(4, 12),
         order = np.argsort(np.abs(xs - xs[k]), kind="stable")
         jobs.append((k, [int(v) for v in order if v != k][:nn]))
 
-    def one(k):
-        w, h = dims[k]
-        K = intrinsics(w, h)
-        img, dep = render(geom, K, poses[k][0], poses[k][1], w, h)
-        return View(K=K, R=poses[k][0], t=poses[k][1], image=img,
-                    mask=np.full_like(img, 255), depth=dep)
-
-    with concurrent.futures.ThreadPoolExecutor(_threads()) as ex:
-        views = list(ex.map(one, range(n_views)))
+    cams = [(intrinsics(*dims[k]), poses[k][0], poses[k][1], dims[k][0], dims[k][1])
+            for k in range(n_views)]
+    views = [View(K=c[0], R=c[1], t=c[2], image=img, mask=np.full_like(img, 255), depth=dep)
+             for c, (img, dep) in zip(cams, _render_all(geom, cams, device))]
     return BatchScene(views=views, jobs=jobs, min_depth=min_depth,
                       max_depth=max_depth, name=name)
 
@@ -547,7 +594,8 @@ class FractalTerrain:
 
 
 def nadir_scene(rows, cols, width, height, seed=5000, neighbours=10, altitude=40.0,
-                spacing=6.0, amp=6.0, min_depth=20.0, max_depth=60.0, name="nadir"):
+                spacing=6.0, amp=6.0, min_depth=20.0, max_depth=60.0, name="nadir",
+                device=None):
     """rows x cols cameras looking straight down from `altitude` over fractal
     terrain, on a grid of `spacing` metres; every camera is a job whose
     neighbours are its `neighbours` nearest cameras."""
@@ -573,13 +621,9 @@ def nadir_scene(rows, cols, width, height, seed=5000, neighbours=10, altitude=40
         order = [int(v) for v in np.argsort(dist, kind="stable") if v != k]
         jobs.append((k, order[:neighbours]))
 
-    def one(k):
-        img, dep = render(geom, K, poses[k][0], poses[k][1], width, height)
-        return View(K=K.copy(), R=poses[k][0], t=poses[k][1], image=img,
-                    mask=np.full_like(img, 255), depth=dep)
-
-    with concurrent.futures.ThreadPoolExecutor(_threads()) as ex:
-        views = list(ex.map(one, range(n)))
+    cams = [(K, R, t, width, height) for R, t in poses]
+    views = [View(K=K.copy(), R=R, t=t, image=img, mask=np.full_like(img, 255), depth=dep)
+             for (R, t), (img, dep) in zip(poses, _render_all(geom, cams, device))]
     return BatchScene(views=views, jobs=jobs, min_depth=min_depth, max_depth=max_depth,
                       name=name)
 

@@ -11,7 +11,8 @@ pairs where both scores are defined (not -1) must agree to
   FP32 fast scorer: max 5e-4,  FP16: max 5e-3,  BF16: max 4e-2
 (measured: 1.5e-4, 1.5e-3, 1.3e-2; means 6e-6, 6.5e-5, 5.3e-4)
 and full PatchMatch at FP16/BF16 must keep accuracy and mean score within
-0.005 of the FP32 run.
+0.005 of the FP32 run.  Both checks run at 640x480 and at the mixed-precision
+configuration's size (16 reference views of 2048x1536, 10 neighbours each).
 """
 import numpy as np
 import pytest
@@ -142,3 +143,74 @@ def test_batch_precision_equals_single_view(wide_scene):
         want = e.compute_patch_match()
         got = b.result(k)
         assert all(np.array_equal(x, y) for x, y in zip(got, want)), f"job {k}"
+
+
+@pytest.fixture(scope="module")
+def config3_scene():
+    # 16 reference views of 2048x1536 with 10 neighbours each (26 cameras),
+    # rendered on the GPU
+    return scenes.scene_config3(device="cuda")
+
+
+def test_config3_scorer_error_vs_fp64(config3_scene):
+    """The error statement at the mixed-precision configuration's size: one
+    2048x1536 reference view and its 10 neighbours, 1e5 pairs, the bounds
+    above."""
+    print("scorer error vs FP64 on 1e5 pairs, 2048x1536 + 10 neighbours", flush=True)
+    ref, nb = config3_scene.jobs[0]
+    sc = scenes.Scene(views=[config3_scene.views[v] for v in [ref] + nb],
+                      min_depth=config3_scene.min_depth, max_depth=config3_scene.max_depth)
+    ij, planes, views = _pairs(sc, N_PAIRS, seed=13)
+    o = _add(oracle.OracleEstimator(), sc)
+    want = np.array([o.score_f64(int(a), int(b), p.astype(np.float64), int(v))
+                     for (a, b), p, v in zip(ij, planes, views)])
+    e = _add(DepthmapEstimator(), sc)
+    for prec in ("fp32", "fp16", "bf16"):
+        e.set_precision(prec)
+        got = e.score_hypotheses(ij, planes, views, fast=True).astype(np.float64)
+        ok = (want > -1.0) & (got > -1.0)
+        err = np.abs(got - want)[ok]
+        flips = int(((want > -1.0) != (got > -1.0)).sum())
+        print(f"{prec}: max {err.max():.3e} mean {err.mean():.3e} over {ok.sum()} pairs, "
+              f"-1 flips {flips}", flush=True)
+        assert ok.sum() > 0.3 * N_PAIRS
+        assert err.max() <= BOUNDS[prec], (prec, err.max())
+        assert flips <= 0.002 * N_PAIRS
+
+
+def test_config3_batched_precisions(config3_scene):
+    """All 16 jobs in one batch at 5 iterations in FP32, FP16 and BF16
+    samples: valid maps, repeatable runs, and FP16/BF16 accuracy and mean
+    score within 0.005 of FP32's."""
+    sc = config3_scene
+    b = DepthmapBatch()
+    for v in sc.views:
+        b.add_view(v.K, v.R, v.t, v.image, v.mask)
+    for ref, nb in sc.jobs:
+        b.add_job(ref, nb, sc.min_depth, sc.max_depth)
+    b.set_patchmatch_iterations(5)
+    b.set_min_patch_sd(1.0)
+    b.set_seed(3)
+    res = {}
+    for prec in ("fp32", "fp16", "bf16"):
+        print(f"config 3 batch, {prec}", flush=True)
+        b.set_precision(prec)
+        b.run()
+        first = [b.result(k)[0].copy() for k in range(len(sc.jobs))]
+        b.run()
+        accs, scores = [], []
+        for k, (ref, _) in enumerate(sc.jobs):
+            d, p, s, n = b.result(k)
+            assert np.array_equal(d, first[k])
+            live = d > 0
+            assert np.all(np.isfinite(d)) and live.mean() > 0.8
+            assert np.all(s[live] >= -1) and (s[live] > 1 + 1e-3).mean() < 2e-3
+            assert np.all((n[live] >= 1) & (n[live] <= 10))
+            accs.append(scenes.depth_accuracy(d, sc.views[ref].depth, sc.min_depth, sc.max_depth))
+            scores.append(float(s[live].mean()))
+        res[prec] = (float(np.mean(accs)), float(np.mean(scores)))
+        print(f"{prec}: accuracy {res[prec][0]:.4f} mean score {res[prec][1]:.4f}", flush=True)
+    assert res["fp32"][0] > 0.9
+    for prec in ("fp16", "bf16"):
+        assert res[prec][0] >= res["fp32"][0] - 0.005
+        assert res[prec][1] >= res["fp32"][1] - 0.005
