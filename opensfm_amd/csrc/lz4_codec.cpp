@@ -9,7 +9,7 @@
 #include <string.h>
 
 #include "../../include/opensfm_amd.h"
-#include "pm_host.h"
+#include "pm_math.h"
 
 using namespace pmhost;
 
@@ -58,7 +58,7 @@ int pm_lz4_block_decompress(const uint8_t* src, int64_t src_size, uint8_t* out, 
     }
     if (lit > iend - ip) return fail(PM_ERR_INVALID, "lz4: literals past the block");
     if (lit > out_cap - op) return fail(PM_ERR_INVALID, "lz4: output too small");
-    memcpy(out + op, ip, (size_t)lit);
+    if (lit > 0) memcpy(out + op, ip, (size_t)lit);
     ip += lit;
     op += lit;
     if (ip == iend) break;  // last sequence: literals only
@@ -94,6 +94,7 @@ int pm_lz4_block_compress(const uint8_t* src, int64_t n, uint8_t* dst, int64_t c
                           int64_t* written) {
   if ((!src && n) || !dst || !written || n < 0 || cap < 0)
     return fail(PM_ERR_INVALID, "lz4: bad arguments");
+  if (n > 0x7fffffff) return fail(PM_ERR_INVALID, "lz4: blocks are limited to 2^31 - 1 bytes");
   static thread_local int32_t table[1 << 16];
   for (auto& t : table) t = -1;
   uint8_t* op = dst;
@@ -108,7 +109,7 @@ int pm_lz4_block_compress(const uint8_t* src, int64_t n, uint8_t* dst, int64_t c
     *token = (uint8_t)(((lit_len >= 15 ? 15 : lit_len) << 4) | (ml >= 15 ? 15 : ml));
     if (lit_len >= 15 && !put_length(op, oend, lit_len - 15)) return false;
     if (lit_len > oend - op) return false;
-    memcpy(op, src + lit_from, (size_t)lit_len);
+    if (lit_len > 0) memcpy(op, src + lit_from, (size_t)lit_len);
     op += lit_len;
     if (mlen == 0) return true;
     if (oend - op < 2) return false;
