@@ -423,6 +423,10 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
   const float dfdy_y = __fdiv_rn(H[4] * w - H[7] * v, ww);
   const float Hx0 = __fdiv_rn(u, w);
   const float Hy0 = __fdiv_rn(v, w);
+  // A non-finite warp (a NaN or infinite plane, or an overflow) scores -1;
+  // the reference's LinearInterpolation reads all zeros for infinite
+  // coordinates (-1 again, var_y < 0.1) and is undefined for NaN ones.
+  if (!__builtin_isfinite(Hx0 + Hy0 + dfdx_x + dfdx_y + dfdy_x + dfdy_y)) return -1.0f;
 
   float cx[P], cy[P];
 #pragma unroll

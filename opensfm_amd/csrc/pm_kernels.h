@@ -55,7 +55,7 @@ hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
                        float* score, int* nghbr, hipStream_t s);
 // Batched launches over the jobs of a PmJob table (see pm_types.h).  `tile`
 // selects the red/black lane layout: 0 = 8x8, 1 = 16x16, 2 = 32x8 lanes.
-hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s);
+hipError_t launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s);
 void tile_shape(int tile, int* tx, int* ty);
 hipError_t launch_pm_init_batch(const PmJob* jobs, int njobs, int total,
                                 int patch, int precision, bool sample,

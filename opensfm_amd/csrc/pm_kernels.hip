@@ -1312,9 +1312,10 @@ __global__ __launch_bounds__(256) void k_clean(const ClView* __restrict__ views,
 // its own queue slot, cleared by k_fill_u32 at the start of a run (a kernel,
 // not a memset, so that the run can be captured in a graph).
 
-__global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = v;
+__global__ void k_fill_u32(unsigned* __restrict__ p, size_t n, unsigned v) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    p[k] = v;
 }
 
 // Next tile of the workgroup from the queue (wave-uniform result).
@@ -1827,7 +1828,7 @@ hipError_t launch_pm_consist(const PmArgs& a, bool sample, unsigned* count, unsi
                              float* list_med, const ConsistOut& res, unsigned salt,
                              hipStream_t s) {
   const int P_ = 2 * a.hpz + 1;
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, 1, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u);
   hipLaunchKernelGGL(k_consist_mark, grid16(a.W, a.H), dim3(256), 0, s, a, count, list_idx,
                      list_med);
   // a fixed grid striding over the list (its length is on the device)
@@ -1851,7 +1852,7 @@ hipError_t launch_pm_consist_batch(const PmJob* jobs, int njobs, int pix_total, 
                                    const ConsistOut& res, unsigned salt,
                                    unsigned long long* counter, hipStream_t s) {
   const int P_ = patch;
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, 1, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u);
   if (pix_total <= 0) return hipGetLastError();
   hipLaunchKernelGGL(k_consist_mark_batch, dim3(pix_total), dim3(256), 0, s, jobs, njobs, count,
                      list_idx, list_med, list_job);
@@ -1879,9 +1880,10 @@ hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
   return hipGetLastError();
 }
 
-hipError_t launch_fill_u32(unsigned* p, int n, unsigned v, hipStream_t s) {
+hipError_t launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)), dim3(256),
+                     0, s, p, n, v);
   return hipGetLastError();
 }
 

@@ -149,3 +149,80 @@ def test_strided_image_upload(ragged):
     est.set_seed(3)
     e2 = _setup(DepthmapEstimator(), views, planes=24, iters=2, seed=3)
     _equal_maps(est.compute_patch_match(), e2.compute_patch_match())
+
+
+def _pair_views(Kn, Rn, tn, size=(96, 72), seed=5):
+    """A textured reference view at the origin (R = I, t = 0) and one
+    neighbour with the given pose, both showing the same random texture."""
+    rng = np.random.default_rng(seed)
+    w, h = size
+    img = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    K = scenes.intrinsics(w, h)
+    ref = scenes.View(K=K, R=np.eye(3), t=np.zeros(3), image=img, mask=np.full_like(img, 255),
+                      depth=None)
+    nb = scenes.View(K=K if Kn is None else Kn, R=Rn, t=tn, image=img[::-1].copy(),
+                     mask=np.full_like(img, 255), depth=None)
+    return [ref, nb]
+
+
+def test_nonfinite_planes_score_minus_one():
+    """Hypotheses with NaN or infinite planes score -1 on both scorers (the
+    reference's bilinear read is undefined for NaN coordinates)."""
+    views = _pair_views(None, np.eye(3), np.array([-0.5, 0.0, 0.0]))
+    e = _setup(DepthmapEstimator(), views)
+    bad = np.array([[np.nan, 0, -0.2], [0, np.inf, -0.2], [0, 0, -np.inf], [np.nan] * 3],
+                   np.float32)
+    ij = np.array([[36, 48]] * len(bad), np.int32)
+    vw = np.ones(len(bad), np.int32)
+    assert np.all(e.score_hypotheses(ij, bad, vw) == -1)
+    assert np.all(e.score_hypotheses(ij, bad, vw, fast=True) == -1)
+
+
+def test_homography_with_w_zero():
+    """A neighbour centred at depth 4 on the reference's axis, same
+    orientation, and the fronto-parallel plane through its centre
+    (v = (0, 0, -1/4)): the homography's last row is exactly zero, w == 0 at
+    every pixel, and ComputePlaneImageScore returns -1 (depthmap.cc:451) --
+    the oracle, the exact scorer and the PatchMatch scorer alike."""
+    views = _pair_views(None, np.eye(3), np.array([0.0, 0.0, -4.0]))
+    e = _setup(DepthmapEstimator(), views)
+    o = _setup(oracle.OracleEstimator(), views)
+    ij = np.array([[10, 10], [36, 48], [60, 80]], np.int32)
+    planes = np.tile(np.array([0.0, 0.0, -0.25], np.float32), (3, 1))
+    vw = np.ones(3, np.int32)
+    want = np.array([o.score(int(i), int(j), p, 1) for (i, j), p in zip(ij, planes)])
+    assert np.all(want == -1)
+    assert np.all(e.score_hypotheses(ij, planes, vw) == -1)
+    assert np.all(e.score_hypotheses(ij, planes, vw, fast=True) == -1)
+
+
+def test_points_behind_the_neighbour():
+    """A neighbour turned round (180 degrees about x) a little behind the
+    reference: every point in front of the reference lies behind it (w < 0).
+    The reference projects them anyway; the exact scorer equals the oracle's
+    on random hypotheses, and PatchMatch with that neighbour and an ordinary
+    one keeps its invariants."""
+    Rb = np.diag([1.0, -1.0, -1.0])
+    views = _pair_views(None, Rb, np.array([0.0, 0.0, -1.0]))
+    e = _setup(DepthmapEstimator(), views)
+    o = _setup(oracle.OracleEstimator(), views)
+    rng = np.random.default_rng(8)
+    n = 400
+    ij = np.stack([rng.integers(3, 69, n), rng.integers(3, 93, n)], 1).astype(np.int32)
+    K = views[0].K
+    planes = np.stack([oracle.plane_from_depth_and_normal(
+        int(j), int(i), K, np.float32(rng.uniform(2, 20)),
+        np.array([rng.uniform(-1, 1), rng.uniform(-1, 1), -1], np.float32))
+        for i, j in ij]).astype(np.float32)
+    vw = np.ones(n, np.int32)
+    got = e.score_hypotheses(ij, planes, vw)
+    want = np.array([o.score(int(i), int(j), p, 1) for (i, j), p in zip(ij, planes)])
+    assert np.max(np.abs(got - want)) <= 1e-4
+    # PatchMatch: the turned neighbour next to an ordinary one
+    third = scenes.View(K=views[0].K, R=np.eye(3), t=np.array([-0.4, 0.0, 0.0]),
+                        image=views[0].image, mask=views[0].mask, depth=None)
+    pm = _setup(DepthmapEstimator(), views + [third], seed=3)
+    d, p, s, nb = pm.compute_patch_match()
+    live = d > 0
+    assert np.all(np.isfinite(d)) and np.all(np.isfinite(s))
+    assert np.all((nb[live] >= 1) & (nb[live] <= 2))
