@@ -525,35 +525,21 @@ def neighbors_cpu_baseline(n_points=20000, nx=25, ny=20):
             "mpts_s": round(n_points / s / 1e6, 4), "cores": 1, "kind": "port"}
 
 
-def learned_bench(width=2000, height=1504, views=4, seed=5000):
-    """Learned initialiser vs plain PatchMatch on nadir views of fractal
-    terrain (a 4 x 4 camera grid, 10 neighbours each; the first `views`
-    reference views timed): network input + network + seeded PatchMatch
-    against PatchMatch from random planes, same GPU, same settings."""
-    from opensfm_amd import initializer as I, scenes
-    log(f"learned initialiser: rendering 16 nadir views of {width}x{height}")
-    sc = scenes.nadir_scene(4, 4, width, height, seed=seed, neighbours=10)
-    jobs = [(I.scene_views(sc, r, nb), sc.min_depth, sc.max_depth, sc.views[r])
-            for r, nb in sc.jobs[:views]]
-    init = I.LearnedInitializer(weights=I.WEIGHTS)
-    res = I.evaluate(init, jobs)
-    out = {"workload": f"{views} nadir views {width}x{height}, 10 neighbours, patch 7",
-           "params": init.net.n_params}
-    out.update(res)
-    base = res["plain_5"]["mpix_s"]
-    out["learned_1_over_plain_5"] = round(res["learned_1"]["mpix_s"] / base, 2)
-    out["learned_2_over_plain_5"] = round(res["learned_2"]["mpix_s"] / base, 2)
-    out["batched"] = learned_batch_bench(init, sc, views)
-    return out
-
-
-def learned_batch_bench(init, sc, views):
-    """The same reference views as one DepthmapBatch: plain (random planes,
-    5 and 2 iterations) against learned (network input + network per view,
-    seeded batch run of 1 and 2 iterations, all inside the timed region)."""
+def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
+    """Learned initialiser configuration: 64 nadir views of 4000x3000 over
+    fractal terrain (an 8 x 8 camera grid, rendered on the GPU), 10
+    neighbours each, all in one DepthmapBatch.  Plain PatchMatch from random
+    planes at 5, 2 and 1 iterations against the learned initialiser (network
+    input from the batch's 1/4-resolution views and 32-plane cost volume,
+    the U-Net, then seeded PatchMatch) at 1 and 2 iterations, each timed once
+    from device-resident views to device-resident maps."""
     import torch
     from opensfm_amd import DepthmapBatch, initializer as I, nn, scenes
-    jobs = sc.jobs[:views]
+    log(f"learned initialiser: rendering {rows * cols} nadir views of {width}x{height} on the GPU")
+    t0 = time.perf_counter()
+    sc = scenes.nadir_scene(rows, cols, width, height, seed=seed, neighbours=10, device="cuda")
+    log(f"rendered in {time.perf_counter() - t0:.1f} s")
+    jobs = sc.jobs
     npx = sum(sc.views[r].image.size for r, _ in jobs)
     b = DepthmapBatch()
     for v in sc.views:
@@ -562,45 +548,54 @@ def learned_batch_bench(init, sc, views):
         b.add_job(r, nb, sc.min_depth, sc.max_depth)
     b.set_patch_size(7)
     b.set_min_patch_sd(1.0)
-    b.set_seed(7000)
-    ests = [I.make_estimator(I.scene_views(sc, r, nb), sc.min_depth, sc.max_depth)
-            for r, nb in jobs]
+    b.set_seed(seed)
+    b.set_graph(False)
+    init = I.LearnedInitializer(weights=I.WEIGHTS)
 
-    def acc():
+    def accuracy():
         return float(np.mean([scenes.depth_accuracy(b.result(k)[0], sc.views[r].depth,
                                                     sc.min_depth, sc.max_depth)
                               for k, (r, _) in enumerate(jobs)]))
 
-    out = {}
+    out = {"workload": f"{len(jobs)} nadir views {width}x{height}, 10 neighbours, patch 7, one "
+                       f"batch", "mpix": round(npx / 1e6, 1), "params": init.net.n_params}
+    # warm the network and the coarse views outside the timed region
+    init.predict_batch(b, 0, width, height)
     for mode, learned, iters in (("plain_5", False, 5), ("plain_2", False, 2),
-                                 ("learned_1", True, 1), ("learned_2", True, 2)):
+                                 ("plain_1", False, 1), ("learned_1", True, 1),
+                                 ("learned_2", True, 2)):
         b.set_patchmatch_iterations(iters)
-        for k in range(len(jobs)):
-            b.set_init_map(k, None)
-        b.run()  # warm (and, for learned, the coarse views and the network below)
-        if learned:
-            for e, (r, _) in zip(ests, jobs):
-                H, W = sc.views[r].image.shape
-                init.predict(e, W, H)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        keep = []
         if learned:
-            for k, (e, (r, _)) in enumerate(zip(ests, jobs)):
-                H, W = sc.views[r].image.shape
-                m, ow, oh = init.predict(e, W, H)
-                keep.append(m)
-                b.set_init_map(k, m.data_ptr(), ow, oh, nn.OUT_CHANNELS, I.SCALE)
+            for k, (r, _) in enumerate(jobs):
+                m, ow, oh = init.predict_batch(b, k, width, height)
+                b.set_init_map(k, m, ow, oh, nn.OUT_CHANNELS, I.SCALE)
             torch.cuda.synchronize()
+        t_net = time.perf_counter() - t0
         b.run()
         dt = time.perf_counter() - t0
-        out[mode] = {"acc": round(acc(), 4), "ms": round(dt * 1e3, 2),
+        out[mode] = {"acc": round(accuracy(), 4), "s": round(dt, 2),
                      "mpix_s": round(npx / dt / 1e6, 2)}
-        for k in range(len(jobs)):
-            b.set_init_map(k, None)
-        del keep
-    out["learned_1_over_plain_5"] = round(out["learned_1"]["mpix_s"] / out["plain_5"]["mpix_s"], 2)
-    out["learned_2_over_plain_5"] = round(out["learned_2"]["mpix_s"] / out["plain_5"]["mpix_s"], 2)
+        if learned:
+            out[mode]["net_s"] = round(t_net, 2)
+            for k in range(len(jobs)):
+                b.set_init_map(k, None)
+        log(f"learned initialiser {mode}: {out[mode]}")
+    base = out["plain_5"]["mpix_s"]
+    for mode in ("learned_1", "learned_2", "plain_2", "plain_1"):
+        out[f"{mode}_over_plain_5"] = round(out[mode]["mpix_s"] / base, 2)
+    # the fewest plain iterations reaching plain-5's accuracy, and the
+    # learned runs at that accuracy
+    target = out["plain_5"]["acc"]
+    eq_plain = next(m for m in ("plain_1", "plain_2", "plain_5") if out[m]["acc"] >= target)
+    eq_learned = [m for m in ("learned_1", "learned_2") if out[m]["acc"] >= target]
+    out["equal_accuracy"] = {
+        "plain_fewest_iterations": eq_plain,
+        "learned_at_equal_accuracy": eq_learned[0] if eq_learned else None,
+        "ratio_vs_plain_5": (round(out[eq_learned[0]]["mpix_s"] / base, 2) if eq_learned else None),
+        "ratio_vs_fewest_plain": (round(out[eq_learned[0]]["mpix_s"] / out[eq_plain]["mpix_s"], 2)
+                                  if eq_learned else None)}
     return out
 
 
@@ -616,8 +611,8 @@ def main():
                          "views (0: skip; 128 is the full batched configuration)")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the mixed-precision section (16 views of 2048x1536)")
-    ap.add_argument("--learned-views", type=int, default=4, metavar="N",
-                    help="also time the learned initialiser on N nadir views (0: skip)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-initialiser section (64 nadir views of 4000x3000)")
     ap.add_argument("--no-fusion", action="store_true",
                     help="skip the GPU cleaner / fusion / pruning sections")
     ap.add_argument("--no-neighbors", action="store_true",
@@ -867,8 +862,8 @@ def main():
         if not args.no_cpu_baseline:
             out["neighbors"]["cpu_baseline"] = neighbors_cpu_baseline()
         log(f"neighbors: {out['neighbors']}")
-    if args.learned_views > 0:
-        out["learned_init"] = learned_bench(views=args.learned_views)
+    if not args.no_learned:
+        out["learned_init"] = learned_bench()
         log(f"learned initialiser: {out['learned_init']}")
     print(json.dumps(out), flush=True)
 

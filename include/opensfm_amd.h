@@ -484,6 +484,11 @@ int pm_batch_set_precision(pm_batch* b, int precision);
  * same meaning and defaults): job `job` of a batch with a map equals a
  * pm_estimator given the same map, seed + job and settings.  The map must
  * stay valid through every run made while it is set; NULL clears it. */
+/* pm_estimator_initializer_input for job `job` (its reference view and
+ * neighbours from the pool, the pool's views box-averaged by `scale` once and
+ * shared by every job); the batch's patch size (5 or 7). */
+int pm_batch_initializer_input(pm_batch* b, int job, int scale, int nplanes, void* out,
+                               int out_w, int out_h);
 int pm_batch_set_init_map(pm_batch* b, int job, const float* map, int w, int h, int stride,
                           int scale);
 int pm_batch_set_init_search(pm_batch* b, int n);

@@ -151,6 +151,8 @@ SIGNATURES = {
     "pm_batch_set_patch_size": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_min_patch_sd": (C.c_int, [_vp, C.c_float]),
     "pm_batch_set_seed": (C.c_int, [_vp, C.c_uint64]),
+    "pm_batch_initializer_input": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
+                                             C.c_int]),
     "pm_batch_set_init_map": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "pm_batch_set_init_search": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_init_spread": (C.c_int, [_vp, C.c_float, C.c_float]),
@@ -262,3 +264,19 @@ def precision_code(p) -> int:
 
 def ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _device_tensor(t, n, what, dtype_name):
+    """Checks that t is a torch-like device tensor (data_ptr, numel, device,
+    dtype) of at least n elements of the given dtype; returns its address."""
+    if not (hasattr(t, "data_ptr") and hasattr(t, "numel")):
+        raise TypeError(f"{what} must be a device tensor (an object with data_ptr() and numel())")
+    dev = getattr(t, "device", None)
+    if dev is not None and getattr(dev, "type", "cuda") != "cuda":
+        raise ValueError(f"{what} must be in device memory, got {dev}")
+    dt = getattr(t, "dtype", None)
+    if dt is not None and str(dt).split(".")[-1] != dtype_name:
+        raise ValueError(f"{what} must be {dtype_name}, got {dt}")
+    if t.numel() < n:
+        raise ValueError(f"{what} holds {t.numel()} elements, needs at least {n}")
+    return t.data_ptr()

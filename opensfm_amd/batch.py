@@ -29,6 +29,7 @@ class DepthmapBatch:
         self._h = h
         self._shapes = []   # per pool view (h, w)
         self._jobs = []     # per job: ref view index
+        self._init_maps = {}  # job -> the device tensor its init map lives in
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -91,13 +92,31 @@ class DepthmapBatch:
     def set_exploration(self, n: int) -> None:
         _capi.check(self._lib.pm_batch_set_exploration(self._h, int(n)))
 
-    def set_init_map(self, job: int, map_ptr, w: int = 0, h: int = 0, stride: int = 8,
+    def set_init_map(self, job: int, init_map, w: int = 0, h: int = 0, stride: int = 8,
                      scale: int = 4) -> None:
-        """Seeds job `job`'s initialisation from a coarse (t, sx, sy, c) map in
-        device memory, as DepthmapEstimator.set_init_map (None: random)."""
+        """Seeds job `job`'s initialisation from a coarse (t, sx, sy, c) map,
+        a float32 device tensor, as DepthmapEstimator.set_init_map (None:
+        random).  The batch keeps a reference to each job's tensor until it is
+        replaced or cleared."""
+        ptr = None
+        if init_map is not None:
+            ptr = _capi._device_tensor(init_map, w * h * stride, "init map", "float32")
         _capi.check(self._lib.pm_batch_set_init_map(
-            self._h, int(job), C.c_void_p(map_ptr) if map_ptr else None, int(w), int(h),
+            self._h, int(job), C.c_void_p(ptr) if ptr else None, int(w), int(h),
             int(stride), int(scale)))
+        if init_map is None:
+            self._init_maps.pop(int(job), None)
+        else:
+            self._init_maps[int(job)] = init_map
+
+    def initializer_input(self, job: int, out, out_w: int, out_h: int, scale: int = 4,
+                          nplanes: int = 32) -> None:
+        """The learned initialiser's input of job `job` into the device
+        tensor `out` (BF16, out_h x out_w x 40), from the pool's views at
+        1/scale (as DepthmapEstimator.initializer_input)."""
+        ptr = _capi._device_tensor(out, out_w * out_h * 40, "initializer input", "bfloat16")
+        _capi.check(self._lib.pm_batch_initializer_input(
+            self._h, int(job), int(scale), int(nplanes), C.c_void_p(ptr), int(out_w), int(out_h)))
 
     def set_init_search(self, n: int) -> None:
         _capi.check(self._lib.pm_batch_set_init_search(self._h, int(n)))

@@ -33,6 +33,15 @@ struct PoolView {
   int quad16_kind = 0;  // precision they were built for (0: none)
 };
 
+// A pool view at 1/scale for the learned initialiser's input (box mean of
+// the image, K scaled about pixel centres, FP16 quads), as the estimator's
+// coarse views.
+struct CoarseView {
+  double K[9], Kinv[9];
+  int width = 0, height = 0;
+  DevBuf img, quad16;
+};
+
 struct BatchJob {
   int ref = 0;
   std::vector<int> nb;
@@ -68,6 +77,11 @@ struct pm_batch {
   DevBuf viewtab, jobtab, wlut, state, score, nghbr;
   DevBuf out_depth, out_plane, out_score, out_nghbr, queue, counter;
   DevBuf cons_count, cons_idx, cons_med, cons_job;  // median-consistent re-selection list
+  // coarse pool views of the learned initialiser's input
+  int coarse_scale = 0;
+  std::vector<std::unique_ptr<CoarseView>> coarse;
+  DevBuf coarse_viewtab, coarse_wlut;
+  int coarse_wlut_patch = -1;
   DevBuf cluster_out;  // staging of pm_batch_download_cluster
   DevBuf cl_views, cl_jobs, cl_out;
   int wlut_patch = -1;
@@ -271,9 +285,9 @@ void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   HIP_CHECK(pm::launch_fill_u32(reinterpret_cast<unsigned*>(cnt), 4, 0u, b->stream));
   // every pixel starts at the zero state (border, masked and low-variance
   // pixels keep it)
-  HIP_CHECK(pm::launch_fill_u32(b->state.as<unsigned>(), (int)(4 * b->total_px), 0u, b->stream));
-  HIP_CHECK(pm::launch_fill_u32(b->score.as<unsigned>(), (int)b->total_px, 0u, b->stream));
-  HIP_CHECK(pm::launch_fill_u32(b->nghbr.as<unsigned>(), (int)b->total_px, 0u, b->stream));
+  HIP_CHECK(pm::launch_fill_u32(b->state.as<unsigned>(), 4 * b->total_px, 0u, b->stream));
+  HIP_CHECK(pm::launch_fill_u32(b->score.as<unsigned>(), b->total_px, 0u, b->stream));
+  HIP_CHECK(pm::launch_fill_u32(b->nghbr.as<unsigned>(), b->total_px, 0u, b->stream));
   // FP32: the random planes of the initialisation and the first iteration
   // are scored from the u8 quads (pm_types.h, kPrecFp32Quads)
   const int first_prec = b->precision == PM_PRECISION_FP32 ? pm::kPrecFp32Quads : b->precision;
@@ -317,8 +331,8 @@ int run(pm_batch* b, int mode) {
     b->drop_graph();
     prepare(b, (int)b->jobs.size());
   }
-  if ((size_t)4 * b->total_px > 0x7fffffffu)
-    return fail(PM_ERR_INVALID, "batch too large: at most 536870911 reference pixels");
+  if (b->total_px > 0x7fffffffu)
+    return fail(PM_ERR_INVALID, "batch too large: at most 2147483647 reference pixels");
   b->have_result = b->have_clean = false;
   HIP_CHECK(hipEventRecord(b->ev[0], b->stream));
   if (b->use_graph) {
@@ -520,6 +534,99 @@ int pm_batch_set_min_patch_sd(pm_batch* b, float sd) {
   b->min_patch_variance = sd * sd;
   b->invalidate();
   return PM_OK;
+}
+
+int pm_batch_initializer_input(pm_batch* b, int job, int scale, int nplanes, void* out,
+                               int out_w, int out_h) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (job < 0 || job >= (int)b->jobs.size()) return fail(PM_ERR_RANGE, "job index out of range");
+  if (!out) return fail(PM_ERR_INVALID, "null output");
+  if (scale < 1 || scale > 16) return fail(PM_ERR_INVALID, "scale must be in [1, 16]");
+  if (nplanes < 2 || nplanes > 39) return fail(PM_ERR_INVALID, "nplanes must be in [2, 39]");
+  if (b->patch_size != 5 && b->patch_size != 7)
+    return fail(PM_ERR_INVALID, "the initialiser input needs patch size 5 or 7");
+  const BatchJob& J = b->jobs[job];
+  const PoolView& R0 = *b->views[J.ref];
+  const int wl = R0.width / scale, hl = R0.height / scale;
+  if (wl < b->patch_size || hl < b->patch_size)
+    return fail(PM_ERR_INVALID, "reference view too small for this scale");
+  if (out_w < wl || out_h < hl)
+    return fail(PM_ERR_INVALID, "output grid smaller than the coarse view (" +
+                                    std::to_string(wl) + "x" + std::to_string(hl) + ")");
+  return guarded([&] {
+    DeviceGuard dg(b->device);
+    if (b->coarse_scale != scale) {
+      b->coarse.clear();
+      b->coarse_scale = scale;
+    }
+    // coarse copies of the pool views, built once per scale (the estimator's
+    // build_coarse, per pool view)
+    for (size_t v = b->coarse.size(); v < b->views.size(); ++v) {
+      const PoolView& V = *b->views[v];
+      std::unique_ptr<CoarseView> c(new CoarseView());
+      memcpy(c->K, V.K, sizeof(c->K));
+      const double inv = 1.0 / scale;
+      for (int k = 0; k < 3; ++k) {
+        c->K[k] = V.K[k] * inv;
+        c->K[3 + k] = V.K[3 + k] * inv;
+      }
+      c->K[2] += (inv - 1.0) * 0.5 * V.K[8];
+      c->K[5] += (inv - 1.0) * 0.5 * V.K[8];
+      if (!inv3(c->K, c->Kinv)) throw std::runtime_error("coarse K is singular");
+      c->width = V.width / scale;
+      c->height = V.height / scale;
+      c->img.ensure((size_t)c->width * c->height);
+      HIP_CHECK(pm::launch_downsample_u8(V.img.as<uint8_t>(), V.width, V.height, scale, 0,
+                                         c->img.as<uint8_t>(), c->width, c->height, b->stream));
+      c->quad16.ensure((size_t)(c->width + 1) * (c->height + 1) * 8);
+      HIP_CHECK(pm::launch_build_quad16(c->img.as<uint8_t>(), c->quad16.as<uint32_t>(),
+                                        c->width, c->height, PM_PRECISION_FP16, b->stream));
+      b->coarse.push_back(std::move(c));
+    }
+    std::vector<int> vs(1, J.ref);
+    vs.insert(vs.end(), J.nb.begin(), J.nb.end());
+    const int n = (int)vs.size();
+    const CoarseView& c0 = *b->coarse[J.ref];
+    std::vector<pm::PmView> tab(n);
+    for (int k = 0; k < n; ++k) {
+      const PoolView& V = *b->views[vs[k]];
+      const CoarseView& C = *b->coarse[vs[k]];
+      double Q[9], a[3];
+      relative_pose(R0.R, R0.t, V.R, V.t, Q, a);
+      fill_view(tab[k], nullptr, C.width, C.height, C.K, Q, a, c0.Kinv);
+      tab[k].quad16 = C.quad16.as<uint32_t>();
+    }
+    b->coarse_viewtab.ensure(sizeof(pm::PmView) * n);
+    HIP_CHECK(hipMemcpyAsync(b->coarse_viewtab.p, tab.data(), sizeof(pm::PmView) * n,
+                             hipMemcpyHostToDevice, b->stream));
+    const int hpz = (b->patch_size - 1) / 2;
+    if (b->coarse_wlut_patch != b->patch_size) {
+      std::vector<float> t = bilateral_table(hpz);
+      b->coarse_wlut.ensure(t.size() * sizeof(float));
+      HIP_CHECK(hipMemcpyAsync(b->coarse_wlut.p, t.data(), t.size() * sizeof(float),
+                               hipMemcpyHostToDevice, b->stream));
+      b->coarse_wlut_patch = b->patch_size;
+    }
+    pm::PmArgs a;
+    memset(&a, 0, sizeof(a));
+    a.W = c0.width;
+    a.H = c0.height;
+    a.nviews = n;
+    a.hpz = hpz;
+    a.views = b->coarse_viewtab.as<pm::PmView>();
+    a.ref_img = c0.img.as<uint8_t>();
+    a.wlut = b->coarse_wlut.as<float>();
+    for (int k = 0; k < 9; ++k) {
+      a.Kinv0f[k] = (float)c0.Kinv[k];
+      a.Kinv0[k] = c0.Kinv[k];
+    }
+    a.precision = PM_PRECISION_FP16;
+    const float inv_min = (float)(1.0 / J.min_depth);
+    const float inv_step = (float)((1.0 / J.max_depth - 1.0 / J.min_depth) / (nplanes - 1));
+    HIP_CHECK(pm::launch_init_input(a, nplanes, inv_min, inv_step, out, out_w, out_h, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    return PM_OK;
+  });
 }
 
 int pm_batch_set_init_map(pm_batch* b, int job, const float* map, int w, int h, int stride,

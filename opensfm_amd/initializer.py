@@ -78,7 +78,7 @@ class LearnedInitializer:
         ow, oh = coarse_grid(width, height)
         x = torch.empty((oh * ow, nn.IN_CHANNELS), dtype=torch.bfloat16, device=self.device)
         torch.cuda.synchronize()
-        est.initializer_input(x.data_ptr(), ow, oh, SCALE, NPLANES)
+        est.initializer_input(x, ow, oh, SCALE, NPLANES)
         return x, ow, oh
 
     def predict(self, est: DepthmapEstimator, width, height):
@@ -86,13 +86,22 @@ class LearnedInitializer:
         out = self.net.forward(x, 1, oh, ow)
         return out, ow, oh
 
+    def predict_batch(self, batch, job, width, height):
+        """The network's map for job `job` of a DepthmapBatch (its input from
+        the batch's shared coarse views); returns (map, ow, oh)."""
+        ow, oh = coarse_grid(width, height)
+        x = torch.empty((oh * ow, nn.IN_CHANNELS), dtype=torch.bfloat16, device=self.device)
+        torch.cuda.synchronize()
+        batch.initializer_input(job, x, ow, oh, SCALE, NPLANES)
+        return self.net.forward(x, 1, oh, ow), ow, oh
+
     def attach(self, est: DepthmapEstimator, width, height):
         """Runs the network and hands its map to the estimator's next
         compute; returns the fraction of confident coarse cells."""
         out, ow, oh = self.predict(est, width, height)
         torch.cuda.synchronize()
         self._keep = out
-        est.set_init_map(out.data_ptr(), ow, oh, nn.OUT_CHANNELS, SCALE)
+        est.set_init_map(out, ow, oh, nn.OUT_CHANNELS, SCALE)
         wl, hl = width // SCALE, height // SCALE
         conf = out[:, 3].reshape(oh, ow)[:hl, :wl]
         return float((conf >= 0).float().mean())
@@ -118,7 +127,7 @@ def scene_views(scene, ref, nb):
              scene.views[v].mask) for v in [ref] + list(nb)]
 
 
-def training_scenes(n, width, height, seed, nadir_fraction=1 / 3):
+def training_scenes(n, width, height, seed, nadir_fraction=1 / 3, device=None):
     """Arc scenes of boxes on terrain or a ground plane (the single-view
     family) and nadir fractal-terrain scenes, as (views, min, max) jobs."""
     rng = np.random.default_rng(seed)
@@ -127,7 +136,7 @@ def training_scenes(n, width, height, seed, nadir_fraction=1 / 3):
         s = int(rng.integers(1, 2 ** 31))
         if rng.random() < nadir_fraction:
             sc = scenes.nadir_scene(2, 3, width, height, seed=s, neighbours=5,
-                                    altitude=float(rng.uniform(30, 50)))
+                                    altitude=float(rng.uniform(30, 50)), device=device)
             ref, nb = sc.jobs[int(rng.integers(0, len(sc.jobs)))]
             jobs.append((scene_views(sc, ref, nb), sc.min_depth, sc.max_depth, sc.views[ref]))
         else:
@@ -136,7 +145,7 @@ def training_scenes(n, width, height, seed, nadir_fraction=1 / 3):
             sc = scenes.make_scene(width, height, int(rng.integers(4, 9)),
                                    int(rng.integers(4, 20)), s, 1.0, 40.0, terrain=terrain,
                                    elevation_deg=float(rng.uniform(25, 60)),
-                                   spacing_deg=float(rng.uniform(5, 9)))
+                                   spacing_deg=float(rng.uniform(5, 9)), device=device)
             jobs.append((scene_views(sc, 0, range(1, len(sc.views))), sc.min_depth,
                          sc.max_depth, sc.views[0]))
     return jobs
@@ -232,14 +241,17 @@ def evaluate(init, jobs, modes=MODES):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--train", action="store_true")
-    ap.add_argument("--scenes", type=int, default=200)
+    ap.add_argument("--scenes", type=int, default=4000)
     ap.add_argument("--eval-scenes", type=int, default=12)
     ap.add_argument("--size", type=int, nargs=2, default=(320, 256))
     ap.add_argument("--eval-size", type=int, nargs=2, default=(1000, 752))
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=12000)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--seed", type=int, default=123)
     ap.add_argument("--nadir-fraction", type=float, default=0.67)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--render-device", default="cuda",
+                    help="device of the synthetic scene renderer (\"\" for the host)")
     args = ap.parse_args(argv)
 
     def log(m):
@@ -249,12 +261,20 @@ def main(argv=None):
     init = LearnedInitializer(seed=args.seed)
     if args.train:
         log(f"rendering {args.scenes} training scenes of {W}x{H}")
-        jobs = training_scenes(args.scenes, W, H, args.seed, args.nadir_fraction)
-        log("PatchMatch targets")
-        xs, ts, grid = build_samples(init, jobs)
+        t0 = time.perf_counter()
+        xs, ts, grid = [], [], None
+        # render and convert in chunks, so that only the network inputs and
+        # targets of every scene stay resident
+        for c0 in range(0, args.scenes, 200):
+            jobs = training_scenes(min(200, args.scenes - c0), W, H, args.seed + c0,
+                                   args.nadir_fraction, device=args.render_device or None)
+            x, t, grid = build_samples(init, jobs)
+            xs += x
+            ts += t
+            log(f"{len(xs)} scenes ({time.perf_counter() - t0:.1f} s)")
         log(f"training {init.net.n_params} parameters for {args.steps} steps")
         t0 = time.perf_counter()
-        hist = train(init, xs, ts, grid, steps=args.steps, log=log)
+        hist = train(init, xs, ts, grid, steps=args.steps, batch=args.batch, log=log)
         log(f"trained in {time.perf_counter() - t0:.1f} s")
         os.makedirs(os.path.dirname(args.out), exist_ok=True)
         np.savez(args.out, **init.net.state_dict())

@@ -1,8 +1,9 @@
 """The learned PatchMatch initialiser's network, on the engine's own kernels.
 
-A 4-level U-Net (channels 16, 32, 64, 128; two 3x3 convolutions with leaky
-ReLU per level; max-pool down, nearest upsampling and skip concatenation up;
-a 3x3 head with 4 outputs) whose every forward and backward step is a BF16
+A 4-level U-Net (channels 16, 32, 64, 128; 3x3 convolutions with leaky ReLU,
+two per level at full and 1/2 resolution, three at 1/4, eight at the 1/8
+bottleneck, 1.45 M parameters; max-pool down, nearest upsampling and skip
+concatenation up; a 3x3 head with 4 outputs) whose every forward and backward step is a BF16
 MFMA kernel of the engine (``pm_nn_*`` in include/opensfm_amd.h).  PyTorch
 only holds the device memory and supplies the stream: no torch operator
 computes a layer or a gradient.  Training uses the engine's Adam kernel.
@@ -23,6 +24,9 @@ from . import _capi
 IN_CHANNELS = 40
 OUT_CHANNELS = 8   # 4 used; a multiple of 8 keeps the head's input gradient aligned
 CHANNELS = (16, 32, 64, 128)
+ENC = (2, 2, 3)    # convolutions per encoder level
+DEC = (2, 2, 3)    # convolutions per decoder level (after the concatenation)
+BOTTLENECK = 8     # convolutions at 1/8 resolution
 
 
 def _p(t):
@@ -118,24 +122,31 @@ class Conv:
 
 
 class UNet:
-    """See the module docstring.  forward() keeps what backward() needs."""
+    """See the module docstring.  forward() keeps what backward() needs.
+
+    Layers: encoder level l (l = 0, 1, 2) has ENC[l] convolutions, the
+    bottleneck (1/8 resolution, 128 channels) BOTTLENECK convolutions, and
+    decoder level l DEC[l] convolutions after the skip concatenation; every
+    convolution but the head is followed by a leaky ReLU."""
 
     def __init__(self, device="cuda", seed=0):
         self.ops = Ops()
         self.device = device
         rng = np.random.default_rng(seed)
-        c0, c1, c2, c3 = CHANNELS
+        c = CHANNELS
         mk = lambda ci, co, act=True: Conv(self.ops, ci, co, device, rng, act)  # noqa: E731
-        self.layers = {
-            "e0a": mk(IN_CHANNELS, c0), "e0b": mk(c0, c0),
-            "e1a": mk(c0, c1), "e1b": mk(c1, c1),
-            "e2a": mk(c1, c2), "e2b": mk(c2, c2),
-            "ba": mk(c2, c3), "bb": mk(c3, c3),
-            "d2a": mk(c2 + c3, c2), "d2b": mk(c2, c2),
-            "d1a": mk(c1 + c2, c1), "d1b": mk(c1, c1),
-            "d0a": mk(c0 + c1, c0), "d0b": mk(c0, c0),
-            "head": mk(c0, OUT_CHANNELS, act=False),
-        }
+        self.layers = {}
+        cin = IN_CHANNELS
+        for lvl in range(3):
+            for k in range(ENC[lvl]):
+                self.layers[f"e{lvl}_{k}"] = mk(cin if k == 0 else c[lvl], c[lvl])
+            cin = c[lvl]
+        for k in range(BOTTLENECK):
+            self.layers[f"b_{k}"] = mk(c[2] if k == 0 else c[3], c[3])
+        for lvl in (2, 1, 0):
+            for k in range(DEC[lvl]):
+                self.layers[f"d{lvl}_{k}"] = mk(c[lvl] + c[lvl + 1] if k == 0 else c[lvl], c[lvl])
+        self.layers["head"] = mk(c[0], OUT_CHANNELS, act=False)
         self.step = 0
 
     @property
@@ -150,72 +161,53 @@ class UNet:
         f = torch.zeros if zero else torch.empty
         return f((n, c), dtype=torch.float32, device=self.device)
 
-    def _conv_fwd(self, name, x, N, H, W, xs, act_out, as_, ao):
-        L = self.layers[name]
-        self.ops.conv(x, False, N, H, W, L.cin, xs, L.w16, L.cout, bias=L.b, act=act_out,
-                      as_=as_, ao=ao)
+    def _chain_fwd(self, s, names, x, xs, N, H, W, npix, cout, last=None, last_s=0):
+        """Runs the convolutions `names` on x (stride xs); the last writes
+        into `last` (stride last_s) when given.  Returns (output, stride)."""
+        for k, name in enumerate(names):
+            L = self.layers[name]
+            if k == len(names) - 1 and last is not None:
+                out, os_ = last, last_s
+            else:
+                out, os_ = self._bf(npix, cout), cout
+            self.ops.conv(x, False, N, H, W, L.cin, xs, L.w16, L.cout, bias=L.b, act=out,
+                          as_=os_, ao=0)
+            s[name] = (x, xs, out, os_)
+            x, xs = out, os_
+        return x, xs
 
     def forward(self, x, N, H, W):
         """x: BF16 (N*H*W, 40) NHWC; H, W multiples of 8.  Returns FP32
         (N*H*W, 8) outputs."""
         assert H % 8 == 0 and W % 8 == 0 and x.shape == (N * H * W, IN_CHANNELS)
-        c0, c1, c2, c3 = CHANNELS
-        s = {"N": N, "H": H, "W": W, "x": x}
+        c = CHANNELS
         P = [N * H * W, N * H * W // 4, N * H * W // 16, N * H * W // 64]
         hs = [(H, W), (H // 2, W // 2), (H // 4, W // 4), (H // 8, W // 8)]
-        cat = [self._bf(P[0], c0 + c1), self._bf(P[1], c1 + c2), self._bf(P[2], c2 + c3)]
-        s["cat"] = cat
-        # encoder
-        a = self._bf(P[0], c0)
-        self._conv_fwd("e0a", x, N, H, W, IN_CHANNELS, a, c0, 0)
-        s["e0a"] = a
-        self._conv_fwd("e0b", a, N, H, W, c0, cat[0], c0 + c1, 0)
-        inp, ins = cat[0], c0 + c1
-        pooled, idxs = [], []
-        for lvl, (ca, cb, name) in enumerate([(c0, c1, "e1"), (c1, c2, "e2")]):
-            h, w = hs[lvl]
-            p = self._bf(P[lvl + 1], ca)
-            idx = torch.empty((P[lvl + 1], ca), dtype=torch.uint8, device=self.device)
-            self.ops.maxpool(inp, ins, 0, N, h, w, ca, p, idx)
-            pooled.append(p)
-            idxs.append(idx)
-            h2, w2 = hs[lvl + 1]
-            a = self._bf(P[lvl + 1], cb)
-            self._conv_fwd(name + "a", p, N, h2, w2, ca, a, cb, 0)
-            s[name + "a"] = a
-            nxt = cat[lvl + 1]
-            self._conv_fwd(name + "b", a, N, h2, w2, cb, nxt, nxt.shape[1], 0)
-            inp, ins = nxt, nxt.shape[1]
-        h, w = hs[2]
-        p = self._bf(P[3], c2)
-        idx = torch.empty((P[3], c2), dtype=torch.uint8, device=self.device)
-        self.ops.maxpool(inp, ins, 0, N, h, w, c2, p, idx)
-        pooled.append(p)
-        idxs.append(idx)
-        s["pooled"], s["idx"] = pooled, idxs
-        h3, w3 = hs[3]
-        ba = self._bf(P[3], c3)
-        self._conv_fwd("ba", p, N, h3, w3, c2, ba, c3, 0)
-        bb = self._bf(P[3], c3)
-        self._conv_fwd("bb", ba, N, h3, w3, c3, bb, c3, 0)
-        s["ba"], s["bb"] = ba, bb
-        # decoder
-        low, lc = bb, c3
-        for lvl, (cs, cd, name) in [(2, (c2, c2, "d2")), (1, (c1, c1, "d1")), (0, (c0, c0, "d0"))]:
-            h, w = hs[lvl]
-            hl, wl = hs[lvl + 1]
-            cc = cat[lvl].shape[1]
-            self.ops.upsample(low, N, hl, wl, lc, cat[lvl], cc, cs)
-            a = self._bf(P[lvl], cd)
-            self._conv_fwd(name + "a", cat[lvl], N, h, w, cc, a, cd, 0)
-            b = self._bf(P[lvl], cd)
-            self._conv_fwd(name + "b", a, N, h, w, cd, b, cd, 0)
-            s[name + "a"], s[name + "b"] = a, b
-            low, lc = b, cd
+        s = {"N": N, "H": H, "W": W}
+        cat = [self._bf(P[l], c[l] + c[l + 1]) for l in range(3)]
+        s["cat"], s["idx"] = cat, []
+        inp, ins = x, IN_CHANNELS
+        for lvl in range(3):
+            names = [f"e{lvl}_{k}" for k in range(ENC[lvl])]
+            self._chain_fwd(s, names, inp, ins, N, *hs[lvl], P[lvl], c[lvl], cat[lvl],
+                            c[lvl] + c[lvl + 1])
+            p = self._bf(P[lvl + 1], c[lvl])
+            idx = torch.empty((P[lvl + 1], c[lvl]), dtype=torch.uint8, device=self.device)
+            self.ops.maxpool(cat[lvl], c[lvl] + c[lvl + 1], 0, N, *hs[lvl], c[lvl], p, idx)
+            s["idx"].append(idx)
+            inp, ins = p, c[lvl]
+        low, lc = self._chain_fwd(s, [f"b_{k}" for k in range(BOTTLENECK)], inp, ins, N,
+                                  *hs[3], P[3], c[3])
+        for lvl in (2, 1, 0):
+            cc = c[lvl] + c[lvl + 1]
+            self.ops.upsample(low, N, *hs[lvl + 1], lc, cat[lvl], cc, c[lvl])
+            low, lc = self._chain_fwd(s, [f"d{lvl}_{k}" for k in range(DEC[lvl])], cat[lvl], cc,
+                                      N, *hs[lvl], P[lvl], c[lvl])
         out = self._f(P[0], OUT_CHANNELS)
         L = self.layers["head"]
-        self.ops.conv(low, False, N, H, W, c0, c0, L.w16, L.cout, bias=L.b, y=out, y_f32=True,
+        self.ops.conv(low, False, N, H, W, c[0], lc, L.w16, L.cout, bias=L.b, y=out, y_f32=True,
                       ys=OUT_CHANNELS, yo=0)
+        s["head"] = (low, lc, out, OUT_CHANNELS)
         s["out"] = out
         self._saved = s
         return out
@@ -235,56 +227,45 @@ class UNet:
                       ys=dxs, yo=0)
         return dx
 
-    def _act_bwd(self, da, das, dao, a, as_, ao, npix, c):
-        dz = self._f(npix, c)
-        self.ops.lrelu_backward(da, das, dao, a, as_, ao, dz, npix, c)
-        return dz
+    def _chain_bwd(self, names, d, ds, N, H, W, npix, need_first_dx=True):
+        """Back through the activated convolutions `names` given the
+        gradient d (stride ds) of the last one's output; returns the
+        gradient of the chain's input (FP32, contiguous)."""
+        s = self._saved
+        for k in range(len(names) - 1, -1, -1):
+            x, xs, a, as_ = s[names[k]]
+            L = self.layers[names[k]]
+            dz = self._f(npix, L.cout)
+            self.ops.lrelu_backward(d, ds, 0, a, as_, 0, dz, npix, L.cout)
+            d = self._conv_bwd(names[k], x, xs, N, H, W, dz, need_dx=k > 0 or need_first_dx)
+            ds = L.cin
+        return d
 
     def backward(self, dout):
         """Accumulates parameter gradients for dL/dout (FP32 (P0, 8))."""
         s = self._saved
         N, H, W = s["N"], s["H"], s["W"]
-        c0, c1, c2, c3 = CHANNELS
+        c = CHANNELS
         P = [N * H * W, N * H * W // 4, N * H * W // 16, N * H * W // 64]
         hs = [(H, W), (H // 2, W // 2), (H // 4, W // 4), (H // 8, W // 8)]
         cat = s["cat"]
-        # head (no activation)
-        d = self._conv_bwd("head", s["d0b"], c0, N, H, W, dout)
+        x, xs, _, _ = s["head"]
+        d = self._conv_bwd("head", x, xs, N, H, W, dout)
         dcat = [None, None, None]
-        for lvl, name, cd, cs in [(0, "d0", c0, c0), (1, "d1", c1, c1), (2, "d2", c2, c2)]:
-            h, w = hs[lvl]
-            cc = cat[lvl].shape[1]
-            dz = self._act_bwd(d, cd, 0, s[name + "b"], cd, 0, P[lvl], cd)
-            d = self._conv_bwd(name + "b", s[name + "a"], cd, N, h, w, dz)
-            dz = self._act_bwd(d, cd, 0, s[name + "a"], cd, 0, P[lvl], cd)
-            dcat[lvl] = self._conv_bwd(name + "a", cat[lvl], cc, N, h, w, dz)
-            # gradient of the upsampled part -> the next-lower decoder output
-            hl, wl = hs[lvl + 1]
-            lc = c3 if lvl == 2 else (c2 if lvl == 1 else c1)
-            d = self._f(P[lvl + 1], lc)
-            self.ops.upsample_backward(dcat[lvl], cc, cs, N, hl, wl, lc, d)
-        # d is now the gradient of bb (bottleneck output)
-        h3, w3 = hs[3]
-        dz = self._act_bwd(d, c3, 0, s["bb"], c3, 0, P[3], c3)
-        d = self._conv_bwd("bb", s["ba"], c3, N, h3, w3, dz)
-        dz = self._act_bwd(d, c3, 0, s["ba"], c3, 0, P[3], c3)
-        d = self._conv_bwd("ba", s["pooled"][2], c2, N, h3, w3, dz)
+        for lvl in (0, 1, 2):
+            cc = c[lvl] + c[lvl + 1]
+            dcat[lvl] = self._chain_bwd([f"d{lvl}_{k}" for k in range(DEC[lvl])], d, c[lvl], N,
+                                        *hs[lvl], P[lvl])
+            # gradient of the upsampled part -> the next-lower output
+            d = self._f(P[lvl + 1], c[lvl + 1])
+            self.ops.upsample_backward(dcat[lvl], cc, c[lvl], N, *hs[lvl + 1], c[lvl + 1], d)
+        d = self._chain_bwd([f"b_{k}" for k in range(BOTTLENECK)], d, c[3], N, *hs[3], P[3])
         # encoder, deepest first: pool backward adds into the skip part of dcat
-        for lvl, name, ca, cb in [(2, "e2", c1, c2), (1, "e1", c0, c1)]:
-            h, w = hs[lvl]
-            cc = cat[lvl].shape[1]
-            self.ops.maxpool_backward(d, s["idx"][lvl], N, h, w, cb, dcat[lvl], cc, 0)
-            dz = self._act_bwd(dcat[lvl], cc, 0, cat[lvl], cc, 0, P[lvl], cb)
-            d = self._conv_bwd(name + "b", s[name + "a"], cb, N, h, w, dz)
-            dz = self._act_bwd(d, cb, 0, s[name + "a"], cb, 0, P[lvl], cb)
-            d = self._conv_bwd(name + "a", s["pooled"][lvl - 1], ca, N, h, w, dz)
-        h, w = hs[0]
-        cc = cat[0].shape[1]
-        self.ops.maxpool_backward(d, s["idx"][0], N, h, w, c0, dcat[0], cc, 0)
-        dz = self._act_bwd(dcat[0], cc, 0, cat[0], cc, 0, P[0], c0)
-        d = self._conv_bwd("e0b", s["e0a"], c0, N, h, w, dz)
-        dz = self._act_bwd(d, c0, 0, s["e0a"], c0, 0, P[0], c0)
-        self._conv_bwd("e0a", s["x"], IN_CHANNELS, N, h, w, dz, need_dx=False)
+        for lvl in (2, 1, 0):
+            cc = c[lvl] + c[lvl + 1]
+            self.ops.maxpool_backward(d, s["idx"][lvl], N, *hs[lvl], c[lvl], dcat[lvl], cc, 0)
+            d = self._chain_bwd([f"e{lvl}_{k}" for k in range(ENC[lvl])], dcat[lvl], cc, N,
+                                *hs[lvl], P[lvl], need_first_dx=lvl > 0)
 
     def zero_grad(self):
         for L in self.layers.values():

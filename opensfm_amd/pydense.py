@@ -41,6 +41,7 @@ class DepthmapEstimator:
         _capi.check(self._lib.pm_estimator_create(int(device), C.byref(h)))
         self._h = h
         self._shape = None
+        self._init_map = None  # the device tensor the init map lives in
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -159,20 +160,26 @@ class DepthmapEstimator:
                        _capi.ptr(views, C.c_int32), _capi.ptr(out, C.c_float)))
         return out
 
-    def initializer_input(self, out_ptr: int, out_w: int, out_h: int, scale: int = 4,
+    def initializer_input(self, out, out_w: int, out_h: int, scale: int = 4,
                           nplanes: int = 32) -> None:
-        """Writes the learned initialiser's input (BF16 NHWC, 40 channels) to
-        device memory at out_ptr."""
+        """Writes the learned initialiser's input (BF16 NHWC, 40 channels,
+        out_h x out_w cells) into the device tensor `out`."""
+        ptr = _capi._device_tensor(out, out_w * out_h * 40, "initializer input", "bfloat16")
         _capi.check(self._lib.pm_estimator_initializer_input(
-            self._h, int(scale), int(nplanes), C.c_void_p(out_ptr), int(out_w), int(out_h)))
+            self._h, int(scale), int(nplanes), C.c_void_p(ptr), int(out_w), int(out_h)))
 
-    def set_init_map(self, map_ptr, w: int = 0, h: int = 0, stride: int = 8,
+    def set_init_map(self, init_map, w: int = 0, h: int = 0, stride: int = 8,
                      scale: int = 4) -> None:
-        """Seeds PatchMatch's initialisation from a coarse (t, sx, sy, c) map
-        in device memory (None: random initialisation)."""
+        """Seeds PatchMatch's initialisation from a coarse (t, sx, sy, c) map:
+        a float32 device tensor of h x w cells of `stride` values (None:
+        random initialisation).  The estimator keeps a reference to the
+        tensor until the map is replaced or cleared."""
+        ptr = None
+        if init_map is not None:
+            ptr = _capi._device_tensor(init_map, w * h * stride, "init map", "float32")
         _capi.check(self._lib.pm_estimator_set_init_map(
-            self._h, C.c_void_p(map_ptr) if map_ptr else None, int(w), int(h), int(stride),
-            int(scale)))
+            self._h, C.c_void_p(ptr) if ptr else None, int(w), int(h), int(stride), int(scale)))
+        self._init_map = init_map
 
     def set_init_search(self, n: int) -> None:
         """Depth hypotheses tried around the init map's depth (1..15)."""

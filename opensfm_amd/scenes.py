@@ -353,7 +353,7 @@ def random_boxes(rng, n, extent=4.0, base_fn=None):
 
 def make_scene(width, height, n_neighbours, n_boxes, seed, min_depth, max_depth,
                terrain=None, corr_px=4.0, radius=10.0, spacing_deg=8.0,
-               elevation_deg=45.0, name=""):
+               elevation_deg=45.0, name="", device=None):
     """Arc-camera scene of a ground plane (or sinusoidal terrain) and random
     axis-aligned boxes, textured with value noise of `corr_px` correlation."""
     rng = np.random.default_rng(seed)
@@ -364,9 +364,7 @@ def make_scene(width, height, n_neighbours, n_boxes, seed, min_depth, max_depth,
         ground=True, terrain=terrain, tex_spacing=spacing,
         tex_seed=int(rng.integers(1, 2 ** 62)))
     poses = arc_cameras(n_neighbours, radius, spacing_deg, elevation_deg)
-    with concurrent.futures.ThreadPoolExecutor(min(len(poses), _threads())) as ex:
-        rendered = list(ex.map(lambda p: render(geom, K, p[0], p[1], width, height),
-                               poses))
+    rendered = _render_all(geom, [(K, R, t, width, height) for R, t in poses], device)
     views = [View(K=K.copy(), R=R, t=t, image=img, mask=np.full_like(img, 255),
                   depth=dep) for (R, t), (img, dep) in zip(poses, rendered)]
     return Scene(views=views, min_depth=min_depth, max_depth=max_depth, name=name)
