@@ -214,8 +214,22 @@ __device__ __forceinline__ float sqrt_rn(float x) {
 }
 
 // ---------------------------------------------------------------------------
-// Texel gathers through a buffer resource: 32-bit byte offsets (one VGPR per
-// address), hardware range check as a second line of defence.
+// Texel gathers through a structured buffer resource: the record index (one
+// VGPR per address) is scaled by the texel size in the address unit, and
+// records outside [0, num_records) read as zero (the hardware range check, a
+// second line of defence behind the coordinate clamps).  Against a byte
+// offset this saves the shift of every gather's address (measured 0.7 % on
+// the red/black kernel).
+
+typedef unsigned int pm_u32x2 __attribute__((ext_vector_type(2)));
+// The LLVM buffer intrinsics with an index operand (buffer_load ... idxen),
+// for which clang has no builtin.
+__device__ unsigned int pm_struct_load_b32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
+                                           int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.i32");
+__device__ pm_u32x2 pm_struct_load_b64(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
+                                       int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
 
 struct QuadTex {
   __amdgpu_buffer_rsrc_t rsrc;  // base = texel (0, 0)
@@ -250,7 +264,7 @@ __device__ __forceinline__ QuadTex make_tex(const uint32_t* quad, int W, int H) 
   // -(pitch + 1), an out-of-range record that reads 0 like the border
   const int records = t.pitch * (H + 1) - (t.pitch + 1);
   t.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(quad + t.pitch + 1),
-                                             (short)0, records * 4, 0x00020000);
+                                             (short)4, records, 0x00020000);
   t.xmax = (float)(W - 1);
   t.ymax = (float)(H - 1);
   return t;
@@ -265,7 +279,7 @@ __device__ __forceinline__ uint32_t fetch_quad(const QuadTex& t, float flx,
   const int ix = (int)__builtin_amdgcn_fmed3f(flx, -1.0f, t.xmax);
   const int iy = (int)__builtin_amdgcn_fmed3f(fly, -1.0f, t.ymax);
   const int rec = __mul24(iy, t.pitch) + ix;
-  return __builtin_amdgcn_raw_buffer_load_b32(t.rsrc, rec << 2, 0, 0);
+  return pm_struct_load_b32(t.rsrc, rec, 0, 0, 0);
 }
 
 // floor(x) as an int in one instruction (x within the int range).
@@ -282,7 +296,7 @@ __device__ __forceinline__ uint32_t fetch_quad_xy(const QuadTex& t, float x, flo
   const int ix = cvt_floor(__builtin_amdgcn_fmed3f(x, -1.0f, t.xmax));
   const int iy = cvt_floor(__builtin_amdgcn_fmed3f(y, -1.0f, t.ymax));
   const int rec = __mul24(iy, t.pitch) + ix;
-  return __builtin_amdgcn_raw_buffer_load_b32(t.rsrc, rec << 2, 0, 0);
+  return pm_struct_load_b32(t.rsrc, rec, 0, 0, 0);
 }
 
 // Texture of 8-byte quads (FP16 or BF16 values) of a wave-uniform view.
@@ -295,7 +309,7 @@ __device__ __forceinline__ QuadTex make_tex16(const uint32_t* quad16, int W,
   t.pitch = W + 1;
   const int records = t.pitch * (H + 1) - (t.pitch + 1);
   t.rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      (void*)(quad16 + 2 * (t.pitch + 1)), (short)0, records * 8, 0x00020000);
+      (void*)(quad16 + 2 * (t.pitch + 1)), (short)8, records, 0x00020000);
   t.xmax = (float)(W - 1);
   t.ymax = (float)(H - 1);
   return t;
@@ -306,7 +320,7 @@ __device__ __forceinline__ uint2 fetch_quad16_xy(const QuadTex& t, float x, floa
   const int ix = cvt_floor(__builtin_amdgcn_fmed3f(x, -1.0f, t.xmax));
   const int iy = cvt_floor(__builtin_amdgcn_fmed3f(y, -1.0f, t.ymax));
   const int rec = __mul24(iy, t.pitch) + ix;
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(t.rsrc, rec << 3, 0, 0);
+  const pm_u32x2 v = pm_struct_load_b64(t.rsrc, rec, 0, 0, 0);
   return make_uint2(v[0], v[1]);
 }
 
