@@ -77,6 +77,10 @@ struct pm_batch {
   DevBuf viewtab, jobtab, wlut, state, score, nghbr;
   DevBuf out_depth, out_plane, out_score, out_nghbr, queue, counter;
   DevBuf cons_count, cons_idx, cons_med, cons_job;  // median-consistent re-selection list
+  // and its window-phase work items (pm::ConsistItems)
+  DevBuf item_count, item_entry, item_win, item_score, item_nghbr, item_depth, item_first,
+      item_todo;
+  unsigned item_cap = 0;
   // coarse pool views of the learned initialiser's input
   int coarse_scale = 0;
   std::vector<std::unique_ptr<CoarseView>> coarse;
@@ -202,6 +206,18 @@ void prepare(pm_batch* b, int count) {
     b->cons_idx.ensure(px * sizeof(unsigned));
     b->cons_med.ensure(px * sizeof(float));
     b->cons_job.ensure(px * sizeof(int));
+    // window items: room for two per pixel, at most 2^26 (more are scored
+    // inline)
+    const size_t cap = std::min<size_t>(2 * px, (size_t)1 << 26);
+    b->item_count.ensure(sizeof(unsigned long long));
+    b->item_entry.ensure(cap * sizeof(unsigned));
+    b->item_win.ensure(cap);
+    b->item_score.ensure(cap * sizeof(float));
+    b->item_nghbr.ensure(cap * sizeof(int));
+    b->item_depth.ensure(cap * sizeof(float));
+    b->item_first.ensure(px * sizeof(unsigned));
+    b->item_todo.ensure(px * sizeof(unsigned));
+    b->item_cap = (unsigned)cap;
   }
   // one init queue, then 8 band queues per red/black launch
   b->queue.ensure((1 + 16 * (size_t)std::max(b->iterations, 1)) * sizeof(unsigned));
@@ -311,10 +327,14 @@ void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
       const pm::ConsistOut res{b->out_plane.as<float>(), b->out_depth.as<float>(),
                                b->out_score.as<float>(), b->out_nghbr.as<int>(),
                                pm::kConsistMinScore, last ? pm::kConsistFinalSupport : 0};
+      const pm::ConsistItems items{
+          b->item_count.as<unsigned long long>(), b->item_cap, b->item_entry.as<unsigned>(),
+          b->item_win.as<unsigned char>(), b->item_score.as<float>(), b->item_nghbr.as<int>(),
+          b->item_depth.as<float>(), b->item_first.as<unsigned>(), b->item_todo.as<unsigned>()};
       HIP_CHECK(pm::launch_pm_consist_batch(
           jt, nj, b->pix_total, b->patch_size, b->precision, sample,
           b->cons_count.as<unsigned>(), b->cons_idx.as<unsigned>(), b->cons_med.as<float>(),
-          b->cons_job.as<int>(), res, (unsigned)it, cnt, b->stream));
+          b->cons_job.as<int>(), res, (unsigned)it, items, cnt, b->stream));
     }
   }
   HIP_CHECK(pm::launch_pack_batch(jt, nj, b->pix_total, b->post_process, b->stream));

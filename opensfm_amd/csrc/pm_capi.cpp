@@ -59,6 +59,9 @@ struct pm_estimator {
   // per-compute device buffers
   DevBuf viewtab, wlut, state, score, nghbr, counter, queues;
   DevBuf cons_count, cons_idx, cons_med;  // median-consistent re-selection list
+  // and its window-phase work items (pm::ConsistItems)
+  DevBuf item_count, item_entry, item_win, item_score, item_nghbr, item_depth, item_first,
+      item_todo;
   DevBuf out_depth, out_plane, out_score, out_nghbr;
   DevBuf sweep_H, sweep_planes;
   bool have_result = false;
@@ -235,9 +238,24 @@ hipError_t consistency_pass(pm_estimator* e, const pm::PmArgs& a0, bool sample, 
   const pm::ConsistOut res{e->out_plane.as<float>(), e->out_depth.as<float>(),
                            e->out_score.as<float>(), e->out_nghbr.as<int>(),
                            pm::kConsistMinScore, final_pass ? pm::kConsistFinalSupport : 0};
+  // window items: room for two per pixel (more are scored inline)
+  const size_t cap = std::min<size_t>(2 * np, (size_t)1 << 26);
+  e->item_count.ensure(sizeof(unsigned long long));
+  e->item_entry.ensure(cap * sizeof(unsigned));
+  e->item_win.ensure(cap);
+  e->item_score.ensure(cap * sizeof(float));
+  e->item_nghbr.ensure(cap * sizeof(int));
+  e->item_depth.ensure(cap * sizeof(float));
+  e->item_first.ensure(np * sizeof(unsigned));
+  e->item_todo.ensure(np * sizeof(unsigned));
+  const pm::ConsistItems items{e->item_count.as<unsigned long long>(), (unsigned)cap,
+                               e->item_entry.as<unsigned>(), e->item_win.as<unsigned char>(),
+                               e->item_score.as<float>(), e->item_nghbr.as<int>(),
+                               e->item_depth.as<float>(), e->item_first.as<unsigned>(),
+                               e->item_todo.as<unsigned>()};
   return pm::launch_pm_consist(a, sample, e->cons_count.as<unsigned>(),
                                e->cons_idx.as<unsigned>(), e->cons_med.as<float>(), res, salt,
-                               e->stream);
+                               items, e->stream);
 }
 
 int run(pm_estimator* e, int mode) {

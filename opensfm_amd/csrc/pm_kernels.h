@@ -41,16 +41,19 @@ hipError_t launch_init_input(const PmArgs& a, int nplanes, float inv_min,
                              hipStream_t s);
 // Median-consistent re-selection of a's maps, in place: count (1 unsigned)
 // and list_idx / list_med (W*H entries each) are scratch, res holds W*H
-// entries.  salt varies the refinement draws between passes.
+// entries, items (plain mode) holds the window phase's work items (first /
+// todo W*H entries, the item arrays items.cap entries).  salt varies the
+// refinement draws between passes.
 hipError_t launch_pm_consist(const PmArgs& a, bool sample, unsigned* count, unsigned* list_idx,
                              float* list_med, const ConsistOut& res, unsigned salt,
-                             hipStream_t s);
+                             const ConsistItems& items, hipStream_t s);
 // The same over every job of a batch (list_job: W*H entries over all jobs).
 hipError_t launch_pm_consist_batch(const PmJob* jobs, int njobs, int pix_total, int patch,
                                    int precision, bool sample, unsigned* count,
                                    unsigned* list_idx, float* list_med, int* list_job,
                                    const ConsistOut& res, unsigned salt,
-                                   unsigned long long* counter, hipStream_t s);
+                                   const ConsistItems& items, unsigned long long* counter,
+                                   hipStream_t s);
 hipError_t launch_pack(const PmArgs& a, int median, float* depth, float* plane,
                        float* score, int* nghbr, hipStream_t s);
 // Batched launches over the jobs of a PmJob table (see pm_types.h).  `tile`

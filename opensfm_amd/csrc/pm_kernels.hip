@@ -1086,18 +1086,96 @@ __device__ __forceinline__ unsigned consist_refine(const PmArgs& a, const RefPat
   return tries;
 }
 
-// kCompact (plain mode only): each lane first lists the window planes it
-// will score, then scores its own list, so a wave runs as many scorings as
-// its longest list.  Otherwise all lanes take the window positions in step,
-// so that neighbouring lanes score the planes of neighbouring pixels at the
-// same time (coherent gathers).  Measured: the batched pass (many listed
-// pixels, throughput-bound) is 9-24 % faster compacted; the single-view
-// pass (a short list, bound by the latency of its slowest wave) is twice as
-// slow compacted.  Both give the same maps bit for bit.
-template <int P, bool kSample, int Prec, bool kCompact>
+// The window phase of one listed pixel in plain mode: the window positions
+// whose plane passes the median test at this pixel, except a plane equal to
+// an earlier window position's (it scores the same, or fails the same test,
+// and cannot win a strict comparison).  Bit w stands for window pixel w.
+__device__ __forceinline__ unsigned consist_todo(const PmArgs& a, size_t idx, float m) {
+  const int i = (int)(idx / (size_t)a.W), j = (int)(idx % (size_t)a.W);
+  const float4* st = reinterpret_cast<const float4*>(a.state);
+  const PixelRay ray = pixel_ray(a, i, j);
+  float px[25], py[25], pz[25];
+  unsigned todo = 0;
+#pragma unroll
+  for (int w = 0; w < 25; ++w) {
+    if (w == 12) continue;
+    const float4 c = st[consist_window(a, i, j, w)];
+    px[w] = c.x;
+    py[w] = c.y;
+    pz[w] = c.z;
+    const PlaneAtPixel pp = plane_at_pixel(a, ray, c.x, c.y, c.z);
+    bool go = c.w != 0.0f && median_keeps(__frcp_rn(fmaxf(1e-6f, -pp.gq)), m);
+#pragma unroll
+    for (int u = 0; u < w; ++u) {
+      if (u == 12) continue;
+      go = go && !(px[u] == c.x && py[u] == c.y && pz[u] == c.z);
+    }
+    todo |= go ? (1u << w) : 0u;
+  }
+  return todo;
+}
+
+// Allocates the items of listed entry e (todo from consist_todo; e >= n:
+// none) with one atomic per wave, and writes them in window order.  Every
+// lane of the wave must call this.
+__device__ __forceinline__ void consist_alloc(const ConsistItems& it, unsigned e, unsigned n,
+                                              unsigned todo) {
+  const unsigned c = e < n ? (unsigned)__popc(todo) : 0u;
+  const int lane = (int)(threadIdx.x & 63);
+  unsigned incl = c;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
+  }
+  const unsigned total = __shfl(incl, 63);
+  unsigned long long base = 0;
+  if (lane == 0 && total) base = atomicAdd(it.count, (unsigned long long)total);
+  base = __shfl(base, 0);
+  if (e >= n) return;
+  const unsigned long long first = base + incl - c;
+  const bool fits = first + c <= (unsigned long long)it.cap;
+  it.first[e] = fits ? (unsigned)first : 0u;
+  it.todo[e] = fits ? todo : (todo | 0x80000000u);
+  unsigned long long k = first;
+  for (unsigned t = todo; t; t &= t - 1u, ++k) {
+    if (k >= (unsigned long long)it.cap) break;
+    it.entry[k] = fits ? e : 0xFFFFFFFFu;
+    it.win[k] = (unsigned char)__builtin_ctz(t);
+  }
+}
+
+// Scores window item k (window position w of the pixel at idx) over all
+// views.
+template <int P, int Prec>
+__device__ __forceinline__ unsigned consist_score_item(const PmArgs& a, size_t idx, int w,
+                                                       const ConsistItems& it, unsigned k) {
+  const int i = (int)(idx / (size_t)a.W), j = (int)(idx % (size_t)a.W);
+  const float4* st = reinterpret_cast<const float4*>(a.state);
+  RefPatch<P> rp;
+  float sc = -1.0f, d2 = 0.0f;
+  int nb = 0;
+  unsigned tries = 0;
+  if (load_ref_patch<P, true>(rp, a.ref_img, a.W, i, j, a.wlut)) {
+    const PixelRay ray = pixel_ray(a, i, j);
+    const float4 c = st[consist_window(a, i, j, w)];
+    sc = score_plane_all<P, Prec>(a, rp, ray, i, j, c.x, c.y, c.z, &nb, &d2);
+    tries = (unsigned)(a.nviews - 1);
+  }
+  it.score[k] = sc;
+  it.nghbr[k] = nb;
+  it.depth[k] = d2;
+  return tries;
+}
+
+// Plain mode reads the window phase's scores from the items (in window
+// order, strict >, so the first of equal scores wins as in a sequential
+// loop); sample mode scores each window plane against its pixel's view here,
+// the lanes taking the window positions in step.
+template <int P, bool kSample, int Prec>
 __device__ __forceinline__ unsigned consist_select(const PmArgs& a, size_t idx, float m,
                                                    const ConsistOut& res, unsigned e,
-                                                   unsigned salt) {
+                                                   unsigned salt, const ConsistItems& items) {
   const int i = (int)(idx / (size_t)a.W), j = (int)(idx % (size_t)a.W);
   const float4* st = reinterpret_cast<const float4*>(a.state);
   float4 cur = st[idx];
@@ -1110,9 +1188,9 @@ __device__ __forceinline__ unsigned consist_select(const PmArgs& a, size_t idx, 
     float bs = -2.0f;
     int bn = 0;
     float4 bp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if constexpr (kSample || !kCompact) {
-      // window order; a plane equal to one of the last four scored planes
-      // scores the same and cannot win a strict comparison: skipped
+    if constexpr (kSample) {
+      // a plane equal to one of the last four scored planes scores the same
+      // and cannot win a strict comparison: skipped
       float px[4] = {0.0f, 0.0f, 0.0f, 0.0f}, py[4] = {0.0f, 0.0f, 0.0f, 0.0f},
             pz[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // the last scored planes
       int nseen = 0;
@@ -1128,16 +1206,11 @@ __device__ __forceinline__ unsigned consist_select(const PmArgs& a, size_t idx, 
         for (int k = 0; k < 4; ++k)
           dup |= k < nseen && px[k] == c.x && py[k] == c.y && pz[k] == c.z;
         go = go && !dup;
-        int nb = 0;
-        float sc = -1.0f, d2 = 0.0f;
-        if constexpr (kSample) {
-          nb = a.nghbr[q];
-          sc = score_plane_view<P, Prec>(a, rp, ray, i, j, c.x, c.y, c.z, nb, go, &d2);
-        } else if (go) {
-          sc = score_plane_all<P, Prec>(a, rp, ray, i, j, c.x, c.y, c.z, &nb, &d2);
-        }
+        const int nb = a.nghbr[q];
+        float d2 = 0.0f;
+        const float sc = score_plane_view<P, Prec>(a, rp, ray, i, j, c.x, c.y, c.z, nb, go, &d2);
         if (go) {
-          tries += kSample ? 1u : (unsigned)(a.nviews - 1);
+          tries += 1u;
           const int slot = nseen & 3;
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
@@ -1156,36 +1229,31 @@ __device__ __forceinline__ unsigned consist_select(const PmArgs& a, size_t idx, 
         }
       }
     } else {
-      // The window planes to score: those whose depth at this pixel passes
-      // the median test, except a plane equal to an earlier window pixel's
-      // (it scores the same, or fails the same test, and cannot win a
-      // strict comparison).  Then each lane scores its own list in window
-      // order.
-      unsigned todo = 0;
-      for (int w = 0; w < 25; ++w) {
-        if (w == 12) continue;
-        const float4 c = st[consist_window(a, i, j, w)];
-        const PlaneAtPixel pp = plane_at_pixel(a, ray, c.x, c.y, c.z);
-        bool go = c.w != 0.0f && median_keeps(__frcp_rn(fmaxf(1e-6f, -pp.gq)), m);
-        for (int u = 0; u < w; ++u) {
-          if (u == 12) continue;
-          const float4 c2 = st[consist_window(a, i, j, u)];
-          go = go && !(c2.x == c.x && c2.y == c.y && c2.z == c.z);
+      const unsigned todo = items.todo[e];
+      if (!(todo >> 31)) {
+        unsigned k = items.first[e];
+        for (unsigned t = todo; t; t &= t - 1u, ++k) {
+          const float sc = items.score[k];
+          if (sc > bs) {
+            const float4 c = st[consist_window(a, i, j, __builtin_ctz(t))];
+            bs = sc;
+            bn = items.nghbr[k];
+            bp = make_float4(c.x, c.y, c.z, items.depth[k]);
+          }
         }
-        todo |= go ? (1u << w) : 0u;
-      }
-      while (todo) {
-        const int w = __builtin_ctz(todo);
-        todo &= todo - 1u;
-        const float4 c = st[consist_window(a, i, j, w)];
-        int nb = 0;
-        float d2 = 0.0f;
-        const float sc = score_plane_all<P, Prec>(a, rp, ray, i, j, c.x, c.y, c.z, &nb, &d2);
-        tries += (unsigned)(a.nviews - 1);
-        if (sc > bs) {
-          bs = sc;
-          bn = nb;
-          bp = make_float4(c.x, c.y, c.z, d2);
+      } else {
+        // items did not fit: score the window planes here
+        for (unsigned t = todo & 0x7FFFFFFFu; t; t &= t - 1u) {
+          const float4 c = st[consist_window(a, i, j, __builtin_ctz(t))];
+          int nb = 0;
+          float d2 = 0.0f;
+          const float sc = score_plane_all<P, Prec>(a, rp, ray, i, j, c.x, c.y, c.z, &nb, &d2);
+          tries += (unsigned)(a.nviews - 1);
+          if (sc > bs) {
+            bs = sc;
+            bn = nb;
+            bp = make_float4(c.x, c.y, c.z, d2);
+          }
         }
       }
     }
@@ -1214,15 +1282,84 @@ __device__ __forceinline__ unsigned consist_select(const PmArgs& a, size_t idx, 
   return tries;
 }
 
+__global__ __launch_bounds__(256) void k_consist_plan(PmArgs a, const unsigned* __restrict__ count,
+                                                      const unsigned* __restrict__ list_idx,
+                                                      const float* __restrict__ list_med,
+                                                      ConsistItems it) {
+  const unsigned n = *count;
+  for (unsigned base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    const unsigned e = base + threadIdx.x;
+    const unsigned todo = e < n ? consist_todo(a, list_idx[e], list_med[e]) : 0u;
+    consist_alloc(it, e, n, todo);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_consist_plan_batch(const PmJob* __restrict__ jobs,
+                                                            const unsigned* __restrict__ count,
+                                                            const unsigned* __restrict__ list_idx,
+                                                            const float* __restrict__ list_med,
+                                                            const int* __restrict__ list_job,
+                                                            ConsistItems it) {
+  const unsigned n = *count;
+  for (unsigned base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    const unsigned e = base + threadIdx.x;
+    const unsigned todo =
+        e < n ? consist_todo(jobs[list_job[e]].a, list_idx[e], list_med[e]) : 0u;
+    consist_alloc(it, e, n, todo);
+  }
+}
+
+template <int P, int Prec>
+__global__ __launch_bounds__(256) void k_consist_score(PmArgs a,
+                                                       const unsigned* __restrict__ list_idx,
+                                                       ConsistItems it) {
+  const unsigned long long total = *it.count;
+  const unsigned n = (unsigned)(total < it.cap ? total : it.cap);
+  unsigned tries = 0;
+  for (unsigned k = blockIdx.x * 256u + threadIdx.x; k < n; k += gridDim.x * 256u) {
+    const unsigned e = it.entry[k];
+    if (e != 0xFFFFFFFFu) tries += consist_score_item<P, Prec>(a, list_idx[e], it.win[k], it, k);
+  }
+  if (a.counter) wave_count(a.counter, tries);
+}
+
+// The texture descriptors must be wave-uniform, so the wave takes the jobs
+// of its items one at a time (items of one job are mostly contiguous).
+template <int P, int Prec>
+__global__ __launch_bounds__(256) void k_consist_score_batch(
+    const PmJob* __restrict__ jobs, const unsigned* __restrict__ list_idx,
+    const int* __restrict__ list_job, ConsistItems it, unsigned long long* __restrict__ counter) {
+  const unsigned long long total = *it.count;
+  const unsigned n = (unsigned)(total < it.cap ? total : it.cap);
+  unsigned tries = 0;
+  for (unsigned base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    const unsigned k = base + threadIdx.x;
+    const unsigned e = k < n ? it.entry[k] : 0xFFFFFFFFu;
+    bool pend = e != 0xFFFFFFFFu;
+    const int job = pend ? list_job[e] : -1;
+    for (;;) {
+      const unsigned long long m = __ballot(pend);
+      if (m == 0) break;
+      const int ju = __builtin_amdgcn_readfirstlane(__shfl(job, __ffsll((long long)m) - 1));
+      if (pend && job == ju) {
+        tries += consist_score_item<P, Prec>(jobs[ju].a, list_idx[e], it.win[k], it, k);
+        pend = false;
+      }
+    }
+  }
+  if (counter) wave_count(counter, tries);
+}
+
 template <int P, bool kSample, int Prec>
 __global__ __launch_bounds__(256) void k_consist_select(PmArgs a, const unsigned* __restrict__ count,
                                                         const unsigned* __restrict__ list_idx,
                                                         const float* __restrict__ list_med,
-                                                        ConsistOut res, unsigned salt) {
+                                                        ConsistOut res, unsigned salt,
+                                                        ConsistItems items) {
   const unsigned n = *count;
   unsigned tries = 0;
   for (unsigned e = blockIdx.x * 256u + threadIdx.x; e < n; e += gridDim.x * 256u)
-    tries += consist_select<P, kSample, Prec, false>(a, list_idx[e], list_med[e], res, e, salt);
+    tries += consist_select<P, kSample, Prec>(a, list_idx[e], list_med[e], res, e, salt, items);
   if (a.counter) wave_count(a.counter, tries);
 }
 
@@ -1231,7 +1368,7 @@ __global__ __launch_bounds__(256) void k_consist_select_batch(
     const PmJob* __restrict__ jobs, const unsigned* __restrict__ count,
     const unsigned* __restrict__ list_idx, const float* __restrict__ list_med,
     const int* __restrict__ list_job, ConsistOut res, unsigned salt,
-    unsigned long long* __restrict__ counter) {
+    unsigned long long* __restrict__ counter, ConsistItems items) {
   // The views' texture descriptors must be wave-uniform, so the wave takes
   // the jobs of its entries one at a time (entries of one job are mostly
   // contiguous in the list).
@@ -1246,8 +1383,8 @@ __global__ __launch_bounds__(256) void k_consist_select_batch(
       if (m == 0) break;
       const int ju = __builtin_amdgcn_readfirstlane(__shfl(job, __ffsll((long long)m) - 1));
       if (pend && job == ju) {
-        tries += consist_select<P, kSample, Prec, true>(jobs[ju].a, list_idx[e], list_med[e], res, e,
-                                                  salt);
+        tries += consist_select<P, kSample, Prec>(jobs[ju].a, list_idx[e], list_med[e], res, e,
+                                                  salt, items);
         pend = false;
       }
     }
@@ -1893,21 +2030,28 @@ hipError_t launch_score_hyp_fast(const PmArgs& a, int view, int n,
 
 hipError_t launch_pm_consist(const PmArgs& a, bool sample, unsigned* count, unsigned* list_idx,
                              float* list_med, const ConsistOut& res, unsigned salt,
-                             hipStream_t s) {
+                             const ConsistItems& items, hipStream_t s) {
   const int P_ = 2 * a.hpz + 1;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u);
   hipLaunchKernelGGL(k_consist_mark, grid16(a.W, a.H), dim3(256), 0, s, a, count, list_idx,
                      list_med);
-  // a fixed grid striding over the list (its length is on the device)
+  // fixed grids striding over the list and the items (their lengths are on
+  // the device)
   const dim3 grid(1024);
   if (sample) {
     PM_DISPATCH_PREC(a.precision, P_,
                      hipLaunchKernelGGL((k_consist_select<P, true, Prec>), grid, dim3(256), 0, s,
-                                        a, count, list_idx, list_med, res, salt));
+                                        a, count, list_idx, list_med, res, salt, items));
   } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s,
+                       reinterpret_cast<unsigned*>(items.count), (size_t)2, 0u);
+    hipLaunchKernelGGL(k_consist_plan, grid, dim3(256), 0, s, a, count, list_idx, list_med, items);
+    PM_DISPATCH_PREC(a.precision, P_,
+                     hipLaunchKernelGGL((k_consist_score<P, Prec>), dim3(2048), dim3(256), 0, s,
+                                        a, list_idx, items));
     PM_DISPATCH_PREC(a.precision, P_,
                      hipLaunchKernelGGL((k_consist_select<P, false, Prec>), grid, dim3(256), 0, s,
-                                        a, count, list_idx, list_med, res, salt));
+                                        a, count, list_idx, list_med, res, salt, items));
   }
   hipLaunchKernelGGL(k_consist_apply, grid, dim3(256), 0, s, a, count, list_idx, res);
   return hipGetLastError();
@@ -1917,7 +2061,8 @@ hipError_t launch_pm_consist_batch(const PmJob* jobs, int njobs, int pix_total, 
                                    int precision, bool sample, unsigned* count,
                                    unsigned* list_idx, float* list_med, int* list_job,
                                    const ConsistOut& res, unsigned salt,
-                                   unsigned long long* counter, hipStream_t s) {
+                                   const ConsistItems& items, unsigned long long* counter,
+                                   hipStream_t s) {
   const int P_ = patch;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, (size_t)1, 0u);
   if (pix_total <= 0) return hipGetLastError();
@@ -1928,12 +2073,19 @@ hipError_t launch_pm_consist_batch(const PmJob* jobs, int njobs, int pix_total, 
     PM_DISPATCH_PREC(precision, P_,
                      hipLaunchKernelGGL((k_consist_select_batch<P, true, Prec>), grid, dim3(256),
                                         0, s, jobs, count, list_idx, list_med, list_job, res,
-                                        salt, counter));
+                                        salt, counter, items));
   } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s,
+                       reinterpret_cast<unsigned*>(items.count), (size_t)2, 0u);
+    hipLaunchKernelGGL(k_consist_plan_batch, grid, dim3(256), 0, s, jobs, count, list_idx,
+                       list_med, list_job, items);
+    PM_DISPATCH_PREC(precision, P_,
+                     hipLaunchKernelGGL((k_consist_score_batch<P, Prec>), dim3(4096), dim3(256),
+                                        0, s, jobs, list_idx, list_job, items, counter));
     PM_DISPATCH_PREC(precision, P_,
                      hipLaunchKernelGGL((k_consist_select_batch<P, false, Prec>), grid, dim3(256),
                                         0, s, jobs, count, list_idx, list_med, list_job, res,
-                                        salt, counter));
+                                        salt, counter, items));
   }
   hipLaunchKernelGGL(k_consist_apply_batch, grid, dim3(256), 0, s, jobs, count, list_idx, list_job,
                      res);

@@ -87,6 +87,26 @@ struct ConsistOut {
   int min_support;
 };
 
+// The window phase of a re-selection pass as a flat list of work items, one
+// per (listed pixel, window position) to score, so that the lanes of a wave
+// score the candidates of several pixels at once (k_consist_plan allocates
+// them, k_consist_score scores them, k_consist_select reads the scores).
+// Per listed entry e: todo[e] holds the window positions it scores (bit w)
+// and first[e] its first item; bit 31 of todo[e] marks an entry whose items
+// did not fit in `cap` and which k_consist_select scores itself.  Items of
+// such an entry below `cap` carry entry = 0xFFFFFFFF and are skipped.
+struct ConsistItems {
+  unsigned long long* count;  // items allocated (may exceed cap)
+  unsigned cap;               // capacity of the item arrays
+  unsigned* entry;            // listed entry of item k
+  unsigned char* win;         // window position of item k
+  float* score;               // the window plane's score over all views
+  int* nghbr;                 // the view of that score
+  float* depth;               // the plane's depth at the pixel
+  unsigned* first;            // per listed entry
+  unsigned* todo;             // per listed entry
+};
+
 // One reference view of a batch: its estimator arguments plus where its
 // tiles sit in a launch's flat tile space.  Red/black tiles are TX x TY lanes
 // over (columns of one colour) x rows; init/pack/clean tiles are 16 x 16
