@@ -69,8 +69,27 @@ typedef struct pm_stats {
 /* Message of the calling thread's last failed call ("" if none). */
 const char* pm_last_error(void);
 
-/* Number of visible HIP devices. */
+/* Number of visible HIP devices.
+ * DepthmapClusterEstimator::NumDevices  depthmap_bind.h:85 */
 int pm_device_count(int* count);
+
+/* Name (NUL-terminated, truncated to name_capacity - 1 bytes), global memory
+ * and currently free memory of HIP device `device`.
+ * DeviceName / DeviceMemoryBytes  depthmap_bind.h:87-93 (CLDevice name and
+ * global memory size, opencl_utils.h:34-125); free_bytes is the runtime's
+ * live figure (hipMemGetInfo), which the reference does not report. */
+int pm_device_info(int device, char* name, int name_capacity, uint64_t* total_bytes,
+                   uint64_t* free_bytes);
+
+/* Memory-reservation bookkeeping of the dense scheduler: reserve (bytes > 0)
+ * or release (bytes < 0) on device `device`; releasing more than is
+ * reserved leaves zero.  *available = global memory - reserved bytes (0 if
+ * more is reserved than exists).  Nothing is allocated.
+ * ReserveDeviceMemory / ReleaseDeviceMemory / DeviceAvailableMemory
+ * depthmap_bind.h:99-108, CLDevice::ReserveMemory / ReleaseMemory /
+ * AvailableMemory opencl_utils.h:68-76. */
+int pm_device_reserve(int device, int64_t bytes);
+int pm_device_available(int device, uint64_t* available);
 
 /* ---- DepthmapEstimator ------------------------------------------------ */
 

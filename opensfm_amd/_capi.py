@@ -59,6 +59,10 @@ _i64p = C.POINTER(C.c_int64)
 SIGNATURES = {
     "pm_last_error": (C.c_char_p, []),
     "pm_device_count": (C.c_int, [_ip]),
+    "pm_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint64)]),
+    "pm_device_reserve": (C.c_int, [C.c_int, C.c_int64]),
+    "pm_device_available": (C.c_int, [C.c_int, C.POINTER(C.c_uint64)]),
     "pm_estimator_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "pm_estimator_destroy": (C.c_int, [_vp]),
     "pm_estimator_add_view": (C.c_int, [_vp, _dp, _dp, _dp, _u8p, _u8p, C.c_int,
@@ -251,6 +255,14 @@ def device_count() -> int:
     n = C.c_int32(0)
     check(load().pm_device_count(C.byref(n)))
     return n.value
+
+
+def device_info(idx: int):
+    """(name, global memory bytes, free bytes) of HIP device idx."""
+    name = C.create_string_buffer(256)
+    total, free = C.c_uint64(0), C.c_uint64(0)
+    check(load().pm_device_info(int(idx), name, 256, C.byref(total), C.byref(free)))
+    return name.value.decode(errors="replace"), total.value, free.value
 
 
 def precision_code(p) -> int:

@@ -21,6 +21,8 @@ links, the SfM points) without changing the result.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 from typing import List
 
 import numpy as np
@@ -160,27 +162,25 @@ class DepthmapClusterEstimator:
         return _capi.device_count()
 
     @staticmethod
-    def _props(idx: int):
-        import torch
-        return torch.cuda.get_device_properties(int(idx))
-
-    @staticmethod
     def device_name(idx: int) -> str:
-        return DepthmapClusterEstimator._props(idx).name
+        return _capi.device_info(idx)[0]
 
     @staticmethod
     def device_memory_bytes(idx: int) -> int:
-        return int(DepthmapClusterEstimator._props(idx).total_memory)
+        return int(_capi.device_info(idx)[1])
 
     @staticmethod
     def device_is_gpu(idx: int) -> bool:
+        _capi.device_info(idx)  # range check: every visible HIP device is a GPU
         return True
 
     @staticmethod
     def device_available_memory(idx: int) -> int:
-        import torch
-        free, _ = torch.cuda.mem_get_info(int(idx))
-        return int(free)
+        """Global memory minus the bytes reserved with reserve_device_memory
+        (opencl_utils.h:68-72)."""
+        n = C.c_uint64(0)
+        _capi.check(_capi.load().pm_device_available(int(idx), C.byref(n)))
+        return int(n.value)
 
     @staticmethod
     def gpu_memory_bytes() -> int:
@@ -188,9 +188,13 @@ class DepthmapClusterEstimator:
 
     @staticmethod
     def reserve_device_memory(idx: int, nbytes: int) -> None:
-        """Bookkeeping only in the reference's OpenCL context; device memory
-        here is allocated on use."""
+        """Bookkeeping of the dense scheduler, as the reference's OpenCL
+        context keeps it (opencl_utils.h:73): device memory itself is
+        allocated on use."""
+        _capi.check(_capi.load().pm_device_reserve(int(idx), int(nbytes)))
 
     @staticmethod
     def release_device_memory(idx: int, nbytes: int) -> None:
-        """See reserve_device_memory."""
+        """Releases reserved bytes; more than is reserved leaves zero
+        (opencl_utils.h:74-76)."""
+        _capi.check(_capi.load().pm_device_reserve(int(idx), -int(nbytes)))

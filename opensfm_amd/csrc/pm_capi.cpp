@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -555,6 +556,75 @@ int pm_device_count(int* count) {
   hipError_t err = hipGetDeviceCount(&n);
   if (err != hipSuccess) n = 0;
   *count = n;
+  g_last_error.clear();
+  return PM_OK;
+}
+
+namespace {
+// Reserved bytes per device (the dense scheduler's bookkeeping).
+std::mutex g_reserve_mu;
+std::vector<uint64_t> g_reserved;
+
+int device_props(int device, hipDeviceProp_t* p) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  if (device < 0 || device >= n)
+    return fail(PM_ERR_RANGE, "device " + std::to_string(device) + " out of range (" +
+                                  std::to_string(n) + " HIP devices)");
+  if (hipGetDeviceProperties(p, device) != hipSuccess)
+    return fail(PM_ERR_RUNTIME, "hipGetDeviceProperties failed");
+  return PM_OK;
+}
+}  // namespace
+
+int pm_device_info(int device, char* name, int name_capacity, uint64_t* total_bytes,
+                   uint64_t* free_bytes) {
+  hipDeviceProp_t p;
+  int rc = device_props(device, &p);
+  if (rc) return rc;
+  if (name && name_capacity > 0) {
+    const size_t n = std::min(strlen(p.name), (size_t)name_capacity - 1);
+    memcpy(name, p.name, n);
+    name[n] = 0;
+  }
+  if (total_bytes) *total_bytes = (uint64_t)p.totalGlobalMem;
+  if (free_bytes) {
+    size_t fr = 0, tot = 0;
+    DeviceGuard dg(device);
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess)
+      return fail(PM_ERR_RUNTIME, "hipMemGetInfo failed");
+    *free_bytes = (uint64_t)fr;
+  }
+  g_last_error.clear();
+  return PM_OK;
+}
+
+int pm_device_reserve(int device, int64_t bytes) {
+  hipDeviceProp_t p;
+  int rc = device_props(device, &p);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_reserve_mu);
+  if ((int)g_reserved.size() <= device) g_reserved.resize(device + 1, 0);
+  uint64_t& r = g_reserved[device];
+  if (bytes >= 0) {
+    r += (uint64_t)bytes;
+  } else {
+    const uint64_t rel = (uint64_t)(-(bytes + 1)) + 1;
+    r = rel > r ? 0 : r - rel;
+  }
+  g_last_error.clear();
+  return PM_OK;
+}
+
+int pm_device_available(int device, uint64_t* available) {
+  if (!available) return fail(PM_ERR_INVALID, "null output");
+  hipDeviceProp_t p;
+  int rc = device_props(device, &p);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_reserve_mu);
+  const uint64_t total = (uint64_t)p.totalGlobalMem;
+  const uint64_t r = (int)g_reserved.size() > device ? g_reserved[device] : 0;
+  *available = r >= total ? 0 : total - r;
   g_last_error.clear();
   return PM_OK;
 }

@@ -59,6 +59,12 @@ def test_no_device_errors_are_clean():
         DepthmapEstimator()
     with pytest.raises(RuntimeError, match="no HIP device"):
         DepthmapCleaner()
+    # device queries name the missing device (IndexError, like the
+    # reference's out-of-range device index)
+    with pytest.raises(IndexError, match="out of range"):
+        _capi.device_info(0)
+    with pytest.raises(IndexError, match="out of range"):
+        _capi.check(_capi.load().pm_device_reserve(0, 1))
 
 
 def test_null_handle_and_argument_validation():

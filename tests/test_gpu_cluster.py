@@ -60,3 +60,31 @@ def test_cluster_surface(cfg1_scene):
     ce.clear()
     assert ce.run() == [] and ce.num_ref_views() == 0
     assert DepthmapClusterEstimator.is_available() and DepthmapClusterEstimator.num_devices() >= 1
+
+
+@pytest.mark.gpu
+def test_device_queries_and_reservations():
+    """Device name and memory (depthmap_bind.h:85-100) through the C ABI,
+    and the reservation bookkeeping of opencl_utils.h:68-76: available =
+    global memory - reserved, releasing more than is reserved leaves zero."""
+    from opensfm_amd import DepthmapClusterEstimator as CE
+    n = CE.num_devices()
+    assert n >= 1 and CE.is_available()
+    name = CE.device_name(0)
+    total = CE.device_memory_bytes(0)
+    assert isinstance(name, str) and name
+    assert total > 1 << 30 and CE.device_is_gpu(0)
+    assert CE.device_available_memory(0) == total
+    CE.reserve_device_memory(0, 5 << 30)
+    CE.reserve_device_memory(0, 1 << 30)
+    assert CE.device_available_memory(0) == total - (6 << 30)
+    CE.release_device_memory(0, 2 << 30)
+    assert CE.device_available_memory(0) == total - (4 << 30)
+    CE.release_device_memory(0, 100 << 30)
+    assert CE.device_available_memory(0) == total
+    CE.reserve_device_memory(0, total + 1)
+    assert CE.device_available_memory(0) == 0
+    CE.release_device_memory(0, total + 1)
+    assert CE.device_available_memory(0) == total
+    with pytest.raises(IndexError):
+        CE.device_name(n)
