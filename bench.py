@@ -154,11 +154,11 @@ def cpu_baseline(scene, threads, seed_sets=5, target_s=4.0):
     }
 
 
-def street_crops_oracle(sc, patch, iterations, jobs=8, side=160, seed=4000):
+def street_crops_oracle(sc, patch, iterations, jobs=16, side=160, seed=4000, seed_sets=2):
     """Accuracy reference for the street batch: the oracle's ComputePatchMatch
     and the GPU engine on the same crops (side x side from the centre of the
     first `jobs` reference views, K shifted, all of the job's neighbours),
-    one oracle run per host thread."""
+    for `seed_sets` sets of seeds, one oracle run per host thread."""
     import oracle
     from opensfm_amd import DepthmapEstimator, scenes
     crops = []
@@ -185,25 +185,42 @@ def street_crops_oracle(sc, patch, iterations, jobs=8, side=160, seed=4000):
         est.set_seed(s)
         return est
 
-    cpu = [None] * len(crops)
+    runs = [(k, seed + 1000 * ss + k) for ss in range(seed_sets) for k in range(len(crops))]
+    cpu = [None] * len(runs)
+    nthreads = max(1, min(16, len(os.sched_getaffinity(0))))
 
-    def worker(k):
-        d = setup(oracle.OracleEstimator(), crops[k][0], seed + k).compute_patch_match()[0]
-        cpu[k] = scenes.depth_accuracy(d, crops[k][1], sc.min_depth, sc.max_depth)
+    def acc(d, k):
+        """(whole crop, at least 20 px from its edges)"""
+        gt = crops[k][1]
+        return (scenes.depth_accuracy(d, gt, sc.min_depth, sc.max_depth),
+                scenes.depth_accuracy(d[20:-20, 20:-20], gt[20:-20, 20:-20], sc.min_depth,
+                                      sc.max_depth))
+
+    def worker(w):
+        for r in range(w, len(runs), nthreads):
+            k, s = runs[r]
+            cpu[r] = acc(setup(oracle.OracleEstimator(), crops[k][0], s).compute_patch_match()[0], k)
 
     t0 = time.perf_counter()
-    ths = [threading.Thread(target=worker, args=(k,)) for k in range(len(crops))]
+    ths = [threading.Thread(target=worker, args=(w,)) for w in range(nthreads)]
     for t in ths:
         t.start()
     for t in ths:
         t.join()
     cpu_s = time.perf_counter() - t0
-    gpu = [scenes.depth_accuracy(setup(DepthmapEstimator(), v, seed + k).compute_patch_match()[0],
-                                 gt, sc.min_depth, sc.max_depth)
-           for k, (v, gt) in enumerate(crops)]
-    return {"crops": len(crops), "side": side, "oracle_accuracy": round(float(np.mean(cpu)), 4),
-            "gpu_accuracy": round(float(np.mean(gpu)), 4),
-            "gpu_at_least_oracle": bool(np.mean(gpu) >= np.mean(cpu)),
+    gpu = [acc(setup(DepthmapEstimator(), crops[k][0], s).compute_patch_match()[0], k)
+           for k, s in runs]
+    cpu, gpu = np.array(cpu), np.array(gpu)
+    diff = gpu - cpu
+    se = diff.std(axis=0, ddof=1) / np.sqrt(len(diff))
+    return {"crops": len(crops), "side": side, "seed_sets": seed_sets,
+            "oracle_accuracy": round(float(cpu[:, 0].mean()), 4),
+            "gpu_accuracy": round(float(gpu[:, 0].mean()), 4),
+            "gpu_minus_oracle_se": round(float(se[0]), 4),
+            "oracle_accuracy_interior": round(float(cpu[:, 1].mean()), 4),
+            "gpu_accuracy_interior": round(float(gpu[:, 1].mean()), 4),
+            "gpu_minus_oracle_se_interior": round(float(se[1]), 4),
+            "gpu_at_least_oracle": bool(gpu[:, 0].mean() >= cpu[:, 0].mean()),
             "oracle_s": round(cpu_s, 1)}
 
 
@@ -215,7 +232,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
     within 0.2 points of the most accurate), then the cleaner with 2
     consistent views.  Compared with one DepthmapEstimator run per view on
     the same GPU; the cleaner with the oracle's one-thread Clean on the first
-    4 views' depthmaps; accuracy with the oracle's on crops of 8 views."""
+    4 views' depthmaps; accuracy with the oracle's on crops of 16 views, 2 seed sets."""
     from opensfm_amd import DepthmapBatch, DepthmapClusterEstimator, DepthmapEstimator, scenes
     from opensfm_amd._capi import PM_MODE_PATCHMATCH, TILE_NAMES
     log(f"batched workload: rendering {n_views} street views on the GPU")
