@@ -510,15 +510,14 @@ __device__ __forceinline__ float score_exact(const RefPatch<P>& rp,
 // which never wins a strict comparison.
 
 template <int P, class Fetch>
-__device__ __forceinline__ float score_fast_affine(const RefPatch<P>& rp,
-                                                   const Warp& wp,
-                                                   const Fetch& fetch_texel) {
+__device__ __forceinline__ float score_fast_loop(const RefPatch<P>& rp, const Warp& wp,
+                                                 const Fetch& fetch_texel) {
   constexpr int hpz = (P - 1) / 2;
   const float Hx0 = wp.Hx0, Hy0 = wp.Hy0;
   const float dfdx_x = wp.dfdx_x, dfdx_y = wp.dfdx_y;
   const float dfdy_x = wp.dfdy_x, dfdy_y = wp.dfdy_y;
-  // One patch row of gathers in flight at a time (a two-row pipeline was
-  // measured no faster and costs ~20 registers).
+  // One patch row of gathers in flight at a time (score_fast_pipe keeps two,
+  // for kernels with the registers to spare).
   float sumy = 0, sumyy = 0, sumxy = 0, shift = 0;
 #pragma unroll
   for (int dy = -hpz; dy <= hpz; ++dy) {
@@ -550,6 +549,75 @@ __device__ __forceinline__ float score_fast_affine(const RefPatch<P>& rp,
   if (vary < 0.1f) return -1.0f;
   const float cov = __builtin_fmaf(-rp.meanx, meany, meanxy);
   return cov * rsqrtf(rp.varx * vary);
+}
+
+// kPipe: the gathers of patch row r + 1 are issued before row r is blended,
+// so that two rows of gathers are in flight (the texels, and the sample
+// coordinates their fractions come from, of the next row stay live: about
+// 4 P more registers).  The same arithmetic in the same order, so the same
+// score bit for bit.
+template <int P, class Fetch>
+__device__ __forceinline__ float score_fast_pipe(const RefPatch<P>& rp, const Warp& wp,
+                                                 const Fetch& fetch_texel) {
+  constexpr int hpz = (P - 1) / 2;
+  using Tex = decltype(fetch_texel(0.0f, 0.0f));
+  struct Row {
+    Tex t[P];
+    float x[P], y[P];
+  };
+  const float Hx0 = wp.Hx0, Hy0 = wp.Hy0;
+  const float dfdx_x = wp.dfdx_x, dfdx_y = wp.dfdx_y;
+  const float dfdy_x = wp.dfdy_x, dfdy_y = wp.dfdy_y;
+  Row buf[2];
+  auto fetch = [&](Row& f, int dy) {
+    const float rx = __builtin_fmaf(dfdy_x, (float)dy, Hx0);
+    const float ry = __builtin_fmaf(dfdy_y, (float)dy, Hy0);
+#pragma unroll
+    for (int dx = -hpz; dx <= hpz; ++dx) {
+      const float x2 = __builtin_fmaf(dfdx_x, (float)dx, rx);
+      const float y2 = __builtin_fmaf(dfdx_y, (float)dx, ry);
+      f.t[dx + hpz] = fetch_texel(x2, y2);
+      f.x[dx + hpz] = x2;
+      f.y[dx + hpz] = y2;
+    }
+  };
+  float sumy = 0, sumyy = 0, sumxy = 0, shift = 0;
+  fetch(buf[0], -hpz);
+#pragma unroll
+  for (int r = 0; r < P; ++r) {
+    if (r + 1 < P) fetch(buf[(r + 1) & 1], r + 1 - hpz);
+    __builtin_amdgcn_sched_barrier(0);
+    const Row& f = buf[r & 1];
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      const int k = r * P + c;
+      const float yr = Fetch::bilerp(f.t[c], __builtin_amdgcn_fractf(f.x[c]),
+                                     __builtin_amdgcn_fractf(f.y[c]));
+      if (k == 0) shift = yr;
+      const float y = yr - shift;
+      const float wy = rp.w[k] * y;
+      sumy += wy;
+      sumyy = __builtin_fmaf(wy, y, sumyy);
+      sumxy = __builtin_fmaf(rp.wx[k], y, sumxy);
+    }
+    asm volatile("" : "+v"(sumy), "+v"(sumyy), "+v"(sumxy));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const float meany = sumy * rp.rsumw;
+  const float meanyy = sumyy * rp.rsumw;
+  const float meanxy = sumxy * rp.rsumw;
+  const float vary = __builtin_fmaf(-meany, meany, meanyy);
+  if (vary < 0.1f) return -1.0f;
+  const float cov = __builtin_fmaf(-rp.meanx, meany, meanxy);
+  return cov * rsqrtf(rp.varx * vary);
+}
+
+template <int P, bool kPipe = false, class Fetch>
+__device__ __forceinline__ float score_fast_affine(const RefPatch<P>& rp,
+                                                   const Warp& wp,
+                                                   const Fetch& fetch_texel) {
+  if constexpr (kPipe) return score_fast_pipe<P>(rp, wp, fetch_texel);
+  return score_fast_loop<P>(rp, wp, fetch_texel);
 }
 
 }  // namespace pm

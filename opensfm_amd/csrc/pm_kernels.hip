@@ -317,22 +317,22 @@ __device__ __forceinline__ bool warp_to_view(const PmView& V,
 // Fast NCC of one view under a warp, with the view's texture in the given
 // sample precision (0 FP32 from FP16 coefficients, 1 FP16, 2 BF16,
 // kPrecFp32Quads FP32 from u8 quads).
-template <int P, int Prec>
+template <int P, int Prec, bool kPipe = false>
 __device__ __forceinline__ float score_view_fast(const RefPatch<P>& rp,
                                                  const Warp& wp,
                                                  const PmView& V) {
   if constexpr (Prec == 0) {
     CoefFetch g;
     g.tex = make_tex16(V.quadc, V.width, V.height);
-    return score_fast_affine<P>(rp, wp, g);
+    return score_fast_affine<P, kPipe>(rp, wp, g);
   } else if constexpr (Prec == kPrecFp32Quads) {
     QuadFetch g;
     g.tex = make_tex(V.quad, V.width, V.height);
-    return score_fast_affine<P>(rp, wp, g);
+    return score_fast_affine<P, kPipe>(rp, wp, g);
   } else {
     Global16Fetch<Prec == 2> g;
     g.tex = make_tex16(V.quad16, V.width, V.height);
-    return score_fast_affine<P>(rp, wp, g);
+    return score_fast_affine<P, kPipe>(rp, wp, g);
   }
 }
 
@@ -340,7 +340,7 @@ __device__ __forceinline__ float score_view_fast(const RefPatch<P>& rp,
 // with the lowest view index winning ties (ComputePlaneScore,
 // depthmap.cc:408-419).  Returns the score; *nb gets the view and *depth the
 // plane's depth at the pixel.
-template <int P, int Prec>
+template <int P, int Prec, bool kPipe = false>
 __device__ __forceinline__ float score_plane_all(const PmArgs& a,
                                                  const RefPatch<P>& rp,
                                                  const PixelRay& r, int i,
@@ -356,7 +356,7 @@ __device__ __forceinline__ float score_plane_all(const PmArgs& a,
     const PmView& V = a.views[v];
     Warp wp;
     float s = -1.0f;
-    if (warp_to_view(V, p, fj, fi, &wp)) s = score_view_fast<P, Prec>(rp, wp, V);
+    if (warp_to_view(V, p, fj, fi, &wp)) s = score_view_fast<P, Prec, kPipe>(rp, wp, V);
     if (s > best) {
       best = s;
       bn = v;
@@ -654,10 +654,17 @@ constexpr float kExploreBelow = 0.97f;
 // Tile (tile_x, tile_y) of TX x TY lanes covers columns [2 TX tile_x,
 // 2 TX (tile_x + 1)) and rows [TY tile_y, TY (tile_y + 1)); each lane takes
 // the pixel of the given colour in its pair of columns.
-template <int P, bool kSample, int TX, int TY, int Prec>
+// kLean (the single-view kernel): the propagation candidates seen so far in
+// LDS (hist[m * 256 + threadIdx.x]) instead of registers.  The batched
+// kernel keeps them in registers: with the history in LDS it ran at 3 waves
+// per SIMD and 26-38 % slower.  Patch 7 on the FP16 coefficients scores with
+// two patch rows of gathers in flight (kPipe: converged launches 3-4 %
+// faster; on the u8 quads of the first iteration it was slower, and at patch
+// 5 its registers cost the batched kernel its third wave per SIMD).
+template <int P, bool kSample, int TX, int TY, int Prec, bool kLean = false>
 __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
                                                int iter, int tile_x,
-                                               int tile_y) {
+                                               int tile_y, float4* __restrict__ hist) {
   const int cx = tile_x * TX + (int)(threadIdx.x % TX);
   const int i = tile_y * TY + (int)(threadIdx.x / TX);
   const int j = 2 * cx + ((i + colour) & 1);
@@ -679,10 +686,11 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   // random other view), so the scorer is inlined once.  In sample mode every candidate is scored against one view
   // only (depthmap.cc:325-373): a propagated plane against its source
   // pixel's view, a refinement against the pixel's view after propagation.
+  constexpr bool kPipe = Prec == 0 && P == 7;
   const int ncand = 8 + a.n_explore + a.n_refine + (kSample ? 1 : 0);
   unsigned tries = 0;
   float depth_range = 0.02f, normal_range = 0.5f;
-  float tx[8], ty[8], tz[8];  // propagation planes seen so far
+  float tx[8], ty[8], tz[8];  // propagation planes seen so far (not kLean)
   int tvw[8];                 // and their views (sample mode)
   int cur_view = best_n;      // current_nghbr of depthmap.cc:336
   for (int c = 0; c < ncand; ++c) {
@@ -722,17 +730,26 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
       // strict comparison: skip it.
       bool dup = v0 == cur.x && v1 == cur.y && v2 == cur.z &&
                  (!kSample || view == best_n);
+      if constexpr (kLean) {
+        for (int m = 0; m < c; ++m) {
+          const float4 h = hist[m * 256 + threadIdx.x];
+          dup |= h.x == v0 && h.y == v1 && h.z == v2 &&
+                 (!kSample || __float_as_int(h.w) == view);
+        }
+        hist[c * 256 + threadIdx.x] = make_float4(v0, v1, v2, __int_as_float(view));
+      } else {
 #pragma unroll
-      for (int m = 0; m < 7; ++m)
-        dup |= m < c && tx[m] == v0 && ty[m] == v1 && tz[m] == v2 &&
-               (!kSample || tvw[m] == view);
+        for (int m = 0; m < 7; ++m)
+          dup |= m < c && tx[m] == v0 && ty[m] == v1 && tz[m] == v2 &&
+                 (!kSample || tvw[m] == view);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        if (m == c) {
-          tx[m] = v0;
-          ty[m] = v1;
-          tz[m] = v2;
-          tvw[m] = view;
+        for (int m = 0; m < 8; ++m) {
+          if (m == c) {
+            tx[m] = v0;
+            ty[m] = v1;
+            tz[m] = v2;
+            tvw[m] = view;
+          }
         }
       }
       go = ok && nb.w != 0.0f && !dup;
@@ -860,7 +877,8 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
     } else if (go) {
       int nb;
       float dep;
-      const float s = score_plane_all<P, Prec>(a, rp, ray, i, j, v0, v1, v2, &nb, &dep);
+      const float s =
+          score_plane_all<P, Prec, kPipe>(a, rp, ray, i, j, v0, v1, v2, &nb, &dep);
       tries += (unsigned)(a.nviews - 1);
       if (s > best_s) {
         best_s = s;
@@ -1564,6 +1582,7 @@ __global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour, int iter,
   const int tiles_x = ((a.W + 1) / 2 + 15) >> 4;
   const int tiles_y = (a.H + 15) >> 4;
   const int home = (int)(blockIdx.x & 7u);
+  __shared__ float4 hist[8 * 256];
   unsigned n = 0;
   for (int s = 0; s < 8; ++s) {
     const int band = (home + s) & 7;
@@ -1574,9 +1593,9 @@ __global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour, int iter,
       if (u >= rows * tiles_x) break;
       const int blk = u / per_block, rem = u - blk * per_block;
       const int bw = min(kBandBlockCols, tiles_x - blk * kBandBlockCols);
-      n += half_pixel<P, kSample, 16, 16, Prec>(a, colour, iter,
-                                                blk * kBandBlockCols + rem % bw,
-                                                r0 + rem / bw);
+      n += half_pixel<P, kSample, 16, 16, Prec, true>(a, colour, iter,
+                                                      blk * kBandBlockCols + rem % bw,
+                                                      r0 + rem / bw, hist);
     }
   }
   if (a.counter) wave_count(a.counter, n);
@@ -1608,7 +1627,7 @@ __global__ __launch_bounds__(256) void k_pm_half_batch(
       const int blk = lt / per, rem = lt - blk * per;
       const int bw = min(B, J.half_tx - blk * B);
       n += half_pixel<P, kSample, TX, TY, Prec>(J.a, colour, iter, blk * B + rem % bw,
-                                                rem / bw);
+                                                rem / bw, nullptr);
     }
   }
   if (count) wave_count(count, n);
