@@ -551,12 +551,12 @@ __device__ __forceinline__ float score_fast_loop(const RefPatch<P>& rp, const Wa
   return cov * rsqrtf(rp.varx * vary);
 }
 
-// kPipe: the gathers of patch row r + 1 are issued before row r is blended,
-// so that two rows of gathers are in flight (the texels, and the sample
-// coordinates their fractions come from, of the next row stay live: about
-// 4 P more registers).  The same arithmetic in the same order, so the same
-// score bit for bit.
-template <int P, class Fetch>
+// D rows of gathers in flight: the gathers of patch row r + D - 1 are issued
+// before row r is blended (the texels of the rows ahead, and the sample
+// coordinates their fractions come from, stay live: about 4 P registers per
+// extra row).  The same arithmetic in the same order, so the same score bit
+// for bit.
+template <int P, int D, class Fetch>
 __device__ __forceinline__ float score_fast_pipe(const RefPatch<P>& rp, const Warp& wp,
                                                  const Fetch& fetch_texel) {
   constexpr int hpz = (P - 1) / 2;
@@ -568,7 +568,7 @@ __device__ __forceinline__ float score_fast_pipe(const RefPatch<P>& rp, const Wa
   const float Hx0 = wp.Hx0, Hy0 = wp.Hy0;
   const float dfdx_x = wp.dfdx_x, dfdx_y = wp.dfdx_y;
   const float dfdy_x = wp.dfdy_x, dfdy_y = wp.dfdy_y;
-  Row buf[2];
+  Row buf[D];
   auto fetch = [&](Row& f, int dy) {
     const float rx = __builtin_fmaf(dfdy_x, (float)dy, Hx0);
     const float ry = __builtin_fmaf(dfdy_y, (float)dy, Hy0);
@@ -582,12 +582,13 @@ __device__ __forceinline__ float score_fast_pipe(const RefPatch<P>& rp, const Wa
     }
   };
   float sumy = 0, sumyy = 0, sumxy = 0, shift = 0;
-  fetch(buf[0], -hpz);
+#pragma unroll
+  for (int r = 0; r < D - 1; ++r) fetch(buf[r], r - hpz);
 #pragma unroll
   for (int r = 0; r < P; ++r) {
-    if (r + 1 < P) fetch(buf[(r + 1) & 1], r + 1 - hpz);
+    if (r + D - 1 < P) fetch(buf[(r + D - 1) % D], r + D - 1 - hpz);
     __builtin_amdgcn_sched_barrier(0);
-    const Row& f = buf[r & 1];
+    const Row& f = buf[r % D];
 #pragma unroll
     for (int c = 0; c < P; ++c) {
       const int k = r * P + c;
@@ -612,11 +613,12 @@ __device__ __forceinline__ float score_fast_pipe(const RefPatch<P>& rp, const Wa
   return cov * rsqrtf(rp.varx * vary);
 }
 
-template <int P, bool kPipe = false, class Fetch>
+// kPipe: rows of gathers in flight (<= 1: one row at a time).
+template <int P, int kPipe = 1, class Fetch>
 __device__ __forceinline__ float score_fast_affine(const RefPatch<P>& rp,
                                                    const Warp& wp,
                                                    const Fetch& fetch_texel) {
-  if constexpr (kPipe) return score_fast_pipe<P>(rp, wp, fetch_texel);
+  if constexpr (kPipe >= 2) return score_fast_pipe<P, kPipe>(rp, wp, fetch_texel);
   return score_fast_loop<P>(rp, wp, fetch_texel);
 }
 

@@ -317,7 +317,7 @@ __device__ __forceinline__ bool warp_to_view(const PmView& V,
 // Fast NCC of one view under a warp, with the view's texture in the given
 // sample precision (0 FP32 from FP16 coefficients, 1 FP16, 2 BF16,
 // kPrecFp32Quads FP32 from u8 quads).
-template <int P, int Prec, bool kPipe = false>
+template <int P, int Prec, int kPipe = 1>
 __device__ __forceinline__ float score_view_fast(const RefPatch<P>& rp,
                                                  const Warp& wp,
                                                  const PmView& V) {
@@ -340,7 +340,7 @@ __device__ __forceinline__ float score_view_fast(const RefPatch<P>& rp,
 // with the lowest view index winning ties (ComputePlaneScore,
 // depthmap.cc:408-419).  Returns the score; *nb gets the view and *depth the
 // plane's depth at the pixel.
-template <int P, int Prec, bool kPipe = false>
+template <int P, int Prec, int kPipe = 1>
 __device__ __forceinline__ float score_plane_all(const PmArgs& a,
                                                  const RefPatch<P>& rp,
                                                  const PixelRay& r, int i,
@@ -659,8 +659,9 @@ constexpr float kExploreBelow = 0.97f;
 // kernel keeps them in registers: with the history in LDS it ran at 3 waves
 // per SIMD and 26-38 % slower.  Patch 7 on the FP16 coefficients scores with
 // two patch rows of gathers in flight (kPipe: converged launches 3-4 %
-// faster; on the u8 quads of the first iteration it was slower, and at patch
-// 5 its registers cost the batched kernel its third wave per SIMD).
+// faster; three rows were 1 % slower than two, on the u8 quads of the first
+// iteration it was slower, and at patch 5 its registers cost the batched
+// kernel its third wave per SIMD).
 template <int P, bool kSample, int TX, int TY, int Prec, bool kLean = false>
 __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
                                                int iter, int tile_x,
@@ -686,7 +687,7 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
   // random other view), so the scorer is inlined once.  In sample mode every candidate is scored against one view
   // only (depthmap.cc:325-373): a propagated plane against its source
   // pixel's view, a refinement against the pixel's view after propagation.
-  constexpr bool kPipe = Prec == 0 && P == 7;
+  constexpr int kPipe = (Prec == 0 && P == 7) ? 2 : 1;
   const int ncand = 8 + a.n_explore + a.n_refine + (kSample ? 1 : 0);
   unsigned tries = 0;
   float depth_range = 0.02f, normal_range = 0.5f;
