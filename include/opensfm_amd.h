@@ -120,6 +120,17 @@ int pm_estimator_set_seed(pm_estimator* est, uint64_t seed);
 /* Far-propagation probes per direction of the red/black kernel (default 6;
  * 0 = direct neighbours only). */
 int pm_estimator_set_far_probes(pm_estimator* est, int n);
+/* Half-resolution level (not in the reference's legacy estimator; the
+ * reference's OpenCL engine runs a pyramid, estimator.cc:561-658): with
+ * half_iterations > 0, plain PatchMatch first runs half_iterations
+ * iterations on the views box-averaged by 2 (K scaled about pixel centres,
+ * same settings, its own seed), and its planes -- where the depth is live and
+ * the median test keeps it -- seed this level's initialisation (bilinear in
+ * depth and slopes, as an initialisation map would); the rest start from
+ * random planes.  Then set_patchmatch_iterations iterations run here.
+ * Default 0 (off); ignored in sample mode and when an initialisation map
+ * is set. */
+int pm_estimator_set_pyramid(pm_estimator* est, int half_iterations);
 /* Random refinements per pixel update (default 6, the reference's count,
  * depthmap.cc:337; the 0.02 * 0.3^k / 0.5 * 0.8^k schedule restarts every
  * 6 steps). */

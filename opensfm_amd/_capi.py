@@ -73,6 +73,7 @@ SIGNATURES = {
     "pm_estimator_set_min_patch_sd": (C.c_int, [_vp, C.c_float]),
     "pm_estimator_set_seed": (C.c_int, [_vp, C.c_uint64]),
     "pm_estimator_set_far_probes": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_pyramid": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_refinement_steps": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_exploration": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_precision": (C.c_int, [_vp, C.c_int]),

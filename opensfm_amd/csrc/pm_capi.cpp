@@ -7,6 +7,8 @@
 // 480-487).
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <cmath>
 #include <string.h>
 
 #include <algorithm>
@@ -78,6 +80,11 @@ struct pm_estimator {
   int coarse_scale = 0;
   std::vector<std::unique_ptr<HostView>> coarse;
   DevBuf coarse_viewtab;
+  // half-resolution level (set_pyramid): its own estimator over the views
+  // box-averaged by 2, run first; its state seeds this one's initialisation
+  int pyramid_iters = 0;
+  std::unique_ptr<pm_estimator> half;
+  DevBuf half_map;  // the half level's state as an initialisation map
 
   ~pm_estimator() {
     for (auto& e : ev)
@@ -259,6 +266,84 @@ hipError_t consistency_pass(pm_estimator* e, const pm::PmArgs& a0, bool sample, 
                                items, e->stream);
 }
 
+// The half-resolution level's estimator: every view box-averaged by 2
+// (K scaled about pixel centres, x_half = (x + 0.5) / 2 - 0.5; poses
+// unchanged), the reference's mask the minimum of each 2 x 2 block.  Built on
+// the device from the uploaded views, and rebuilt when views are added.
+void ensure_half(pm_estimator* e) {
+  if (e->half && e->half->views.size() == e->views.size()) return;
+  std::unique_ptr<pm_estimator> h(new pm_estimator());
+  h->device = e->device;
+  HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (auto& ev : h->ev) HIP_CHECK(hipEventCreate(&ev));
+  for (size_t k = 0; k < e->views.size(); ++k) {
+    const HostView& V = *e->views[k];
+    std::unique_ptr<HostView> c(new HostView());
+    memcpy(c->R, V.R, sizeof(c->R));
+    memcpy(c->t, V.t, sizeof(c->t));
+    memcpy(c->Q, V.Q, sizeof(c->Q));
+    memcpy(c->a, V.a, sizeof(c->a));
+    memcpy(c->K, V.K, sizeof(c->K));
+    for (int r = 0; r < 2; ++r) {
+      for (int col = 0; col < 3; ++col) c->K[r * 3 + col] *= 0.5;
+      c->K[r * 3 + 2] -= 0.25 * V.K[8];
+    }
+    if (!inv3(c->K, c->Kinv)) throw std::runtime_error("half-level K is singular");
+    c->width = V.width / 2;
+    c->height = V.height / 2;
+    if (c->width < 2 || c->height < 2) throw std::runtime_error("view too small for a half level");
+    const size_t np = (size_t)c->width * c->height;
+    c->img.ensure(np);
+    HIP_CHECK(pm::launch_downsample_u8(V.img.as<uint8_t>(), V.width, V.height, 2, 0,
+                                       c->img.as<uint8_t>(), c->width, c->height, h->stream));
+    c->quad.ensure((size_t)(c->width + 1) * (c->height + 1) * sizeof(uint32_t));
+    c->quadc.ensure((size_t)(c->width + 1) * (c->height + 1) * 8);
+    HIP_CHECK(pm::launch_build_quad(c->img.as<uint8_t>(), c->quad.as<uint32_t>(),
+                                    c->quadc.as<uint32_t>(), c->width, c->height, h->stream));
+    if (k == 0) {
+      h->mask0.ensure(np);
+      HIP_CHECK(pm::launch_downsample_u8(e->mask0.as<uint8_t>(), V.width, V.height, 2, 1,
+                                         h->mask0.as<uint8_t>(), c->width, c->height,
+                                         h->stream));
+    }
+    h->views.push_back(std::move(c));
+  }
+  HIP_CHECK(hipStreamSynchronize(h->stream));
+  e->half = std::move(h);
+}
+
+int run(pm_estimator* e, int mode);
+
+// Runs the half level (same settings, e->pyramid_iters iterations, its own
+// seed) and turns its state into e->half_map: per half-level pixel the
+// normalised log depth, the normal's slopes and a confidence logit (+10 where
+// the depth is live and PostProcess's median test keeps it, else -10).
+int run_half(pm_estimator* e) {
+  ensure_half(e);
+  pm_estimator* h = e->half.get();
+  h->patch_size = e->patch_size;
+  h->min_depth = e->min_depth;
+  h->max_depth = e->max_depth;
+  h->num_depth_planes = e->num_depth_planes;
+  h->iterations = e->pyramid_iters;
+  h->min_patch_variance = e->min_patch_variance;
+  h->seed = e->seed ^ 0x9E3779B97F4A7C15ull;
+  h->n_far = e->n_far;
+  h->n_refine = e->n_refine;
+  h->n_explore = e->n_explore;
+  h->precision = e->precision;
+  h->post_process = e->post_process;
+  h->consistency = e->consistency;
+  int rc = run(h, PM_MODE_PATCHMATCH);
+  if (rc) return rc;
+  const HostView& r = *h->views[0];
+  e->half_map.ensure((size_t)r.width * r.height * 4 * sizeof(float));
+  HIP_CHECK(pm::launch_state_to_map(h->state.as<float>(), r.width, r.height,
+                                    (float)std::log(e->min_depth), (float)std::log(e->max_depth),
+                                    e->post_process, e->half_map.as<float>(), e->stream));
+  return PM_OK;
+}
+
 int run(pm_estimator* e, int mode) {
   int rc = validate_for_compute(e);
   if (rc) return rc;
@@ -269,7 +354,24 @@ int run(pm_estimator* e, int mode) {
     return fail(PM_ERR_INVALID, "FP16/BF16 samples need patch size 5 or 7");
   DeviceGuard dg(e->device);
   e->have_result = false;
+  // the half level seeds plain PatchMatch when no initialisation map is set
+  const bool pyramid = mode == PM_MODE_PATCHMATCH && e->pyramid_iters > 0 && !e->init_map;
+  if (pyramid) {
+    rc = run_half(e);
+    if (rc) return rc;
+  }
   pm::PmArgs a = prepare(e);
+  if (pyramid) {
+    const HostView& hr = *e->half->views[0];
+    a.init_map = e->half_map.as<float>();
+    a.init_w = hr.width;
+    a.init_h = hr.height;
+    a.init_stride = 4;
+    a.init_scale = 2;
+    a.init_search = 1;
+    a.init_sigma = 0.0f;
+    a.init_sigma_slope = 0.0f;
+  }
   const HostView& r = *e->views[0];
   const int n = (int)e->views.size();
   float dom_ms = 0.0f;
@@ -755,6 +857,13 @@ int pm_estimator_set_precision(pm_estimator* e, int precision) {
   if (precision < PM_PRECISION_FP32 || precision > PM_PRECISION_BF16)
     return fail(PM_ERR_INVALID, "unknown precision " + std::to_string(precision));
   e->precision = precision;
+  return PM_OK;
+}
+
+int pm_estimator_set_pyramid(pm_estimator* e, int half_iterations) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (half_iterations < 0) return fail(PM_ERR_INVALID, "half-level iterations must be >= 0");
+  e->pyramid_iters = half_iterations;
   return PM_OK;
 }
 

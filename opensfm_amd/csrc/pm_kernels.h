@@ -27,6 +27,10 @@ hipError_t launch_score_hyp(const PmArgs& a, int view, int n, const int* ij,
                             const float* Hs, float* out, hipStream_t s);
 hipError_t launch_pm_init(const PmArgs& a, bool sample, hipStream_t s);
 // queues: 8 zeroed counters (the band queues of one launch).
+// A level's state (W x H float4) as a 4-float-per-pixel initialisation map
+// for the next finer level (k_state_to_map).
+hipError_t launch_state_to_map(const float* state, int W, int H, float log_min, float log_max,
+                               int median, float* map, hipStream_t s);
 hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
                           unsigned* queues, hipStream_t s);
 // Box downsampling (mode 0 mean, 1 min) by an integer factor.

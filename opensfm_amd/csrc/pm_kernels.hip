@@ -934,6 +934,34 @@ __device__ __forceinline__ bool median_keeps(float d, float m) {
   return d != 0.0f && !(__fdiv_rn(fabsf(d - m), d) > 0.05);
 }
 
+// ---------------------------------------------------------------------------
+// A level's PatchMatch state as an initialisation map for the next finer
+// level (PmArgs::init_map, 4 floats per pixel): t = (log depth - log_min) /
+// (log_max - log_min), the slopes (sx, sy) of the plane's normal (sx, sy, -1)
+// (plane v: sx = -v0 / v2, sy = -v1 / v2) and a confidence logit, +10 where
+// the depth is live (and, with `median`, PostProcess's test keeps it), else
+// -10 (the finer level starts those pixels from random planes).
+
+__global__ __launch_bounds__(256) void k_state_to_map(const float4* __restrict__ st, int W,
+                                                      int H, float log_min, float log_max,
+                                                      int median, float4* __restrict__ map) {
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i >= H || j >= W) return;
+  const size_t idx = (size_t)i * W + j;
+  const float4 s = st[idx];
+  bool ok = s.w > 0.0f && s.z != 0.0f;
+  if (ok && median) ok = median_keeps(s.w, median5_depth(st, W, H, i, j));
+  float4 m = make_float4(0.0f, 0.0f, 0.0f, -10.0f);
+  if (ok) {
+    m.x = (logf(s.w) - log_min) / (log_max - log_min);
+    m.y = -s.x / s.z;
+    m.z = -s.y / s.z;
+    m.w = 10.0f;
+  }
+  map[idx] = m;
+}
+
 __device__ __forceinline__ void pack_pixel(const PmArgs& a, int median, int i,
                                            int j, float* __restrict__ out_depth,
                                            float* __restrict__ out_plane,
@@ -2015,6 +2043,14 @@ hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
   PM_DISPATCH_PREC(a.precision, P_,
                    return (half_banded<P, false, Prec>(a, colour, iter, queues, s)));
   return hipSuccess;
+}
+
+hipError_t launch_state_to_map(const float* state, int W, int H, float log_min, float log_max,
+                               int median, float* map, hipStream_t s) {
+  hipLaunchKernelGGL(k_state_to_map, grid16(W, H), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(state), W, H, log_min, log_max, median,
+                     reinterpret_cast<float4*>(map));
+  return hipGetLastError();
 }
 
 hipError_t launch_downsample_u8(const uint8_t* in, int W, int H, int sc,

@@ -87,6 +87,11 @@ class DepthmapEstimator:
     def set_seed(self, seed: int) -> None:
         _capi.check(self._lib.pm_estimator_set_seed(self._h, int(seed) & (2 ** 64 - 1)))
 
+    def set_pyramid(self, half_iterations: int) -> None:
+        """Plain PatchMatch seeded by half_iterations iterations at half
+        resolution (0: off; not in the reference's legacy estimator)."""
+        _capi.check(self._lib.pm_estimator_set_pyramid(self._h, int(half_iterations)))
+
     def set_far_probes(self, n: int) -> None:
         _capi.check(self._lib.pm_estimator_set_far_probes(self._h, int(n)))
 

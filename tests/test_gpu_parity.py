@@ -197,6 +197,46 @@ def test_patchmatch_full_view_accuracy_vs_oracle():
     assert score >= want["mean_score"]
 
 
+def test_pyramid_full_view_accuracy_vs_oracle():
+    """The half-resolution level (set_pyramid: 5 iterations on the views
+    box-averaged by 2 seed the full level) and 2 full-resolution iterations
+    on the 1600x1200 + 8-neighbour scene: accuracy and mean score over seeds
+    1-5 at least the oracle's 5-iteration full-view figures."""
+    print("PatchMatch with the half-resolution level vs oracle, 1600x1200 full view", flush=True)
+    want = _golden("config2_oracle_fullview.json")
+    s = scenes.scene_config2()
+    acc, score = _gpu_runs(s, (1, 2, 3, 4, 5), 2, planes=50, pyramid=5)
+    print(f"oracle acc {want['mean_accuracy_within_1pct']:.5f} score {want['mean_score']:.5f} | "
+          f"gpu acc {acc:.5f} score {score:.5f}", flush=True)
+    assert acc >= want["mean_accuracy_within_1pct"]
+    assert score >= want["mean_score"]
+
+
+def test_pyramid_runs_and_is_deterministic(cfg1_scene):
+    """Half-resolution level on config 1: same seed, same maps; invariants
+    hold; the level is ignored in sample mode (maps equal to those without
+    it) and 0 half-level iterations is plain PatchMatch."""
+    est = _setup(DepthmapEstimator(), cfg1_scene)
+    est.set_seed(3)
+    plain = est.compute_patch_match()
+    plain_s = est.compute_patch_match_sample()
+    est.set_pyramid(3)
+    a = est.compute_patch_match()
+    b = est.compute_patch_match()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    d, p, sc, n = a
+    live = d > 0
+    assert np.all(np.isfinite(d)) and live.mean() > 0.8
+    assert np.all((sc[live] >= -1) & (sc[live] <= 1 + 1e-5))
+    assert scenes.depth_accuracy(d, cfg1_scene.ref.depth, 2.0, 20.0) > 0.88
+    for x, y in zip(est.compute_patch_match_sample(), plain_s):
+        assert np.array_equal(x, y)
+    est.set_pyramid(0)
+    for x, y in zip(est.compute_patch_match(), plain):
+        assert np.array_equal(x, y)
+
+
 def test_reference_candidate_list_runs(cfg1_scene):
     """Exploration and the median-consistent re-selection off (the
     reference's candidate list on red/black updates): valid maps."""
