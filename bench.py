@@ -3,10 +3,14 @@
 Workload (one "step" = one full ComputePatchMatch of the reference view):
 1600x1200 uint8 reference view + 8 neighbours rendered from the seeded
 scene generator (20 boxes on a sinusoidal terrain of amplitude 1.5 and
-period 6), depth range [1, 40], patch 7x7, min patch SD 1, 5 iterations,
-FP32 samples and FP32 NCC.  Inputs are resident in HBM when the timed region
-starts (views uploaded by add_view); each step runs random init, 5 red/black
-iterations and the median post-process into device-resident output maps.
+period 6), depth range [1, 40], patch 7x7, min patch SD 1, FP32 samples and
+FP32 NCC.  Inputs are resident in HBM when the timed region starts (views
+uploaded by add_view); each step runs the schedule that reaches the
+reference's 5-iteration accuracy and mean score (checked over seeds 1-5
+against the oracle's recorded figures): 5 iterations at half resolution
+(the views box-averaged by 2), whose planes seed 2 full-resolution
+iterations, and the median post-process, into device-resident output maps.
+Plain PatchMatch at the reference's 5 iterations is timed beside it.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the red/black
 PatchMatch half-iteration), timed with HIP events on the stream it runs on;
@@ -37,7 +41,8 @@ FLOP_PER_SCORING = 160
 
 WORKLOAD = dict(width=1600, height=1200, neighbours=8, boxes=20,
                 terrain=(1.5, 6.0), min_depth=1.0, max_depth=40.0, patch=7,
-                min_patch_sd=1.0, iterations=5, seed=2000)
+                min_patch_sd=1.0, iterations=5, seed=2000,
+                half_iterations=5, full_iterations=2)
 
 
 GOLDEN_CFG2 = os.path.join(HERE, "tests", "golden", "config2_oracle_fullview.json")
@@ -154,11 +159,14 @@ def cpu_baseline(scene, threads, seed_sets=5, target_s=4.0):
     }
 
 
-def street_crops_oracle(sc, patch, iterations, jobs=16, side=160, seed=4000, seed_sets=2):
+def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, seed=4000,
+                        seed_sets=2):
     """Accuracy reference for the street batch: the oracle's ComputePatchMatch
-    and the GPU engine on the same crops (side x side from the centre of the
-    first `jobs` reference views, K shifted, all of the job's neighbours),
-    for `seed_sets` sets of seeds, one oracle run per host thread."""
+    at `iterations` and the GPU engine with gpu_schedule = (half-resolution
+    iterations, full iterations) on the same crops (side x side from the
+    centre of the first `jobs` reference views, K shifted, all of the job's
+    neighbours), for `seed_sets` sets of seeds, one oracle run per host
+    thread."""
     import oracle
     from opensfm_amd import DepthmapEstimator, scenes
     crops = []
@@ -208,8 +216,13 @@ def street_crops_oracle(sc, patch, iterations, jobs=16, side=160, seed=4000, see
     for t in ths:
         t.join()
     cpu_s = time.perf_counter() - t0
-    gpu = [acc(setup(DepthmapEstimator(), crops[k][0], s).compute_patch_match()[0], k)
-           for k, s in runs]
+    def gpu_est(views, s):
+        e = setup(DepthmapEstimator(), views, s)
+        e.set_pyramid(gpu_schedule[0])
+        e.set_patchmatch_iterations(gpu_schedule[1])
+        return e
+
+    gpu = [acc(gpu_est(crops[k][0], s).compute_patch_match()[0], k) for k, s in runs]
     cpu, gpu = np.array(cpu), np.array(gpu)
     diff = gpu - cpu
     se = diff.std(axis=0, ddof=1) / np.sqrt(len(diff))
@@ -224,13 +237,18 @@ def street_crops_oracle(sc, patch, iterations, jobs=16, side=160, seed=4000, see
             "oracle_s": round(cpu_s, 1)}
 
 
+BATCH_SCHEDULE = (5, 2)  # half-resolution iterations, full iterations (street batch)
+
+
 def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
     """Batched street workload: n_views reference views of sizes drawn from
     {640x480, 1024x768, 1600x1200, 2048x1536}, 4-12 neighbours each
-    (rendered on the GPU), 3 iterations in one captured hipGraph, tile
+    (rendered on the GPU), 5 iterations at half resolution seeding 2 at full
+    resolution (the reference's count is 3; accuracy checked against the
+    oracle's 3 iterations on crops) in one captured hipGraph, tile
     autotuned by speed and patch size {5, 7} gated on accuracy (the fastest
     within 0.2 points of the most accurate), then the cleaner with 2
-    consistent views.  Compared with one DepthmapEstimator run per view on
+    consistent views.  Plain 3-iteration PatchMatch is timed beside it.  Compared with one DepthmapEstimator run per view on
     the same GPU; the cleaner with the oracle's one-thread Clean on the first
     4 views' depthmaps; accuracy with the oracle's on crops of 16 views, 2 seed sets."""
     from opensfm_amd import DepthmapBatch, DepthmapClusterEstimator, DepthmapEstimator, scenes
@@ -245,7 +263,8 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         b.add_view(v.K, v.R, v.t, v.image, v.mask)
     for ref, nb in sc.jobs:
         b.add_job(ref, nb, sc.min_depth, sc.max_depth)
-    b.set_patchmatch_iterations(3)
+    b.set_pyramid(BATCH_SCHEDULE[0])
+    b.set_patchmatch_iterations(BATCH_SCHEDULE[1])
     b.set_min_patch_sd(1.0)
     b.set_seed(seed)
 
@@ -269,6 +288,19 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         d, _, s, _ = b.result(k)
         accs.append(scenes.depth_accuracy(d, sc.views[ref].depth, sc.min_depth, sc.max_depth))
         scores.append(float(s[d > 0].mean()) if (d > 0).any() else 0.0)
+    # plain PatchMatch at the reference's 3 iterations, same patch and tile
+    b.set_pyramid(0)
+    b.set_patchmatch_iterations(3)
+    b.run()
+    t0 = time.perf_counter()
+    b.run()
+    plain_s = time.perf_counter() - t0
+    plain3 = {"iterations": 3, "mpix_s": round(npx / plain_s / 1e6, 3),
+              "accuracy_within_1pct": round(accuracy(b), 4)}
+    log(f"street batch, plain 3 iterations: {plain3}")
+    b.set_pyramid(BATCH_SCHEDULE[0])
+    b.set_patchmatch_iterations(BATCH_SCHEDULE[1])
+    b.run()
     b.clean(0.01, 2)
     clean_ms = b.stats()["clean_ms"]
     kept, kept_good = 0, 0
@@ -299,7 +331,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         if cpu_px:
             clean_cpu = {"views": n_cmp, "mpix_s": round(cpu_px / cpu_s / 1e6, 3), "cores": 1,
                          "kind": "port", "bit_exact": same}
-        crop_ref = street_crops_oracle(sc, patch, 3)
+        crop_ref = street_crops_oracle(sc, patch, 3, BATCH_SCHEDULE)
         log(f"street crops vs oracle: {crop_ref}")
     # one estimator per view, device-resident runs, same settings
     per_view_s = 0.0
@@ -309,7 +341,8 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
             V = sc.views[v]
             e.add_view(V.K, V.R, V.t, V.image, V.mask)
         e.set_depth_range(sc.min_depth, sc.max_depth, 50)
-        e.set_patchmatch_iterations(3)
+        e.set_pyramid(BATCH_SCHEDULE[0])
+        e.set_patchmatch_iterations(BATCH_SCHEDULE[1])
         e.set_patch_size(patch)
         e.set_min_patch_sd(1.0)
         e.set_seed(seed + k)
@@ -334,7 +367,9 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
                                                        sc.max_depth)
                                  for r, (ref, _) in zip(cres, sc.jobs)]))
     return {
-        "views": n_views, "mpix": round(npx / 1e6, 2), "iterations": 3,
+        "views": n_views, "mpix": round(npx / 1e6, 2),
+        "half_resolution_iterations": BATCH_SCHEDULE[0], "iterations": BATCH_SCHEDULE[1],
+        "plain_reference_iterations": plain3,
         "patch": patch, "tile": TILE_NAMES[tile], "autotune_s": round(tune_s, 2),
         "autotune": b.tune_report,
         "mpix_s": round(npx / batch_s / 1e6, 3),
@@ -546,7 +581,8 @@ def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
     """Learned initialiser configuration: 64 nadir views of 4000x3000 over
     fractal terrain (an 8 x 8 camera grid, rendered on the GPU), 10
     neighbours each, all in one DepthmapBatch.  Plain PatchMatch from random
-    planes at 5, 2 and 1 iterations against the learned initialiser (network
+    planes at 5, 2 and 1 iterations, and seeded by 5 iterations at half
+    resolution with 2 at full, against the learned initialiser (network
     input from the batch's 1/4-resolution views and 32-plane cost volume,
     the U-Net, then seeded PatchMatch) at 1 and 2 iterations, each timed once
     from device-resident views to device-resident maps."""
@@ -579,8 +615,10 @@ def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
     # warm the network and the coarse views outside the timed region
     init.predict_batch(b, 0, width, height)
     for mode, learned, iters in (("plain_5", False, 5), ("plain_2", False, 2),
-                                 ("plain_1", False, 1), ("learned_1", True, 1),
-                                 ("learned_2", True, 2)):
+                                 ("plain_1", False, 1), ("plain_half5_2", False, 2),
+                                 ("learned_1", True, 1), ("learned_2", True, 2)):
+        # plain_half5_2: 5 iterations at half resolution seeding 2 at full
+        b.set_pyramid(5 if mode == "plain_half5_2" else 0)
         b.set_patchmatch_iterations(iters)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -600,12 +638,13 @@ def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
                 b.set_init_map(k, None)
         log(f"learned initialiser {mode}: {out[mode]}")
     base = out["plain_5"]["mpix_s"]
-    for mode in ("learned_1", "learned_2", "plain_2", "plain_1"):
+    for mode in ("learned_1", "learned_2", "plain_2", "plain_1", "plain_half5_2"):
         out[f"{mode}_over_plain_5"] = round(out[mode]["mpix_s"] / base, 2)
     # the fewest plain iterations reaching plain-5's accuracy, and the
     # learned runs at that accuracy
     target = out["plain_5"]["acc"]
-    eq_plain = next(m for m in ("plain_1", "plain_2", "plain_5") if out[m]["acc"] >= target)
+    eq_plain = next(m for m in ("plain_1", "plain_2", "plain_half5_2", "plain_5")
+                    if out[m]["acc"] >= target)
     eq_learned = [m for m in ("learned_1", "learned_2") if out[m]["acc"] >= target]
     out["equal_accuracy"] = {
         "plain_fewest_iterations": eq_plain,
@@ -677,21 +716,31 @@ def main():
     est.set_depth_range(w["min_depth"], w["max_depth"], 64)
     est.set_patch_size(w["patch"])
     est.set_min_patch_sd(w["min_patch_sd"])
-    est.set_patchmatch_iterations(w["iterations"])
     est.set_seed(w["seed"])
 
-    for k in range(args.warmup):
-        log(f"warmup {k + 1}/{args.warmup}")
-        est.run(PM_MODE_PATCHMATCH)
-    dom_ms, dom_launches, dom_scorings = 0.0, 0, 0
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        est.run(PM_MODE_PATCHMATCH)  # synchronous: returns after the stream drains
-        st = est.stats()
-        dom_ms += st["dominant_ms"]
-        dom_launches += st["launches"]
-        dom_scorings += st["dominant_scorings"]
-    elapsed = time.perf_counter() - t0
+    def schedule(half):
+        """half: the half-resolution level + full iterations that reach the
+        reference's 5-iteration accuracy; else plain PatchMatch at the
+        reference's 5 iterations."""
+        est.set_pyramid(w["half_iterations"] if half else 0)
+        est.set_patchmatch_iterations(w["full_iterations"] if half else w["iterations"])
+
+    def timed(steps, warmup):
+        for _ in range(warmup):
+            est.run(PM_MODE_PATCHMATCH)
+        dom = [0.0, 0, 0]
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            est.run(PM_MODE_PATCHMATCH)  # synchronous: returns after the stream drains
+            st = est.stats()
+            dom[0] += st["dominant_ms"]
+            dom[1] += st["launches"]
+            dom[2] += st["dominant_scorings"]
+        return time.perf_counter() - t0, dom
+
+    schedule(True)
+    log(f"warmup {args.warmup}, {args.steps} timed steps")
+    elapsed, (dom_ms, dom_launches, dom_scorings) = timed(args.steps, args.warmup)
     log(f"timed {args.steps} steps in {elapsed:.3f} s")
     depth, plane, score, nghbr = est.download()
 
@@ -699,6 +748,46 @@ def main():
     t1 = time.perf_counter()
     est.compute_patch_match()
     pcie_s = time.perf_counter() - t1
+
+    # equal accuracy: the same compute over seeds 1-5 against the oracle's
+    # full-view figures for those seeds (recorded by
+    # tests/golden/make_config2_oracle.py, about 1300 s per seed on one core)
+    with open(GOLDEN_CFG2) as f:
+        fix = json.load(f)
+
+    def against_oracle():
+        g_acc, g_score = [], []
+        for r in fix["per_seed"]:
+            est.set_seed(r["seed"])
+            est.run(PM_MODE_PATCHMATCH)
+            d_s, _, s_s, _ = est.download()
+            g_acc.append(scenes.depth_accuracy(d_s, scene.ref.depth, w["min_depth"],
+                                               w["max_depth"]))
+            g_score.append(float(s_s[d_s > 0].mean()))
+        est.set_seed(w["seed"])
+        eq = {"seeds": [r["seed"] for r in fix["per_seed"]],
+              "oracle_accuracy": round(fix["mean_accuracy_within_1pct"], 5),
+              "oracle_mean_score": round(fix["mean_score"], 5),
+              "gpu_accuracy": round(float(np.mean(g_acc)), 5),
+              "gpu_mean_score": round(float(np.mean(g_score)), 5)}
+        eq["met"] = bool(eq["gpu_accuracy"] >= eq["oracle_accuracy"] and
+                         eq["gpu_mean_score"] >= eq["oracle_mean_score"])
+        return eq
+
+    equal = against_oracle()
+    log(f"equal accuracy (full view, 5 seeds): {equal}")
+
+    # plain PatchMatch at the reference's 5 iterations, beside it
+    schedule(False)
+    p_elapsed, _ = timed(max(1, min(args.steps, 3)), 1)
+    p_depth = est.download()[0]
+    plain = {"iterations": w["iterations"],
+             "mpix_s": round(w["width"] * w["height"] * max(1, min(args.steps, 3)) / p_elapsed
+                             / 1e6, 3),
+             "accuracy_within_1pct": round(scenes.depth_accuracy(p_depth, scene.ref.depth,
+                                                                 w["min_depth"], w["max_depth"]), 4),
+             "equal_accuracy": against_oracle()}
+    log(f"plain PatchMatch, 5 iterations: {plain}")
 
     # ComputePlaneSweep over 64 fronto-parallel planes (exact FP32 path)
     est.run(PM_MODE_PLANE_SWEEP)
@@ -721,7 +810,6 @@ def main():
     est.run(PM_MODE_PATCHMATCH)
     upd_s = time.perf_counter() - t3
     upd_depth = est.download()[0]
-    est.set_patchmatch_iterations(w["iterations"])
     matched_updates = {
         "iterations": 2 * w["iterations"],
         "mpix_s": round(w["width"] * w["height"] / upd_s / 1e6, 3),
@@ -730,7 +818,8 @@ def main():
     log(f"matched pixel updates: {matched_updates}")
 
     # ComputePatchMatchSample (depthmap.cc:215-227): each candidate scored
-    # against one view, the same iterations
+    # against one view, the reference's iterations
+    est.set_patchmatch_iterations(w["iterations"])
     est.run(PM_MODE_PATCHMATCH_SAMPLE)
     t5 = time.perf_counter()
     est.run(PM_MODE_PATCHMATCH_SAMPLE)
@@ -742,28 +831,7 @@ def main():
         "accuracy_within_1pct": round(scenes.depth_accuracy(smp_depth, scene.ref.depth,
                                                             w["min_depth"], w["max_depth"]), 4)}
     log(f"sample mode: {sample_mode}")
-
-    # equal accuracy: the same compute over seeds 1-5 against the oracle's
-    # full-view figures for those seeds (recorded by
-    # tests/golden/make_config2_oracle.py, about 1300 s per seed on one core)
-    with open(GOLDEN_CFG2) as f:
-        fix = json.load(f)
-    g_acc, g_score = [], []
-    for r in fix["per_seed"]:
-        est.set_seed(r["seed"])
-        est.run(PM_MODE_PATCHMATCH)
-        d_s, _, s_s, _ = est.download()
-        g_acc.append(scenes.depth_accuracy(d_s, scene.ref.depth, w["min_depth"], w["max_depth"]))
-        g_score.append(float(s_s[d_s > 0].mean()))
-    est.set_seed(w["seed"])
-    equal = {"seeds": [r["seed"] for r in fix["per_seed"]],
-             "oracle_accuracy": round(fix["mean_accuracy_within_1pct"], 5),
-             "oracle_mean_score": round(fix["mean_score"], 5),
-             "gpu_accuracy": round(float(np.mean(g_acc)), 5),
-             "gpu_mean_score": round(float(np.mean(g_score)), 5)}
-    equal["met"] = bool(equal["gpu_accuracy"] >= equal["oracle_accuracy"] and
-                        equal["gpu_mean_score"] >= equal["oracle_mean_score"])
-    log(f"equal accuracy (full view, 5 seeds): {equal}")
+    schedule(True)
 
     npx = w["width"] * w["height"]
     ms_per_step = elapsed / args.steps * 1e3
@@ -789,20 +857,26 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded ray-cast scene, value-noise texture)",
         "config": {"workload": "single-view plain PatchMatch 1600x1200 + 8 neighbours, "
-                               "patch 7, 5 iterations (the reference's count), depth [1, 40], "
-                               "FP32; value = device-resident views to device-resident maps",
+                               "patch 7, depth [1, 40], FP32, at the accuracy and mean score of the "
+                               "reference's 5 iterations: 5 iterations at half resolution seeding "
+                               "2 at full resolution; value = device-resident views to "
+                               "device-resident maps",
                    "width": w["width"], "height": w["height"], "neighbours": w["neighbours"],
-                   "patch": w["patch"], "iterations": w["iterations"]},
+                   "patch": w["patch"], "half_resolution_iterations": w["half_iterations"],
+                   "iterations": w["full_iterations"],
+                   "reference_iterations": w["iterations"]},
         "accuracy_within_1pct": round(acc, 4),
         "equal_accuracy": equal,
+        "plain_reference_iterations": plain,
         "matched_pixel_updates": matched_updates,
         "sample_mode": sample_mode,
         "mean_score": round(mean_score, 4),
         "pcie_inclusive_mpix_s": round(npx / pcie_s / 1e6, 3),
         "plane_sweep": sweep,
         "roofline": {
-            "kernel": "k_pm_half<7> (red/black PatchMatch half-iteration; the first iteration's "
-                      "two launches read the u8 quads, the others the FP16 coefficients)",
+            "kernel": "k_pm_half<7> (red/black PatchMatch half-iteration, every launch of a "
+                      "compute at both levels; random-plane launches read the u8 quads, the "
+                      "others the FP16 coefficients)",
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
@@ -832,11 +906,12 @@ def main():
             ce.set_seed(100 + k)
             crop_est.append(ce)
 
-        def gpu_crops(iterations):
+        def gpu_crops():
             acc, inner = [], []   # inner: at least 20 px from the crop's edges
             for ce, (x0, y0, cw, ch, _) in zip(crop_est, crops):
                 gt = scene.ref.depth[y0:y0 + ch, x0:x0 + cw]
-                ce.set_patchmatch_iterations(iterations)
+                ce.set_pyramid(w["half_iterations"])
+                ce.set_patchmatch_iterations(w["full_iterations"])
                 gd = ce.compute_patch_match()[0]
                 acc.append(scenes.depth_accuracy(gd, gt, w["min_depth"], w["max_depth"]))
                 inner.append(scenes.depth_accuracy(gd[20:-20, 20:-20], gt[20:-20, 20:-20],
@@ -849,7 +924,7 @@ def main():
             cpu_acc.append(scenes.depth_accuracy(odepth, gt, w["min_depth"], w["max_depth"]))
             cpu_in.append(scenes.depth_accuracy(odepth[20:-20, 20:-20], gt[20:-20, 20:-20],
                                                 w["min_depth"], w["max_depth"]))
-        gpu_mean, gpu_inner = gpu_crops(w["iterations"])
+        gpu_mean, gpu_inner = gpu_crops()
         # the CPU baseline's own crops (seed set 0) through the GPU engine:
         # crops have a larger share of image-edge pixels than the full view
         out["crop_check"] = {

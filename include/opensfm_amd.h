@@ -61,9 +61,11 @@ typedef struct pm_stats {
   int32_t patch_size;
   int32_t width, height; /* reference view */
   int32_t launches;      /* launches of the dominant kernel (the plane
-                            sweep, or the red/black PatchMatch halves) */
+                            sweep, or the red/black PatchMatch halves,
+                            both levels' with set_pyramid) */
   float dominant_ms;     /* summed HIP-event time of those launches */
-  float total_ms;        /* HIP-event time of the whole device compute */
+  float total_ms;        /* HIP-event time of the whole device compute
+                            (both levels' with set_pyramid) */
 } pm_stats;
 
 /* Message of the calling thread's last failed call ("" if none). */
@@ -505,6 +507,11 @@ int pm_batch_set_patch_size(pm_batch* b, int size);
 int pm_batch_set_min_patch_sd(pm_batch* b, float sd);
 int pm_batch_set_seed(pm_batch* b, uint64_t seed);
 int pm_batch_set_exploration(pm_batch* b, int n);
+/* pm_estimator_set_pyramid for every job (default 0 = off): the half level
+ * is a batch of the pool views box-averaged by 2 with the same jobs, run
+ * first in the same launch sequence (and graph); job k still equals the
+ * estimator with seed + k bit for bit. */
+int pm_batch_set_pyramid(pm_batch* b, int half_iterations);
 /* pm_estimator_set_consistency for every job (default 1). */
 int pm_batch_set_consistency(pm_batch* b, int enable);
 int pm_batch_set_post_process(pm_batch* b, int enable);

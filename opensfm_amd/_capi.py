@@ -162,6 +162,7 @@ SIGNATURES = {
     "pm_batch_set_init_search": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_init_spread": (C.c_int, [_vp, C.c_float, C.c_float]),
     "pm_batch_set_exploration": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_pyramid": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_consistency": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_tile": (C.c_int, [_vp, C.c_int]),

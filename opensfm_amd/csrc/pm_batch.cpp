@@ -12,7 +12,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -71,6 +73,14 @@ struct pm_batch {
   int tile = 1;
   int use_graph = 1;
   int precision = PM_PRECISION_FP32;
+  // half-resolution level (set_pyramid): a batch over the pool views
+  // box-averaged by 2 with the same jobs, run first on this batch's stream;
+  // its planes seed plain-mode runs here through per-job initialisation maps
+  int pyramid_iters = 0;
+  std::unique_ptr<pm_batch> half;
+  DevBuf half_map;  // the half level's state as initialisation maps
+  DevBuf jobtab_pyr;  // the job table with those maps (plain mode)
+  bool own_stream = true;
   std::vector<std::unique_ptr<PoolView>> views;
   std::vector<BatchJob> jobs;
   // device state of the prepared configuration
@@ -112,7 +122,7 @@ struct pm_batch {
     drop_graph();
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
-    if (stream) hipStreamDestroy(stream);
+    if (stream && own_stream) hipStreamDestroy(stream);
   }
 };
 
@@ -133,6 +143,77 @@ int validate(pm_batch* b) {
 }
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+void prepare(pm_batch* b, int count);
+
+// The half-resolution level of b: every pool view box-averaged by 2 (K
+// scaled about pixel centres, x_half = (x + 0.5) / 2 - 0.5; poses unchanged;
+// masks the minimum of each 2 x 2 block), the same jobs, b's settings with
+// pyramid_iters iterations and seed + kHalfLevelSeed, on b's stream (so that
+// a captured run holds both levels).  Rebuilt when views or jobs are added.
+void ensure_half(pm_batch* b, int count) {
+  if (!b->half || b->half->views.size() != b->views.size() ||
+      b->half->jobs.size() != b->jobs.size()) {
+    std::unique_ptr<pm_batch> h(new pm_batch());
+    h->device = b->device;
+    h->stream = b->stream;
+    h->own_stream = false;
+    for (auto& e : h->ev) HIP_CHECK(hipEventCreate(&e));
+    for (const auto& Vp : b->views) {
+      const PoolView& V = *Vp;
+      std::unique_ptr<PoolView> c(new PoolView());
+      memcpy(c->R, V.R, sizeof(c->R));
+      memcpy(c->t, V.t, sizeof(c->t));
+      memcpy(c->K, V.K, sizeof(c->K));
+      for (int r = 0; r < 2; ++r) {
+        for (int col = 0; col < 3; ++col) c->K[r * 3 + col] *= 0.5;
+        c->K[r * 3 + 2] -= 0.25 * V.K[8];
+      }
+      if (!inv3(c->K, c->Kinv)) throw std::runtime_error("half-level K is singular");
+      c->width = V.width / 2;
+      c->height = V.height / 2;
+      if (c->width < 2 || c->height < 2)
+        throw std::runtime_error("view too small for a half level");
+      const size_t np = (size_t)c->width * c->height;
+      c->img.ensure(np);
+      c->mask.ensure(np);
+      HIP_CHECK(pm::launch_downsample_u8(V.img.as<uint8_t>(), V.width, V.height, 2, 0,
+                                         c->img.as<uint8_t>(), c->width, c->height, b->stream));
+      HIP_CHECK(pm::launch_downsample_u8(V.mask.as<uint8_t>(), V.width, V.height, 2, 1,
+                                         c->mask.as<uint8_t>(), c->width, c->height,
+                                         b->stream));
+      c->quad.ensure((size_t)(c->width + 1) * (c->height + 1) * sizeof(uint32_t));
+      c->quadc.ensure((size_t)(c->width + 1) * (c->height + 1) * 8);
+      HIP_CHECK(pm::launch_build_quad(c->img.as<uint8_t>(), c->quad.as<uint32_t>(),
+                                      c->quadc.as<uint32_t>(), c->width, c->height, b->stream));
+      h->views.push_back(std::move(c));
+    }
+    for (const BatchJob& J : b->jobs) {
+      BatchJob c;
+      c.ref = J.ref;
+      c.nb = J.nb;
+      c.min_depth = J.min_depth;
+      c.max_depth = J.max_depth;
+      h->jobs.push_back(std::move(c));
+    }
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    b->half = std::move(h);
+  }
+  pm_batch* h = b->half.get();
+  h->patch_size = b->patch_size;
+  h->iterations = b->pyramid_iters;
+  h->min_patch_variance = b->min_patch_variance;
+  h->seed = b->seed + pm::kHalfLevelSeed;
+  h->n_far = b->n_far;
+  h->n_refine = b->n_refine;
+  h->n_explore = b->n_explore;
+  h->post_process = b->post_process;
+  h->consistency = b->consistency;
+  h->tile = b->tile;
+  h->precision = b->precision;
+  h->use_graph = 0;
+  prepare(h, count);
+}
 
 // Uploads the view and job tables for the first `count` jobs and sizes the
 // per-pixel buffers.  Host-synchronous.
@@ -283,6 +364,29 @@ void prepare(pm_batch* b, int count) {
   b->jobtab.ensure(jt.size() * sizeof(pm::PmJob));
   HIP_CHECK(hipMemcpy(b->jobtab.p, jt.data(), jt.size() * sizeof(pm::PmJob),
                       hipMemcpyHostToDevice));
+  if (b->pyramid_iters > 0) {
+    // plain-mode runs: jobs without a caller's map start from the half
+    // level's, sampled bilinearly with no spread (estimator: run_half)
+    ensure_half(b, count);
+    const pm_batch& h = *b->half;
+    b->half_map.ensure(h.total_px * 4 * sizeof(float));
+    for (int k = 0; k < count; ++k) {
+      if (b->jobs[k].init_map) continue;
+      const PoolView& HR = *h.views[h.jobs[k].ref];
+      pm::PmArgs& a = jt[k].a;
+      a.init_map = b->half_map.as<float>() + 4 * h.jobs[k].off;
+      a.init_w = HR.width;
+      a.init_h = HR.height;
+      a.init_stride = 4;
+      a.init_scale = 2;
+      a.init_search = 1;
+      a.init_sigma = 0.0f;
+      a.init_sigma_slope = 0.0f;
+    }
+    b->jobtab_pyr.ensure(jt.size() * sizeof(pm::PmJob));
+    HIP_CHECK(hipMemcpy(b->jobtab_pyr.p, jt.data(), jt.size() * sizeof(pm::PmJob),
+                        hipMemcpyHostToDevice));
+  }
   b->prepared_jobs = count;
 }
 
@@ -293,7 +397,16 @@ void prepare(pm_batch* b, int count) {
 void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   const bool sample = mode == PM_MODE_PATCHMATCH_SAMPLE;
   const int nj = b->prepared_jobs;
-  const pm::PmJob* jt = b->jobtab.as<pm::PmJob>();
+  const bool pyramid = !sample && b->pyramid_iters > 0;
+  const pm::PmJob* jt = (pyramid ? b->jobtab_pyr : b->jobtab).as<pm::PmJob>();
+  if (pyramid) {
+    pm_batch* h = b->half.get();
+    enqueue(h, PM_MODE_PATCHMATCH, nullptr);
+    HIP_CHECK(pm::launch_state_to_map_batch(h->jobtab.as<pm::PmJob>(), h->prepared_jobs,
+                                            h->pix_total, h->state.as<float>(),
+                                            b->post_process, b->half_map.as<float>(),
+                                            b->stream));
+  }
   unsigned* q = b->queue.as<unsigned>();
   unsigned long long* cnt = b->counter.as<unsigned long long>();
   const int nq = 1 + 16 * b->iterations;
@@ -412,7 +525,9 @@ float probe(pm_batch* b, int mode) {
     ++count;
   }
   const int iters = b->iterations;
+  const int pyr = b->pyramid_iters;  // the tile is timed on plain runs
   b->iterations = 2;
+  b->pyramid_iters = 0;
   std::vector<hipEvent_t> ev(8, nullptr);
   float ms = 0;
   try {
@@ -429,11 +544,13 @@ float probe(pm_batch* b, int mode) {
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
     b->iterations = iters;
+    b->pyramid_iters = pyr;
     b->prepared_jobs = -1;
     throw;
   }
   for (auto& e : ev) hipEventDestroy(e);
   b->iterations = iters;
+  b->pyramid_iters = pyr;
   b->prepared_jobs = -1;
   return ms / (float)(px / 1e6);
 }
@@ -528,6 +645,15 @@ int pm_batch_add_job(pm_batch* b, int ref_view, const int* neighbours,
   J.max_depth = max_depth;
   if (job_id) *job_id = (int)b->jobs.size();
   b->jobs.push_back(std::move(J));
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_pyramid(pm_batch* b, int half_iterations) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (half_iterations < 0 || half_iterations > 1000)
+    return fail(PM_ERR_INVALID, "half-level iterations must be in [0, 1000]");
+  b->pyramid_iters = half_iterations;
   b->invalidate();
   return PM_OK;
 }

@@ -327,7 +327,7 @@ int run_half(pm_estimator* e) {
   h->num_depth_planes = e->num_depth_planes;
   h->iterations = e->pyramid_iters;
   h->min_patch_variance = e->min_patch_variance;
-  h->seed = e->seed ^ 0x9E3779B97F4A7C15ull;
+  h->seed = e->seed + pm::kHalfLevelSeed;
   h->n_far = e->n_far;
   h->n_refine = e->n_refine;
   h->n_explore = e->n_explore;
@@ -478,6 +478,16 @@ int run(pm_estimator* e, int mode) {
   e->stats.launches = launches;
   e->stats.dominant_ms = dom_ms;
   e->stats.total_ms = total_ms;
+  if (pyramid) {
+    // the whole compute: the half level's launches of the same kernels and
+    // its device time count too
+    const pm_stats& hs = e->half->stats;
+    e->stats.scorings += hs.scorings;
+    e->stats.dominant_scorings += hs.dominant_scorings;
+    e->stats.launches += hs.launches;
+    e->stats.dominant_ms += hs.dominant_ms;
+    e->stats.total_ms += hs.total_ms;
+  }
   e->have_result = true;
   return PM_OK;
 }

@@ -31,6 +31,11 @@ hipError_t launch_pm_init(const PmArgs& a, bool sample, hipStream_t s);
 // for the next finer level (k_state_to_map).
 hipError_t launch_state_to_map(const float* state, int W, int H, float log_min, float log_max,
                                int median, float* map, hipStream_t s);
+// The same for every job of a batch (map0 laid out like the state, 4 floats
+// per pixel at the jobs' pixel offsets; depth ranges from the job table).
+hipError_t launch_state_to_map_batch(const PmJob* jobs, int njobs, int pix_total,
+                                     const float* state0, int median, float* map0,
+                                     hipStream_t s);
 hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
                           unsigned* queues, hipStream_t s);
 // Box downsampling (mode 0 mean, 1 min) by an integer factor.

@@ -962,6 +962,35 @@ __global__ __launch_bounds__(256) void k_state_to_map(const float4* __restrict__
   map[idx] = m;
 }
 
+// The same over every job of a batch: the map of job k lies at the same
+// pixel offset as its state (one workgroup per 16 x 16 pixel tile).
+__global__ __launch_bounds__(256) void k_state_to_map_batch(const PmJob* __restrict__ jobs,
+                                                            int njobs,
+                                                            const float4* __restrict__ state0,
+                                                            int median,
+                                                            float4* __restrict__ map0) {
+  const int t = blockIdx.x;
+  const PmJob& J = jobs[find_job(jobs, njobs, t, &PmJob::pix_t0)];
+  const int lt = t - J.pix_t0;
+  const int i = (lt / J.pix_tx) * 16 + (threadIdx.x >> 4);
+  const int j = (lt % J.pix_tx) * 16 + (threadIdx.x & 15);
+  const int W = J.a.W, H = J.a.H;
+  if (i >= H || j >= W) return;
+  const float4* st = reinterpret_cast<const float4*>(J.a.state);
+  const size_t idx = (size_t)i * W + j;
+  const float4 s = st[idx];
+  bool ok = s.w > 0.0f && s.z != 0.0f;
+  if (ok && median) ok = median_keeps(s.w, median5_depth(st, W, H, i, j));
+  float4 m = make_float4(0.0f, 0.0f, 0.0f, -10.0f);
+  if (ok) {
+    m.x = (logf(s.w) - J.a.log_min) / (J.a.log_max - J.a.log_min);
+    m.y = -s.x / s.z;
+    m.z = -s.y / s.z;
+    m.w = 10.0f;
+  }
+  map0[(st - state0) + idx] = m;
+}
+
 __device__ __forceinline__ void pack_pixel(const PmArgs& a, int median, int i,
                                            int j, float* __restrict__ out_depth,
                                            float* __restrict__ out_plane,
@@ -2050,6 +2079,16 @@ hipError_t launch_state_to_map(const float* state, int W, int H, float log_min, 
   hipLaunchKernelGGL(k_state_to_map, grid16(W, H), dim3(256), 0, s,
                      reinterpret_cast<const float4*>(state), W, H, log_min, log_max, median,
                      reinterpret_cast<float4*>(map));
+  return hipGetLastError();
+}
+
+hipError_t launch_state_to_map_batch(const PmJob* jobs, int njobs, int pix_total,
+                                     const float* state0, int median, float* map0,
+                                     hipStream_t s) {
+  if (pix_total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_state_to_map_batch, dim3(pix_total), dim3(256), 0, s, jobs, njobs,
+                     reinterpret_cast<const float4*>(state0), median,
+                     reinterpret_cast<float4*>(map0));
   return hipGetLastError();
 }
 

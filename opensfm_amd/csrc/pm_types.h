@@ -24,6 +24,12 @@ constexpr int kPrecFp32Quads = 3;
 constexpr float kConsistMinScore = 0.5f;
 constexpr int kConsistFinalSupport = 10;
 
+// Seed offset of the half-resolution level: the level of an estimator with
+// seed s runs with seed s + kHalfLevelSeed, so that batched job k (seed
+// s + k) and an estimator with seed s + k draw the same numbers at both
+// levels.
+constexpr uint64_t kHalfLevelSeed = 0x9E3779B97F4A7C15ull;
+
 // One view as the kernels see it.  For neighbour view v (v >= 1) the plane-
 // induced homography of plane v_pi (in the reference camera frame) is
 //   H_v = K_v (Q_v + a_v v_pi^T) K_0^-1 = M_v + b_v (K_0^-T v_pi)^T

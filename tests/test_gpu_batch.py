@@ -38,9 +38,10 @@ def _batch(scene, iters=2, patch=7, tile=PM_TILE_16x16, graph=True):
     return b
 
 
-def _single(scene, k, iters=2, patch=7, sample=False):
+def _single(scene, k, iters=2, patch=7, sample=False, pyramid=0):
     ref, nb = scene.jobs[k]
     e = DepthmapEstimator()
+    e.set_pyramid(pyramid)
     for v in [ref] + list(nb):
         V = scene.views[v]
         e.add_view(V.K, V.R, V.t, V.image, V.mask)
@@ -66,6 +67,25 @@ def test_batch_equals_single_view_runs(street, tile):
         want = _single(street, k)
         assert _equal(got, want), f"job {k}"
         assert (got[0] > 0).mean() > 0.3
+
+
+def test_batch_pyramid_equals_single_view_runs(street):
+    """With the half-resolution level (2 half + 2 full iterations), job k
+    still equals the estimator with seed + k bit for bit, in a captured
+    graph (replayed twice) and eagerly; sample mode ignores the level."""
+    print("batch vs single-view estimator with the half-resolution level", flush=True)
+    for graph in (True, False):
+        b = _batch(street, graph=graph)
+        b.set_pyramid(2)
+        for rep in range(3 if graph else 1):
+            b.run()
+            for k in range(len(street.jobs)):
+                got = b.result(k)
+                assert _equal(got, _single(street, k, pyramid=2)), f"job {k} graph {graph} run {rep}"
+                assert (got[0] > 0).mean() > 0.3
+    b.run(sample=True)
+    for k in range(len(street.jobs)):
+        assert _equal(b.result(k), _single(street, k, sample=True)), f"sample job {k}"
 
 
 def test_batch_graph_replays_and_eager_agree(street):
