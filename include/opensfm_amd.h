@@ -159,9 +159,11 @@ int pm_estimator_set_precision(pm_estimator* est, int precision);
  * re-selects its plane among the planes of that window's other pixels whose
  * depth at this pixel passes the same test: best NCC over the views, then
  * the reference's 6 refinements around it, each kept only while it passes
- * the test; the new plane is kept when its NCC is at least 0.55.  Such
- * pixels hold planes that fit their patch better than the surrounding
- * surface does (next to occlusion edges, planes bridging both sides). */
+ * the test; the new plane is kept when its NCC is at least 0.5 and, after
+ * the last iteration, when its depth lies within 1 % of the depths of at
+ * least 10 of the 24 window pixels.  Such pixels hold planes that fit their
+ * patch better than the surrounding surface does (next to occlusion edges,
+ * planes bridging both sides). */
 int pm_estimator_set_consistency(pm_estimator* est, int enable);
 /* Run PostProcess (5x5 median, depthmap.cc:489-502) after PatchMatch
  * (default 1, as the reference always does; 0 returns the raw depth). */
