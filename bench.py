@@ -7,9 +7,10 @@ period 6), depth range [1, 40], patch 7x7, min patch SD 1, FP32 samples and
 FP32 NCC.  Inputs are resident in HBM when the timed region starts (views
 uploaded by add_view); each step runs the schedule that reaches the
 reference's 5-iteration accuracy and mean score (checked over seeds 1-5
-against the oracle's recorded figures): 5 iterations at half resolution
-(the views box-averaged by 2), whose planes seed 2 full-resolution
-iterations, and the median post-process, into device-resident output maps.
+against the oracle's recorded figures): a 4-level pyramid (the views
+box-averaged by 8, 4 and 2), 2 iterations per coarser level, each level's
+planes seeding the next, then 2 full-resolution iterations and the median
+post-process, into device-resident output maps.
 Plain PatchMatch at the reference's 5 iterations is timed beside it.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the red/black
@@ -42,7 +43,7 @@ FLOP_PER_SCORING = 160
 WORKLOAD = dict(width=1600, height=1200, neighbours=8, boxes=20,
                 terrain=(1.5, 6.0), min_depth=1.0, max_depth=40.0, patch=7,
                 min_patch_sd=1.0, iterations=5, seed=2000,
-                half_iterations=5, full_iterations=2)
+                pyramid_levels=4, half_iterations=2, full_iterations=2)
 
 
 GOLDEN_CFG2 = os.path.join(HERE, "tests", "golden", "config2_oracle_fullview.json")
@@ -162,8 +163,8 @@ def cpu_baseline(scene, threads, seed_sets=5, target_s=4.0):
 def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, seed=4000,
                         seed_sets=2):
     """Accuracy reference for the street batch: the oracle's ComputePatchMatch
-    at `iterations` and the GPU engine with gpu_schedule = (half-resolution
-    iterations, full iterations) on the same crops (side x side from the
+    at `iterations` and the GPU engine with gpu_schedule = (iterations per
+    coarser level, full iterations, pyramid levels) on the same crops (side x side from the
     centre of the first `jobs` reference views, K shifted, all of the job's
     neighbours), for `seed_sets` sets of seeds, one oracle run per host
     thread."""
@@ -218,7 +219,7 @@ def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, 
     cpu_s = time.perf_counter() - t0
     def gpu_est(views, s):
         e = setup(DepthmapEstimator(), views, s)
-        e.set_pyramid(gpu_schedule[0])
+        e.set_pyramid(gpu_schedule[0], levels=gpu_schedule[2])
         e.set_patchmatch_iterations(gpu_schedule[1])
         return e
 
@@ -237,14 +238,15 @@ def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, 
             "oracle_s": round(cpu_s, 1)}
 
 
-BATCH_SCHEDULE = (5, 2)  # half-resolution iterations, full iterations (street batch)
+# street batch: iterations per coarser level, full iterations, pyramid levels
+BATCH_SCHEDULE = (2, 2, 4)
 
 
 def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
     """Batched street workload: n_views reference views of sizes drawn from
     {640x480, 1024x768, 1600x1200, 2048x1536}, 4-12 neighbours each
-    (rendered on the GPU), 5 iterations at half resolution seeding 2 at full
-    resolution (the reference's count is 3; accuracy checked against the
+    (rendered on the GPU), a 4-level pyramid with 2 iterations per level
+    (full resolution included) (the reference's count is 3; accuracy checked against the
     oracle's 3 iterations on crops) in one captured hipGraph, tile
     autotuned by speed and patch size {5, 7} gated on accuracy (the fastest
     within 0.2 points of the most accurate), then the cleaner with 2
@@ -263,7 +265,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         b.add_view(v.K, v.R, v.t, v.image, v.mask)
     for ref, nb in sc.jobs:
         b.add_job(ref, nb, sc.min_depth, sc.max_depth)
-    b.set_pyramid(BATCH_SCHEDULE[0])
+    b.set_pyramid(BATCH_SCHEDULE[0], levels=BATCH_SCHEDULE[2])
     b.set_patchmatch_iterations(BATCH_SCHEDULE[1])
     b.set_min_patch_sd(1.0)
     b.set_seed(seed)
@@ -298,7 +300,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
     plain3 = {"iterations": 3, "mpix_s": round(npx / plain_s / 1e6, 3),
               "accuracy_within_1pct": round(accuracy(b), 4)}
     log(f"street batch, plain 3 iterations: {plain3}")
-    b.set_pyramid(BATCH_SCHEDULE[0])
+    b.set_pyramid(BATCH_SCHEDULE[0], levels=BATCH_SCHEDULE[2])
     b.set_patchmatch_iterations(BATCH_SCHEDULE[1])
     b.run()
     b.clean(0.01, 2)
@@ -341,7 +343,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
             V = sc.views[v]
             e.add_view(V.K, V.R, V.t, V.image, V.mask)
         e.set_depth_range(sc.min_depth, sc.max_depth, 50)
-        e.set_pyramid(BATCH_SCHEDULE[0])
+        e.set_pyramid(BATCH_SCHEDULE[0], levels=BATCH_SCHEDULE[2])
         e.set_patchmatch_iterations(BATCH_SCHEDULE[1])
         e.set_patch_size(patch)
         e.set_min_patch_sd(1.0)
@@ -350,9 +352,12 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         e.run(PM_MODE_PATCHMATCH)
         per_view_s += time.perf_counter() - t0
     # the same jobs through the cluster estimator's surface (dense.py's call
-    # sequence): host images in, [depth, normal, cost, confidence] out
+    # sequence with its default depthmap_max_iterations = 4 and
+    # depthmap_hierarchy_levels = 4, config.py:385, 411): host images in,
+    # [depth, normal, cost, confidence] out
     ce = DepthmapClusterEstimator()
-    ce.set_max_iterations(3)
+    ce.set_max_iterations(4)
+    ce.set_hierarchy_levels(4)
     ce.set_patch_size(patch)
     for ref, nb in sc.jobs:
         R = sc.views[ref]
@@ -368,7 +373,8 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
                                  for r, (ref, _) in zip(cres, sc.jobs)]))
     return {
         "views": n_views, "mpix": round(npx / 1e6, 2),
-        "half_resolution_iterations": BATCH_SCHEDULE[0], "iterations": BATCH_SCHEDULE[1],
+        "pyramid_levels": BATCH_SCHEDULE[2], "coarse_level_iterations": BATCH_SCHEDULE[0],
+        "iterations": BATCH_SCHEDULE[1],
         "plain_reference_iterations": plain3,
         "patch": patch, "tile": TILE_NAMES[tile], "autotune_s": round(tune_s, 2),
         "autotune": b.tune_report,
@@ -382,7 +388,8 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         "clean_mpix_s": round(npx / (clean_ms / 1e3) / 1e6, 1) if clean_ms > 0 else None,
         "clean_kept_within_1pct": round(kept_good / max(1, kept), 4),
         "clean_cpu_baseline": clean_cpu,
-        "cluster_estimator": {"wall_ms": round(cluster_s * 1e3, 1),
+        "cluster_estimator": {"max_iterations": 4, "hierarchy_levels": 4,
+                              "wall_ms": round(cluster_s * 1e3, 1),
                               "mpix_s": round(npx / cluster_s / 1e6, 3),
                               "accuracy_within_1pct": round(cluster_acc, 4)},
     }
@@ -581,8 +588,8 @@ def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
     """Learned initialiser configuration: 64 nadir views of 4000x3000 over
     fractal terrain (an 8 x 8 camera grid, rendered on the GPU), 10
     neighbours each, all in one DepthmapBatch.  Plain PatchMatch from random
-    planes at 5, 2 and 1 iterations, and seeded by 5 iterations at half
-    resolution with 2 at full, against the learned initialiser (network
+    planes at 5, 2 and 1 iterations, and through a 4-level pyramid with 2
+    iterations per level, against the learned initialiser (network
     input from the batch's 1/4-resolution views and 32-plane cost volume,
     the U-Net, then seeded PatchMatch) at 1 and 2 iterations, each timed once
     from device-resident views to device-resident maps."""
@@ -615,10 +622,11 @@ def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
     # warm the network and the coarse views outside the timed region
     init.predict_batch(b, 0, width, height)
     for mode, learned, iters in (("plain_5", False, 5), ("plain_2", False, 2),
-                                 ("plain_1", False, 1), ("plain_half5_2", False, 2),
+                                 ("plain_1", False, 1), ("plain_pyramid4_2", False, 2),
                                  ("learned_1", True, 1), ("learned_2", True, 2)):
-        # plain_half5_2: 5 iterations at half resolution seeding 2 at full
-        b.set_pyramid(5 if mode == "plain_half5_2" else 0)
+        # plain_pyramid4_2: levels at 1/8, 1/4, 1/2 and full resolution, 2
+        # iterations each, every level seeding the next
+        b.set_pyramid(2 if mode == "plain_pyramid4_2" else 0, levels=4)
         b.set_patchmatch_iterations(iters)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -638,12 +646,12 @@ def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
                 b.set_init_map(k, None)
         log(f"learned initialiser {mode}: {out[mode]}")
     base = out["plain_5"]["mpix_s"]
-    for mode in ("learned_1", "learned_2", "plain_2", "plain_1", "plain_half5_2"):
+    for mode in ("learned_1", "learned_2", "plain_2", "plain_1", "plain_pyramid4_2"):
         out[f"{mode}_over_plain_5"] = round(out[mode]["mpix_s"] / base, 2)
     # the fewest plain iterations reaching plain-5's accuracy, and the
     # learned runs at that accuracy
     target = out["plain_5"]["acc"]
-    eq_plain = next(m for m in ("plain_1", "plain_2", "plain_half5_2", "plain_5")
+    eq_plain = next(m for m in ("plain_1", "plain_2", "plain_pyramid4_2", "plain_5")
                     if out[m]["acc"] >= target)
     eq_learned = [m for m in ("learned_1", "learned_2") if out[m]["acc"] >= target]
     out["equal_accuracy"] = {
@@ -719,10 +727,10 @@ def main():
     est.set_seed(w["seed"])
 
     def schedule(half):
-        """half: the half-resolution level + full iterations that reach the
-        reference's 5-iteration accuracy; else plain PatchMatch at the
+        """half: the pyramid (coarser levels + full iterations) that reaches
+        the reference's 5-iteration accuracy; else plain PatchMatch at the
         reference's 5 iterations."""
-        est.set_pyramid(w["half_iterations"] if half else 0)
+        est.set_pyramid(w["half_iterations"] if half else 0, levels=w["pyramid_levels"])
         est.set_patchmatch_iterations(w["full_iterations"] if half else w["iterations"])
 
     def timed(steps, warmup):
@@ -858,11 +866,12 @@ def main():
         "data": "synthetic (seeded ray-cast scene, value-noise texture)",
         "config": {"workload": "single-view plain PatchMatch 1600x1200 + 8 neighbours, "
                                "patch 7, depth [1, 40], FP32, at the accuracy and mean score of the "
-                               "reference's 5 iterations: 5 iterations at half resolution seeding "
-                               "2 at full resolution; value = device-resident views to "
-                               "device-resident maps",
+                               "reference's 5 iterations: a 4-level pyramid (1/8, 1/4, 1/2, full "
+                               "resolution), 2 iterations per level, each level seeding the next; "
+                               "value = device-resident views to device-resident maps",
                    "width": w["width"], "height": w["height"], "neighbours": w["neighbours"],
-                   "patch": w["patch"], "half_resolution_iterations": w["half_iterations"],
+                   "patch": w["patch"], "pyramid_levels": w["pyramid_levels"],
+                   "coarse_level_iterations": w["half_iterations"],
                    "iterations": w["full_iterations"],
                    "reference_iterations": w["iterations"]},
         "accuracy_within_1pct": round(acc, 4),
@@ -875,8 +884,8 @@ def main():
         "plane_sweep": sweep,
         "roofline": {
             "kernel": "k_pm_half<7> (red/black PatchMatch half-iteration, every launch of a "
-                      "compute at both levels; random-plane launches read the u8 quads, the "
-                      "others the FP16 coefficients)",
+                      "compute at all pyramid levels; random-plane launches read the u8 quads, "
+                      "the others the FP16 coefficients)",
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
@@ -910,7 +919,7 @@ def main():
             acc, inner = [], []   # inner: at least 20 px from the crop's edges
             for ce, (x0, y0, cw, ch, _) in zip(crop_est, crops):
                 gt = scene.ref.depth[y0:y0 + ch, x0:x0 + cw]
-                ce.set_pyramid(w["half_iterations"])
+                ce.set_pyramid(w["half_iterations"], levels=w["pyramid_levels"])
                 ce.set_patchmatch_iterations(w["full_iterations"])
                 gd = ce.compute_patch_match()[0]
                 acc.append(scenes.depth_accuracy(gd, gt, w["min_depth"], w["max_depth"]))

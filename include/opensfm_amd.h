@@ -133,6 +133,12 @@ int pm_estimator_set_far_probes(pm_estimator* est, int n);
  * Default 0 (off); ignored in sample mode and when an initialisation map
  * is set. */
 int pm_estimator_set_pyramid(pm_estimator* est, int half_iterations);
+/* Levels of that pyramid, the full resolution included (default 2; in
+ * [2, 8]; the reference's OpenCL engine's hierarchy_levels, estimator.h:28,
+ * scale 1 / 2^(N - 1 - level) at estimator.cc:168-172): with N > 2 the half
+ * level is itself seeded by a quarter level, and so on; every level coarser
+ * than the full one runs half_iterations iterations. */
+int pm_estimator_set_pyramid_levels(pm_estimator* est, int levels);
 /* Random refinements per pixel update (default 6, the reference's count,
  * depthmap.cc:337; the 0.02 * 0.3^k / 0.5 * 0.8^k schedule restarts every
  * 6 steps). */
@@ -514,6 +520,8 @@ int pm_batch_set_exploration(pm_batch* b, int n);
  * first in the same launch sequence (and graph); job k still equals the
  * estimator with seed + k bit for bit. */
 int pm_batch_set_pyramid(pm_batch* b, int half_iterations);
+/* pm_estimator_set_pyramid_levels for every job (default 2). */
+int pm_batch_set_pyramid_levels(pm_batch* b, int levels);
 /* pm_estimator_set_consistency for every job (default 1). */
 int pm_batch_set_consistency(pm_batch* b, int enable);
 int pm_batch_set_post_process(pm_batch* b, int enable);

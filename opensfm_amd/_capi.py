@@ -74,6 +74,7 @@ SIGNATURES = {
     "pm_estimator_set_seed": (C.c_int, [_vp, C.c_uint64]),
     "pm_estimator_set_far_probes": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_pyramid": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_pyramid_levels": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_refinement_steps": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_exploration": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_precision": (C.c_int, [_vp, C.c_int]),
@@ -163,6 +164,7 @@ SIGNATURES = {
     "pm_batch_set_init_spread": (C.c_int, [_vp, C.c_float, C.c_float]),
     "pm_batch_set_exploration": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_pyramid": (C.c_int, [_vp, C.c_int]),
+    "pm_batch_set_pyramid_levels": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_consistency": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_tile": (C.c_int, [_vp, C.c_int]),

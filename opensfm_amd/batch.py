@@ -89,9 +89,11 @@ class DepthmapBatch:
     def set_seed(self, seed: int) -> None:
         _capi.check(self._lib.pm_batch_set_seed(self._h, int(seed) & (2 ** 64 - 1)))
 
-    def set_pyramid(self, half_iterations: int) -> None:
+    def set_pyramid(self, half_iterations: int, levels: int = 2) -> None:
         """Plain PatchMatch of every job seeded by half_iterations iterations
-        at half resolution (0: off), as DepthmapEstimator.set_pyramid."""
+        at half resolution (0: off), with levels - 1 coarser levels, as
+        DepthmapEstimator.set_pyramid."""
+        _capi.check(self._lib.pm_batch_set_pyramid_levels(self._h, int(levels)))
         _capi.check(self._lib.pm_batch_set_pyramid(self._h, int(half_iterations)))
 
     def set_exploration(self, n: int) -> None:

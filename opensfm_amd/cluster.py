@@ -10,12 +10,15 @@ set of launches): depth, the unit camera-frame normal of the winning
 plane, cost = 1 - NCC score, and the reference's logistic confidence
 (estimator.cc:1022-1040).
 
-The reference's estimator is a different PatchMatch (multi-scale pyramid,
-top-k view costs with census, SfM planar prior, cross-view geometric
-consistency).  Settings with a counterpart here are applied
-(``max_iterations``, ``patch_size``); the others are accepted and kept
-(``max_image_size`` -- the caller already scales images --,
-``hierarchy_levels``, ``sigma_spatial``, ``sigma_color``, ``top_k``,
+The reference's estimator is a different PatchMatch (top-k view costs
+with census, SfM planar prior, cross-view geometric consistency).  Settings
+with a counterpart here are applied: ``max_iterations``, ``patch_size`` and
+``hierarchy_levels`` -- as in the reference (estimator.cc:140, 168-172),
+N levels at scales 1 / 2^(N - 1), ..., 1/2, 1, each running max_iterations
+iterations and seeding the next (the engine's pyramid; levels whose
+smallest view would be narrower than one patch are dropped).  The others
+are accepted and kept (``max_image_size`` -- the caller already scales
+images --, ``sigma_spatial``, ``sigma_color``, ``top_k``,
 ``census_weight``, ``smooth_weight``, the geometric-consistency weight and
 links, the SfM points) without changing the result.
 """
@@ -127,11 +130,26 @@ class DepthmapClusterEstimator:
             nb = [view_index(s) for s in ref["sources"]]
             lo, hi = ref["depth_range"]
             b.add_job(r, nb, lo, hi)
-        b.set_patchmatch_iterations(max(1, self._params["max_iterations"]))
-        b.set_patch_size(self._params["patch_size"])
+        iters = max(1, self._params["max_iterations"])
+        patch = self._params["patch_size"]
+        b.set_patchmatch_iterations(iters)
+        b.set_patch_size(patch)
+        levels = self.pyramid_levels(
+            min(min(v["image"].shape) for ref in self._refs
+                for v in [ref["view"]] + ref["sources"]), patch)
+        if levels >= 2:
+            b.set_pyramid(iters, levels=levels)
         b.run()
         self.last_stats = b.stats()
         return [b.cluster_result(k) for k in range(len(self._refs))]
+
+    def pyramid_levels(self, min_side: int, patch: int) -> int:
+        """hierarchy_levels, less the levels at which a view of min_side
+        pixels would be narrower than one patch (at most 8)."""
+        levels = min(8, max(1, self._params["hierarchy_levels"]))
+        while levels > 1 and (min_side >> (levels - 1)) < patch:
+            levels -= 1
+        return levels
 
     def clear(self) -> None:
         self._refs = []

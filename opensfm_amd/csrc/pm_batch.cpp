@@ -75,8 +75,11 @@ struct pm_batch {
   int precision = PM_PRECISION_FP32;
   // half-resolution level (set_pyramid): a batch over the pool views
   // box-averaged by 2 with the same jobs, run first on this batch's stream;
-  // its planes seed plain-mode runs here through per-job initialisation maps
+  // its planes seed plain-mode runs here through per-job initialisation maps;
+  // with pyramid_levels > 2 that batch has a half level of its own, and so
+  // on, every coarser level running pyramid_iters iterations
   int pyramid_iters = 0;
+  int pyramid_levels = 2;
   std::unique_ptr<pm_batch> half;
   DevBuf half_map;  // the half level's state as initialisation maps
   DevBuf jobtab_pyr;  // the job table with those maps (plain mode)
@@ -202,6 +205,8 @@ void ensure_half(pm_batch* b, int count) {
   pm_batch* h = b->half.get();
   h->patch_size = b->patch_size;
   h->iterations = b->pyramid_iters;
+  h->pyramid_iters = b->pyramid_levels > 2 ? b->pyramid_iters : 0;
+  h->pyramid_levels = b->pyramid_levels - 1;
   h->min_patch_variance = b->min_patch_variance;
   h->seed = b->seed + pm::kHalfLevelSeed;
   h->n_far = b->n_far;
@@ -507,6 +512,18 @@ int run(pm_batch* b, int mode) {
   b->stats.width = 0;
   b->stats.height = 0;
   b->stats.launches = 2 * b->iterations;
+  // the coarser levels ran in the same sequence: their launches of the same
+  // kernels count too (the device time above already holds them)
+  if (mode == PM_MODE_PATCHMATCH) {
+    for (const pm_batch* c = b; c->pyramid_iters > 0; c = c->half.get()) {
+      const pm_batch* h = c->half.get();
+      unsigned long long hc[2] = {0, 0};
+      HIP_CHECK(hipMemcpy(hc, h->counter.p, sizeof(hc), hipMemcpyDeviceToHost));
+      b->stats.scorings += (int64_t)(hc[0] + hc[1]);
+      b->stats.dominant_scorings += (int64_t)hc[1];
+      b->stats.launches += 2 * h->iterations;
+    }
+  }
   b->stats.dominant_ms = ms;
   b->stats.total_ms = ms;
   b->have_result = true;
@@ -645,6 +662,14 @@ int pm_batch_add_job(pm_batch* b, int ref_view, const int* neighbours,
   J.max_depth = max_depth;
   if (job_id) *job_id = (int)b->jobs.size();
   b->jobs.push_back(std::move(J));
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_pyramid_levels(pm_batch* b, int levels) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (levels < 2 || levels > 8) return fail(PM_ERR_INVALID, "pyramid levels must be in [2, 8]");
+  b->pyramid_levels = levels;
   b->invalidate();
   return PM_OK;
 }

@@ -81,8 +81,11 @@ struct pm_estimator {
   std::vector<std::unique_ptr<HostView>> coarse;
   DevBuf coarse_viewtab;
   // half-resolution level (set_pyramid): its own estimator over the views
-  // box-averaged by 2, run first; its state seeds this one's initialisation
+  // box-averaged by 2, run first; its state seeds this one's initialisation.
+  // With pyramid_levels > 2 that estimator has a half level of its own, and
+  // so on: every coarser level runs pyramid_iters iterations.
   int pyramid_iters = 0;
+  int pyramid_levels = 2;
   std::unique_ptr<pm_estimator> half;
   DevBuf half_map;  // the half level's state as an initialisation map
 
@@ -326,6 +329,8 @@ int run_half(pm_estimator* e) {
   h->max_depth = e->max_depth;
   h->num_depth_planes = e->num_depth_planes;
   h->iterations = e->pyramid_iters;
+  h->pyramid_iters = e->pyramid_levels > 2 ? e->pyramid_iters : 0;
+  h->pyramid_levels = e->pyramid_levels - 1;
   h->min_patch_variance = e->min_patch_variance;
   h->seed = e->seed + pm::kHalfLevelSeed;
   h->n_far = e->n_far;
@@ -874,6 +879,13 @@ int pm_estimator_set_pyramid(pm_estimator* e, int half_iterations) {
   if (!e) return fail(PM_ERR_INVALID, "null estimator");
   if (half_iterations < 0) return fail(PM_ERR_INVALID, "half-level iterations must be >= 0");
   e->pyramid_iters = half_iterations;
+  return PM_OK;
+}
+
+int pm_estimator_set_pyramid_levels(pm_estimator* e, int levels) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (levels < 2 || levels > 8) return fail(PM_ERR_INVALID, "pyramid levels must be in [2, 8]");
+  e->pyramid_levels = levels;
   return PM_OK;
 }
 

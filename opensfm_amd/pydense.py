@@ -87,9 +87,12 @@ class DepthmapEstimator:
     def set_seed(self, seed: int) -> None:
         _capi.check(self._lib.pm_estimator_set_seed(self._h, int(seed) & (2 ** 64 - 1)))
 
-    def set_pyramid(self, half_iterations: int) -> None:
+    def set_pyramid(self, half_iterations: int, levels: int = 2) -> None:
         """Plain PatchMatch seeded by half_iterations iterations at half
-        resolution (0: off; not in the reference's legacy estimator)."""
+        resolution (0: off; not in the reference's legacy estimator); with
+        levels > 2 each coarser level is seeded by the next, down to
+        1 / 2^(levels - 1) of the full size."""
+        _capi.check(self._lib.pm_estimator_set_pyramid_levels(self._h, int(levels)))
         _capi.check(self._lib.pm_estimator_set_pyramid(self._h, int(half_iterations)))
 
     def set_far_probes(self, n: int) -> None:

@@ -38,10 +38,10 @@ def _batch(scene, iters=2, patch=7, tile=PM_TILE_16x16, graph=True):
     return b
 
 
-def _single(scene, k, iters=2, patch=7, sample=False, pyramid=0):
+def _single(scene, k, iters=2, patch=7, sample=False, pyramid=0, levels=2):
     ref, nb = scene.jobs[k]
     e = DepthmapEstimator()
-    e.set_pyramid(pyramid)
+    e.set_pyramid(pyramid, levels=levels)
     for v in [ref] + list(nb):
         V = scene.views[v]
         e.add_view(V.K, V.R, V.t, V.image, V.mask)
@@ -69,19 +69,24 @@ def test_batch_equals_single_view_runs(street, tile):
         assert (got[0] > 0).mean() > 0.3
 
 
-def test_batch_pyramid_equals_single_view_runs(street):
-    """With the half-resolution level (2 half + 2 full iterations), job k
-    still equals the estimator with seed + k bit for bit, in a captured
-    graph (replayed twice) and eagerly; sample mode ignores the level."""
-    print("batch vs single-view estimator with the half-resolution level", flush=True)
+@pytest.mark.parametrize("levels", [2, 3])
+def test_batch_pyramid_equals_single_view_runs(street, levels):
+    """With the pyramid (2 iterations on each coarser level -- 1/2, and 1/4
+    with 3 levels -- + 2 full iterations), job k still equals the estimator
+    with seed + k bit for bit, in a captured graph (replayed twice) and
+    eagerly; the statistics count every level's launches; sample mode
+    ignores the pyramid."""
+    print(f"batch vs single-view estimator with a {levels}-level pyramid", flush=True)
     for graph in (True, False):
         b = _batch(street, graph=graph)
-        b.set_pyramid(2)
+        b.set_pyramid(2, levels=levels)
         for rep in range(3 if graph else 1):
             b.run()
+            assert b.stats()["launches"] == 2 * 2 * levels
             for k in range(len(street.jobs)):
                 got = b.result(k)
-                assert _equal(got, _single(street, k, pyramid=2)), f"job {k} graph {graph} run {rep}"
+                want = _single(street, k, pyramid=2, levels=levels)
+                assert _equal(got, want), f"job {k} graph {graph} run {rep}"
                 assert (got[0] > 0).mean() > 0.3
     b.run(sample=True)
     for k in range(len(street.jobs)):

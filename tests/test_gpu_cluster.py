@@ -62,6 +62,43 @@ def test_cluster_surface(cfg1_scene):
     assert DepthmapClusterEstimator.is_available() and DepthmapClusterEstimator.num_devices() >= 1
 
 
+def test_cluster_hierarchy_levels(cfg1_scene):
+    """hierarchy_levels = 3 (estimator.cc:140, 168-172: scales 1/4, 1/2, 1,
+    max_iterations at each): the depths equal the batched engine's with a
+    3-level pyramid of max_iterations per level; levels are dropped where
+    the smallest view would be narrower than one patch."""
+    print("DepthmapClusterEstimator with hierarchy levels", flush=True)
+    sc = cfg1_scene
+    v = sc.views
+    refs = [(0, [1, 2, 3, 4]), (3, [2, 4])]
+    ce = DepthmapClusterEstimator()
+    ce.set_max_iterations(2)
+    ce.set_patch_size(7)
+    ce.set_hierarchy_levels(3)
+    for r, nb in refs:
+        ce.begin_ref_view(v[r].K, v[r].R, v[r].t, v[r].image, sc.min_depth, sc.max_depth)
+        for s in nb:
+            ce.add_source_view(v[s].K, v[s].R, v[s].t, v[s].image)
+    res = ce.run()
+    assert ce.last_stats["launches"] == 3 * 2 * 2
+    b = DepthmapBatch()
+    ids = [b.add_view(x.K, x.R, x.t, x.image) for x in v]
+    for r, nb in refs:
+        b.add_job(ids[r], [ids[s] for s in nb], sc.min_depth, sc.max_depth)
+    b.set_patchmatch_iterations(2)
+    b.set_patch_size(7)
+    b.set_pyramid(2, levels=3)
+    b.run()
+    for k, (r, _) in enumerate(refs):
+        assert np.array_equal(res[k][0], oracle.cluster_readback(*b.result(k)[:3], sc.min_depth,
+                                                                 sc.max_depth)[0])
+        acc = scenes.depth_accuracy(res[k][0], v[r].depth, sc.min_depth, sc.max_depth)
+        print(k, round(acc, 4), flush=True)
+        assert acc > 0.7  # view 3 has two sources only: 0.77
+    assert ce.pyramid_levels(240, 7) == 3 and ce.pyramid_levels(20, 7) == 2
+    assert ce.pyramid_levels(6, 7) == 1
+
+
 @pytest.mark.gpu
 def test_device_queries_and_reservations():
     """Device name and memory (depthmap_bind.h:85-100) through the C ABI,

@@ -157,7 +157,7 @@ def _gpu_runs(scene, seeds, iters, sample=False, lim=(None, None), planes=16, **
     acc, score = [], []
     est = _setup(DepthmapEstimator(), scene, planes=planes, iters=iters)
     for k, v in settings.items():
-        getattr(est, "set_" + k)(v)
+        getattr(est, "set_" + k)(*(v if isinstance(v, tuple) else (v,)))
     for seed in seeds:
         est.set_seed(seed)
         d, _, s, _ = est.compute_patch_match_sample() if sample else est.compute_patch_match()
@@ -197,15 +197,17 @@ def test_patchmatch_full_view_accuracy_vs_oracle():
     assert score >= want["mean_score"]
 
 
-def test_pyramid_full_view_accuracy_vs_oracle():
-    """The half-resolution level (set_pyramid: 5 iterations on the views
-    box-averaged by 2 seed the full level) and 2 full-resolution iterations
-    on the 1600x1200 + 8-neighbour scene: accuracy and mean score over seeds
-    1-5 at least the oracle's 5-iteration full-view figures."""
-    print("PatchMatch with the half-resolution level vs oracle, 1600x1200 full view", flush=True)
+@pytest.mark.parametrize("coarse_iters,levels", [(5, 2), (2, 4)])
+def test_pyramid_full_view_accuracy_vs_oracle(coarse_iters, levels):
+    """The pyramid (set_pyramid: coarse_iters iterations on the views
+    box-averaged by 2 -- and by 4, 8 with levels 4, each level seeding the
+    next) and 2 full-resolution iterations on the 1600x1200 + 8-neighbour
+    scene: accuracy and mean score over seeds 1-5 at least the oracle's
+    5-iteration full-view figures."""
+    print(f"PatchMatch with a {levels}-level pyramid vs oracle, 1600x1200 full view", flush=True)
     want = _golden("config2_oracle_fullview.json")
     s = scenes.scene_config2()
-    acc, score = _gpu_runs(s, (1, 2, 3, 4, 5), 2, planes=50, pyramid=5)
+    acc, score = _gpu_runs(s, (1, 2, 3, 4, 5), 2, planes=50, pyramid=(coarse_iters, levels))
     print(f"oracle acc {want['mean_accuracy_within_1pct']:.5f} score {want['mean_score']:.5f} | "
           f"gpu acc {acc:.5f} score {score:.5f}", flush=True)
     assert acc >= want["mean_accuracy_within_1pct"]
@@ -232,6 +234,16 @@ def test_pyramid_runs_and_is_deterministic(cfg1_scene):
     assert scenes.depth_accuracy(d, cfg1_scene.ref.depth, 2.0, 20.0) > 0.88
     for x, y in zip(est.compute_patch_match_sample(), plain_s):
         assert np.array_equal(x, y)
+    # three levels (80x60, 160x120, 320x240): deterministic and valid; the
+    # statistics count every level's launches
+    est.set_pyramid(2, levels=3)
+    a = est.compute_patch_match()
+    st = est.stats()
+    assert st["launches"] == 2 * 3 + 2 * 2 * 2
+    for x, y in zip(a, est.compute_patch_match()):
+        assert np.array_equal(x, y)
+    assert (a[0] > 0).mean() > 0.8
+    assert scenes.depth_accuracy(a[0], cfg1_scene.ref.depth, 2.0, 20.0) > 0.88
     est.set_pyramid(0)
     for x, y in zip(est.compute_patch_match(), plain):
         assert np.array_equal(x, y)
