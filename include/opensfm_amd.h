@@ -139,6 +139,14 @@ int pm_estimator_set_pyramid(pm_estimator* est, int half_iterations);
  * level is itself seeded by a quarter level, and so on; every level coarser
  * than the full one runs half_iterations iterations. */
 int pm_estimator_set_pyramid_levels(pm_estimator* est, int levels);
+/* Lanes per pixel of the red/black PatchMatch launches (no counterpart in
+ * the reference, whose CPU loop scores the views of a candidate one after
+ * another, depthmap.cc:408-419): 0 (default) scores each pixel on 8 lanes,
+ * one share of the views each, when a launch has fewer than 131072 pixels of
+ * one colour (the coarse levels of a pyramid) and FP32 samples are used, else
+ * on one lane; 1 always one lane; 8 always 8 lanes (FP32 samples).  The maps
+ * are bit-identical for every setting; sample mode always uses one lane. */
+int pm_estimator_set_lanes_per_pixel(pm_estimator* est, int lanes);
 /* Random refinements per pixel update (default 6, the reference's count,
  * depthmap.cc:337; the 0.02 * 0.3^k / 0.5 * 0.8^k schedule restarts every
  * 6 steps). */

@@ -75,6 +75,7 @@ SIGNATURES = {
     "pm_estimator_set_far_probes": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_pyramid": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_pyramid_levels": (C.c_int, [_vp, C.c_int]),
+    "pm_estimator_set_lanes_per_pixel": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_refinement_steps": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_exploration": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_precision": (C.c_int, [_vp, C.c_int]),

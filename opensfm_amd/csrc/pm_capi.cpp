@@ -86,6 +86,8 @@ struct pm_estimator {
   // so on: every coarser level runs pyramid_iters iterations.
   int pyramid_iters = 0;
   int pyramid_levels = 2;
+  // lanes per pixel of the red/black launches (0: chosen by size)
+  int lanes_per_pixel = 0;
   std::unique_ptr<pm_estimator> half;
   DevBuf half_map;  // the half level's state as an initialisation map
 
@@ -329,6 +331,7 @@ int run_half(pm_estimator* e) {
   h->max_depth = e->max_depth;
   h->num_depth_planes = e->num_depth_planes;
   h->iterations = e->pyramid_iters;
+  h->lanes_per_pixel = e->lanes_per_pixel;
   h->pyramid_iters = e->pyramid_levels > 2 ? e->pyramid_iters : 0;
   h->pyramid_levels = e->pyramid_levels - 1;
   h->min_patch_variance = e->min_patch_variance;
@@ -443,7 +446,8 @@ int run(pm_estimator* e, int mode) {
       for (int colour = 0; colour < 2; ++colour) {
         HIP_CHECK(hipEventRecord(e->half_ev[2 * launches], e->stream));
         HIP_CHECK(pm::launch_pm_half(ah, sample, colour, it,
-                                     e->queues.as<unsigned>() + 8 * launches, e->stream));
+                                     e->queues.as<unsigned>() + 8 * launches, e->stream,
+                                     e->lanes_per_pixel));
         HIP_CHECK(hipEventRecord(e->half_ev[2 * launches + 1], e->stream));
         ++launches;
       }
@@ -879,6 +883,15 @@ int pm_estimator_set_pyramid(pm_estimator* e, int half_iterations) {
   if (!e) return fail(PM_ERR_INVALID, "null estimator");
   if (half_iterations < 0) return fail(PM_ERR_INVALID, "half-level iterations must be >= 0");
   e->pyramid_iters = half_iterations;
+  return PM_OK;
+}
+
+int pm_estimator_set_lanes_per_pixel(pm_estimator* e, int lanes) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (lanes != 0 && lanes != 1 && lanes != pm::kLanesPerPixelSmall)
+    return fail(PM_ERR_INVALID, "lanes per pixel must be 0, 1 or " +
+                                    std::to_string(pm::kLanesPerPixelSmall));
+  e->lanes_per_pixel = lanes;
   return PM_OK;
 }
 

@@ -324,6 +324,39 @@ __device__ __forceinline__ uint2 fetch_quad16_xy(const QuadTex& t, float x, floa
   return make_uint2(v[0], v[1]);
 }
 
+// A texture whose view may differ between the lanes of a wave (the lane
+// groups of the red/black kernel, where lane k of a group scores its own
+// views): per-lane base pointer and size, read with global loads.  Texel
+// (ix, iy) is the record of make_tex / make_tex16 at the same clamped
+// indices, and a negative record -- row -1, or column -1 of row 0, which the
+// buffer unit returns as 0 -- reads 0 here too; every non-negative record lies
+// inside the texture (at most (H - 1) pitch + W - 1 < pitch (H + 1) -
+// (pitch + 1) records).
+struct LaneTex {
+  const char* base;  // texel (0, 0)
+  int pitch;
+  float xmax, ymax;
+};
+
+template <int kWords>
+__device__ __forceinline__ LaneTex make_lane_tex(const uint32_t* quad, int W, int H) {
+  LaneTex t;
+  t.pitch = W + 1;
+  t.base = reinterpret_cast<const char*>(quad + kWords * (t.pitch + 1));
+  t.xmax = (float)(W - 1);
+  t.ymax = (float)(H - 1);
+  return t;
+}
+
+template <typename T>
+__device__ __forceinline__ T fetch_lane_xy(const LaneTex& t, float x, float y) {
+  const int ix = cvt_floor(__builtin_amdgcn_fmed3f(x, -1.0f, t.xmax));
+  const int iy = cvt_floor(__builtin_amdgcn_fmed3f(y, -1.0f, t.ymax));
+  const int rec = __mul24(iy, t.pitch) + ix;
+  const T v = *reinterpret_cast<const T*>(t.base + (size_t)max(rec, 0) * sizeof(T));
+  return rec < 0 ? T{} : v;
+}
+
 // Texel sources for the fast scorer: the texel holding (floor x, floor y)
 // from the unfloored coordinates, and its bilinear blend at fractions
 // (fx, fy).
@@ -367,6 +400,26 @@ struct CoefFetch {
     const float top = __builtin_fmaf(fx, (float)ab[1], (float)ab[0]);
     const float dif = __builtin_fmaf(fx, (float)cd[1], (float)cd[0]);
     return __builtin_fmaf(fy, dif, top);
+  }
+};
+
+// CoefFetch and QuadFetch over a LaneTex (per-lane views).
+struct CoefLaneFetch {
+  LaneTex tex;
+  __device__ __forceinline__ uint2 operator()(float x, float y) const {
+    return fetch_lane_xy<uint2>(tex, x, y);
+  }
+  static __device__ __forceinline__ float bilerp(uint2 t, float fx, float fy) {
+    return CoefFetch::bilerp(t, fx, fy);
+  }
+};
+struct QuadLaneFetch {
+  LaneTex tex;
+  __device__ __forceinline__ uint32_t operator()(float x, float y) const {
+    return fetch_lane_xy<uint32_t>(tex, x, y);
+  }
+  static __device__ __forceinline__ float bilerp(uint32_t t, float fx, float fy) {
+    return QuadFetch::bilerp(t, fx, fy);
   }
 };
 

@@ -36,8 +36,12 @@ hipError_t launch_state_to_map(const float* state, int W, int H, float log_min, 
 hipError_t launch_state_to_map_batch(const PmJob* jobs, int njobs, int pix_total,
                                      const float* state0, int median, float* map0,
                                      hipStream_t s);
+// lanes_per_pixel: 0 chooses (kLanesPerPixelSmall lanes per pixel below
+// kGroupBelowPixels pixels of one colour, FP32 samples, not in sample mode),
+// 1 forces one lane per pixel, kLanesPerPixelSmall forces the groups (FP32).
 hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
-                          unsigned* queues, hipStream_t s);
+                          unsigned* queues, hipStream_t s, int lanes_per_pixel = 0);
+bool half_uses_groups(const PmArgs& a, int lanes_per_pixel);
 // Box downsampling (mode 0 mean, 1 min) by an integer factor.
 hipError_t launch_downsample_u8(const uint8_t* in, int W, int H, int sc,
                                 int mode, uint8_t* out, int Wl, int Hl,

@@ -317,11 +317,24 @@ __device__ __forceinline__ bool warp_to_view(const PmView& V,
 // Fast NCC of one view under a warp, with the view's texture in the given
 // sample precision (0 FP32 from FP16 coefficients, 1 FP16, 2 BF16,
 // kPrecFp32Quads FP32 from u8 quads).
-template <int P, int Prec, int kPipe = 1>
+// kLaneView: the view may differ between lanes (global loads through a
+// LaneTex instead of a wave-uniform buffer descriptor; same samples).
+template <int P, int Prec, int kPipe = 1, bool kLaneView = false>
 __device__ __forceinline__ float score_view_fast(const RefPatch<P>& rp,
                                                  const Warp& wp,
                                                  const PmView& V) {
-  if constexpr (Prec == 0) {
+  if constexpr (kLaneView) {
+    static_assert(Prec == 0 || Prec == kPrecFp32Quads, "per-lane views: FP32 samples");
+    if constexpr (Prec == 0) {
+      CoefLaneFetch g;
+      g.tex = make_lane_tex<2>(V.quadc, V.width, V.height);
+      return score_fast_affine<P, kPipe>(rp, wp, g);
+    } else {
+      QuadLaneFetch g;
+      g.tex = make_lane_tex<1>(V.quad, V.width, V.height);
+      return score_fast_affine<P, kPipe>(rp, wp, g);
+    }
+  } else if constexpr (Prec == 0) {
     CoefFetch g;
     g.tex = make_tex16(V.quadc, V.width, V.height);
     return score_fast_affine<P, kPipe>(rp, wp, g);
@@ -339,27 +352,44 @@ __device__ __forceinline__ float score_view_fast(const RefPatch<P>& rp,
 // Score a plane against every neighbour view from global memory: max NCC
 // with the lowest view index winning ties (ComputePlaneScore,
 // depthmap.cc:408-419).  Returns the score; *nb gets the view and *depth the
-// plane's depth at the pixel.
-template <int P, int Prec, int kPipe = 1>
+// plane's depth at the pixel.  With G > 1 the G adjacent lanes of a group
+// score the same plane of the same pixel, lane `sub` the views 1 + sub,
+// 1 + sub + G, ..., and the group's results are merged by the same rule (max,
+// then the lowest view; -1 and view 0 when no view scores above -1), so every
+// lane of the group returns the bits of the one-lane loop.
+template <int P, int Prec, int kPipe = 1, int G = 1>
 __device__ __forceinline__ float score_plane_all(const PmArgs& a,
                                                  const RefPatch<P>& rp,
                                                  const PixelRay& r, int i,
                                                  int j, float v0, float v1,
                                                  float v2, int* nb,
-                                                 float* depth) {
+                                                 float* depth, int sub = 0) {
   const PlaneAtPixel p = plane_at_pixel(a, r, v0, v1, v2);
   *depth = __frcp_rn(fmaxf(1e-6f, -p.gq));
   const float fj = (float)j, fi = (float)i;
   float best = -1.0f;
   int bn = 0;
-  for (int v = 1; v < a.nviews; ++v) {
+  for (int v = 1 + sub; v < a.nviews; v += G) {
     const PmView& V = a.views[v];
     Warp wp;
     float s = -1.0f;
-    if (warp_to_view(V, p, fj, fi, &wp)) s = score_view_fast<P, Prec, kPipe>(rp, wp, V);
+    if (warp_to_view(V, p, fj, fi, &wp)) s = score_view_fast<P, Prec, kPipe, (G > 1)>(rp, wp, V);
     if (s > best) {
       best = s;
       bn = v;
+    }
+  }
+  if constexpr (G > 1) {
+    // equal scores are either both -1 with view 0 or both found, so the
+    // lower view index decides
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+      const float ob = __shfl_xor(best, o, G);
+      const int on = __shfl_xor(bn, o, G);
+      if (ob > best || (ob == best && on < bn)) {
+        best = ob;
+        bn = on;
+      }
     }
   }
   *nb = bn;
@@ -662,12 +692,20 @@ constexpr float kExploreBelow = 0.97f;
 // faster; three rows were 1 % slower than two, on the u8 quads of the first
 // iteration it was slower, and at patch 5 its registers cost the batched
 // kernel its third wave per SIMD).
-template <int P, bool kSample, int TX, int TY, int Prec, bool kLean = false>
+// G > 1 (not in sample mode): G adjacent lanes per pixel, the tile being TX x
+// TY pixels of G * TX * TY lanes; every lane of a group runs the pixel's
+// candidate loop (same state reads, same random numbers) and scores its share
+// of the views (score_plane_all), and lane 0 of the group writes the result.
+// The maps are bit-identical for every G; more lanes per pixel fill the chip
+// when a launch has few pixels.
+template <int P, bool kSample, int TX, int TY, int Prec, bool kLean = false, int G = 1>
 __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
                                                int iter, int tile_x,
                                                int tile_y, float4* __restrict__ hist) {
-  const int cx = tile_x * TX + (int)(threadIdx.x % TX);
-  const int i = tile_y * TY + (int)(threadIdx.x / TX);
+  static_assert(G == 1 || !kSample, "lane groups score all views of a candidate");
+  const int sub = (int)(threadIdx.x % G);
+  const int cx = tile_x * TX + (int)((threadIdx.x / G) % TX);
+  const int i = tile_y * TY + (int)((threadIdx.x / G) / TX);
   const int j = 2 * cx + ((i + colour) & 1);
   if (!is_interior(a, i, j)) return 0;
   const size_t idx = (size_t)i * a.W + j;
@@ -879,8 +917,8 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
       int nb;
       float dep;
       const float s =
-          score_plane_all<P, Prec, kPipe>(a, rp, ray, i, j, v0, v1, v2, &nb, &dep);
-      tries += (unsigned)(a.nviews - 1);
+          score_plane_all<P, Prec, kPipe, G>(a, rp, ray, i, j, v0, v1, v2, &nb, &dep, sub);
+      tries += sub == 0 ? (unsigned)(a.nviews - 1) : 0u;
       if (s > best_s) {
         best_s = s;
         best_n = nb;
@@ -888,9 +926,11 @@ __device__ __forceinline__ unsigned half_pixel(const PmArgs& a, int colour,
       }
     }
   }
-  reinterpret_cast<float4*>(a.state)[idx] = cur;
-  a.score[idx] = best_s;
-  a.nghbr[idx] = best_n;
+  if (sub == 0) {
+    reinterpret_cast<float4*>(a.state)[idx] = cur;
+    a.score[idx] = best_s;
+    a.nghbr[idx] = best_n;
+  }
   return tries;
 }
 
@@ -1659,6 +1699,22 @@ __global__ __launch_bounds__(256) void k_pm_half(PmArgs a, int colour, int iter,
   if (a.counter) wave_count(a.counter, n);
 }
 
+// Single-view red/black half-iteration with G lanes per pixel, for launches
+// with few pixels (the coarse levels of a pyramid), where one lane per pixel
+// leaves most of the chip idle and each wave waits on its own serial loop
+// over the views.  One workgroup per tile of 8 x (32 / (G / 8)) pixels; maps
+// bit-identical to k_pm_half's.
+template <int P, int Prec, int G>
+__global__ __launch_bounds__(256) void k_pm_half_group(PmArgs a, int colour, int iter) {
+  constexpr int TX = 8, TY = 256 / G / TX;
+  __shared__ float4 hist[8 * 256];
+  const int tiles_x = ((a.W + 1) / 2 + TX - 1) / TX;
+  const int t = (int)blockIdx.x;
+  const unsigned n = half_pixel<P, false, TX, TY, Prec, true, G>(a, colour, iter, t % tiles_x,
+                                                                 t / tiles_x, hist);
+  if (a.counter) wave_count(a.counter, n);
+}
+
 // Batched red/black half-iteration: the flat tile space of all jobs split
 // into 8 bands, pulled XCD by XCD as in k_pm_half.  queues[0..7] start at
 // zero.
@@ -2062,9 +2118,35 @@ static hipError_t half_banded(const PmArgs& a, int colour, int iter,
   return hipGetLastError();
 }
 
+template <int P, int Prec>
+static hipError_t half_group(const PmArgs& a, int colour, int iter, hipStream_t s) {
+  constexpr int G = kLanesPerPixelSmall, TX = 8, TY = 256 / G / TX;
+  const int ntiles = (((a.W + 1) / 2 + TX - 1) / TX) * ((a.H + TY - 1) / TY);
+  hipLaunchKernelGGL((k_pm_half_group<P, Prec, G>), dim3(ntiles), dim3(256), 0, s, a, colour,
+                     iter);
+  return hipGetLastError();
+}
+
+bool half_uses_groups(const PmArgs& a, int lanes_per_pixel) {
+  if (lanes_per_pixel == 1) return false;
+  if (a.precision != 0 && a.precision != kPrecFp32Quads) return false;
+  return lanes_per_pixel == kLanesPerPixelSmall ||
+         (size_t)a.W * a.H / 2 < kGroupBelowPixels;
+}
+
 hipError_t launch_pm_half(const PmArgs& a, bool sample, int colour, int iter,
-                          unsigned* queues, hipStream_t s) {
+                          unsigned* queues, hipStream_t s, int lanes_per_pixel) {
   const int P_ = 2 * a.hpz + 1;
+  if (!sample && half_uses_groups(a, lanes_per_pixel)) {
+    if (a.precision == 0) {
+      constexpr int Prec = 0;
+      PM_DISPATCH_P(P_, return (half_group<P, Prec>(a, colour, iter, s)));
+    } else {
+      constexpr int Prec = kPrecFp32Quads;
+      PM_DISPATCH_P(P_, return (half_group<P, Prec>(a, colour, iter, s)));
+    }
+    return hipErrorInvalidValue;
+  }
   if (sample) {
     PM_DISPATCH_PREC(a.precision, P_,
                      return (half_banded<P, true, Prec>(a, colour, iter, queues, s)));

@@ -1,6 +1,7 @@
 // Plain-data structures shared by the host code and the kernels.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 namespace pm {
@@ -29,6 +30,11 @@ constexpr int kConsistFinalSupport = 10;
 // s + k) and an estimator with seed s + k draw the same numbers at both
 // levels.
 constexpr uint64_t kHalfLevelSeed = 0x9E3779B97F4A7C15ull;
+// Single-view red/black launches with fewer pixels of one colour than this
+// (about the lanes the chip holds at 2 waves per SIMD) score each pixel on
+// kLanesPerPixelSmall lanes, one share of the views each (k_pm_half_group).
+constexpr int kLanesPerPixelSmall = 8;
+constexpr size_t kGroupBelowPixels = 131072;
 
 // One view as the kernels see it.  For neighbour view v (v >= 1) the plane-
 // induced homography of plane v_pi (in the reference camera frame) is

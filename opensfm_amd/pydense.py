@@ -95,6 +95,12 @@ class DepthmapEstimator:
         _capi.check(self._lib.pm_estimator_set_pyramid_levels(self._h, int(levels)))
         _capi.check(self._lib.pm_estimator_set_pyramid(self._h, int(half_iterations)))
 
+    def set_lanes_per_pixel(self, lanes: int) -> None:
+        """Lanes per pixel of the red/black launches: 0 (default) 8 for
+        launches with few pixels (FP32 samples), else 1; 1 or 8 forces it.
+        The maps are the same for every setting."""
+        _capi.check(self._lib.pm_estimator_set_lanes_per_pixel(self._h, int(lanes)))
+
     def set_far_probes(self, n: int) -> None:
         _capi.check(self._lib.pm_estimator_set_far_probes(self._h, int(n)))
 

@@ -249,6 +249,34 @@ def test_pyramid_runs_and_is_deterministic(cfg1_scene):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("patch", [3, 7, 11])
+def test_lane_groups_bit_identical(cfg1_scene, patch):
+    """Red/black launches scoring each pixel on 8 lanes (one share of the
+    views each, merged by the max-then-lowest-view rule of
+    depthmap.cc:408-419) give the maps of one lane per pixel bit for bit:
+    4 neighbours (lanes 5-8 of a group idle), 10 neighbours (two views on
+    some lanes), the u8-quad first iteration and seeded pyramid levels; the
+    scoring counts agree too."""
+    print(f"lane groups vs one lane per pixel, patch {patch}", flush=True)
+    wide = scenes.make_scene(160, 120, 10, 4, seed=21, min_depth=2.0, max_depth=20.0)
+    for sc, pyramid in ((cfg1_scene, 0), (cfg1_scene, 2), (wide, 0)):
+        est = _setup(DepthmapEstimator(), sc, patch=patch)
+        est.set_pyramid(pyramid, levels=3)
+        est.set_seed(5)
+        maps, counts = [], []
+        for lanes in (1, 8, 0):
+            est.set_lanes_per_pixel(lanes)
+            maps.append(est.compute_patch_match())
+            counts.append(est.stats()["scorings"])
+        for m in maps[1:]:
+            for x, y in zip(maps[0], m):
+                assert np.array_equal(x, y)
+        assert counts[0] == counts[1] == counts[2]
+        assert (maps[0][0] > 0).mean() > 0.5
+    with pytest.raises(ValueError):
+        est.set_lanes_per_pixel(4)
+
+
 def test_reference_candidate_list_runs(cfg1_scene):
     """Exploration and the median-consistent re-selection off (the
     reference's candidate list on red/black updates): valid maps."""
