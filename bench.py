@@ -826,20 +826,25 @@ def main():
     log(f"matched pixel updates: {matched_updates}")
 
     # ComputePatchMatchSample (depthmap.cc:215-227): each candidate scored
-    # against one view, the reference's iterations
-    est.set_patchmatch_iterations(w["iterations"])
-    est.run(PM_MODE_PATCHMATCH_SAMPLE)
-    t5 = time.perf_counter()
-    est.run(PM_MODE_PATCHMATCH_SAMPLE)
-    smp_s = time.perf_counter() - t5
-    smp_depth = est.download()[0]
-    sample_mode = {
-        "iterations": w["iterations"],
-        "mpix_s": round(w["width"] * w["height"] / smp_s / 1e6, 3),
-        "accuracy_within_1pct": round(scenes.depth_accuracy(smp_depth, scene.ref.depth,
-                                                            w["min_depth"], w["max_depth"]), 4)}
-    log(f"sample mode: {sample_mode}")
+    # against one view, at the reference's iterations on one level, and with
+    # the headline's pyramid (every level sampling)
+    def sample_run():
+        est.run(PM_MODE_PATCHMATCH_SAMPLE)
+        t5 = time.perf_counter()
+        est.run(PM_MODE_PATCHMATCH_SAMPLE)
+        smp_s = time.perf_counter() - t5
+        smp_depth = est.download()[0]
+        return {"mpix_s": round(w["width"] * w["height"] / smp_s / 1e6, 3),
+                "accuracy_within_1pct": round(scenes.depth_accuracy(
+                    smp_depth, scene.ref.depth, w["min_depth"], w["max_depth"]), 4)}
+
+    schedule(False)
+    sample_mode = {"iterations": w["iterations"], **sample_run()}
     schedule(True)
+    sample_mode["pyramid"] = {"pyramid_levels": w["pyramid_levels"],
+                              "coarse_level_iterations": w["half_iterations"],
+                              "iterations": w["full_iterations"], **sample_run()}
+    log(f"sample mode: {sample_mode}")
 
     npx = w["width"] * w["height"]
     ms_per_step = elapsed / args.steps * 1e3

@@ -130,8 +130,8 @@ int pm_estimator_set_far_probes(pm_estimator* est, int n);
  * the median test keeps it -- seed this level's initialisation (bilinear in
  * depth and slopes, as an initialisation map would); the rest start from
  * random planes.  Then set_patchmatch_iterations iterations run here.
- * Default 0 (off); ignored in sample mode and when an initialisation map
- * is set. */
+ * Default 0 (off); ignored when an initialisation map is set.  In sample
+ * mode every level runs in sample mode. */
 int pm_estimator_set_pyramid(pm_estimator* est, int half_iterations);
 /* Levels of that pyramid, the full resolution included (default 2; in
  * [2, 8]; the reference's OpenCL engine's hierarchy_levels, estimator.h:28,

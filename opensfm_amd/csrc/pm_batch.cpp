@@ -370,8 +370,8 @@ void prepare(pm_batch* b, int count) {
   HIP_CHECK(hipMemcpy(b->jobtab.p, jt.data(), jt.size() * sizeof(pm::PmJob),
                       hipMemcpyHostToDevice));
   if (b->pyramid_iters > 0) {
-    // plain-mode runs: jobs without a caller's map start from the half
-    // level's, sampled bilinearly with no spread (estimator: run_half)
+    // jobs without a caller's map start from the half level's, sampled
+    // bilinearly with no spread (estimator: run_half)
     ensure_half(b, count);
     const pm_batch& h = *b->half;
     b->half_map.ensure(h.total_px * 4 * sizeof(float));
@@ -402,11 +402,12 @@ void prepare(pm_batch* b, int count) {
 void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   const bool sample = mode == PM_MODE_PATCHMATCH_SAMPLE;
   const int nj = b->prepared_jobs;
-  const bool pyramid = !sample && b->pyramid_iters > 0;
+  const bool pyramid = b->pyramid_iters > 0;
   const pm::PmJob* jt = (pyramid ? b->jobtab_pyr : b->jobtab).as<pm::PmJob>();
   if (pyramid) {
+    // the coarser level in the same mode (sample mode's levels sample too)
     pm_batch* h = b->half.get();
-    enqueue(h, PM_MODE_PATCHMATCH, nullptr);
+    enqueue(h, mode, nullptr);
     HIP_CHECK(pm::launch_state_to_map_batch(h->jobtab.as<pm::PmJob>(), h->prepared_jobs,
                                             h->pix_total, h->state.as<float>(),
                                             b->post_process, b->half_map.as<float>(),
@@ -514,15 +515,13 @@ int run(pm_batch* b, int mode) {
   b->stats.launches = 2 * b->iterations;
   // the coarser levels ran in the same sequence: their launches of the same
   // kernels count too (the device time above already holds them)
-  if (mode == PM_MODE_PATCHMATCH) {
-    for (const pm_batch* c = b; c->pyramid_iters > 0; c = c->half.get()) {
-      const pm_batch* h = c->half.get();
-      unsigned long long hc[2] = {0, 0};
-      HIP_CHECK(hipMemcpy(hc, h->counter.p, sizeof(hc), hipMemcpyDeviceToHost));
-      b->stats.scorings += (int64_t)(hc[0] + hc[1]);
-      b->stats.dominant_scorings += (int64_t)hc[1];
-      b->stats.launches += 2 * h->iterations;
-    }
+  for (const pm_batch* c = b; c->pyramid_iters > 0; c = c->half.get()) {
+    const pm_batch* h = c->half.get();
+    unsigned long long hc[2] = {0, 0};
+    HIP_CHECK(hipMemcpy(hc, h->counter.p, sizeof(hc), hipMemcpyDeviceToHost));
+    b->stats.scorings += (int64_t)(hc[0] + hc[1]);
+    b->stats.dominant_scorings += (int64_t)hc[1];
+    b->stats.launches += 2 * h->iterations;
   }
   b->stats.dominant_ms = ms;
   b->stats.total_ms = ms;

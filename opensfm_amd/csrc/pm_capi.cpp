@@ -323,7 +323,7 @@ int run(pm_estimator* e, int mode);
 // seed) and turns its state into e->half_map: per half-level pixel the
 // normalised log depth, the normal's slopes and a confidence logit (+10 where
 // the depth is live and PostProcess's median test keeps it, else -10).
-int run_half(pm_estimator* e) {
+int run_half(pm_estimator* e, int mode) {
   ensure_half(e);
   pm_estimator* h = e->half.get();
   h->patch_size = e->patch_size;
@@ -342,7 +342,7 @@ int run_half(pm_estimator* e) {
   h->precision = e->precision;
   h->post_process = e->post_process;
   h->consistency = e->consistency;
-  int rc = run(h, PM_MODE_PATCHMATCH);
+  int rc = run(h, mode);
   if (rc) return rc;
   const HostView& r = *h->views[0];
   e->half_map.ensure((size_t)r.width * r.height * 4 * sizeof(float));
@@ -363,9 +363,9 @@ int run(pm_estimator* e, int mode) {
   DeviceGuard dg(e->device);
   e->have_result = false;
   // the half level seeds plain PatchMatch when no initialisation map is set
-  const bool pyramid = mode == PM_MODE_PATCHMATCH && e->pyramid_iters > 0 && !e->init_map;
+  const bool pyramid = mode != PM_MODE_PLANE_SWEEP && e->pyramid_iters > 0 && !e->init_map;
   if (pyramid) {
-    rc = run_half(e);
+    rc = run_half(e, mode);
     if (rc) return rc;
   }
   pm::PmArgs a = prepare(e);

@@ -74,8 +74,8 @@ def test_batch_pyramid_equals_single_view_runs(street, levels):
     """With the pyramid (2 iterations on each coarser level -- 1/2, and 1/4
     with 3 levels -- + 2 full iterations), job k still equals the estimator
     with seed + k bit for bit, in a captured graph (replayed twice) and
-    eagerly; the statistics count every level's launches; sample mode
-    ignores the pyramid."""
+    eagerly; the statistics count every level's launches; in sample mode
+    too (every level samples)."""
     print(f"batch vs single-view estimator with a {levels}-level pyramid", flush=True)
     for graph in (True, False):
         b = _batch(street, graph=graph)
@@ -89,8 +89,10 @@ def test_batch_pyramid_equals_single_view_runs(street, levels):
                 assert _equal(got, want), f"job {k} graph {graph} run {rep}"
                 assert (got[0] > 0).mean() > 0.3
     b.run(sample=True)
+    assert b.stats()["launches"] == 2 * 2 * levels
     for k in range(len(street.jobs)):
-        assert _equal(b.result(k), _single(street, k, sample=True)), f"sample job {k}"
+        want = _single(street, k, sample=True, pyramid=2, levels=levels)
+        assert _equal(b.result(k), want), f"sample job {k}"
 
 
 def test_batch_graph_replays_and_eager_agree(street):
