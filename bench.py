@@ -893,8 +893,10 @@ def main():
         "plane_sweep": sweep,
         "roofline": {
             "kernel": "k_pm_half<7> (red/black PatchMatch half-iteration, every launch of a "
-                      "compute at all pyramid levels; random-plane launches read the u8 quads, "
-                      "the others the FP16 coefficients)",
+                      "compute at all pyramid levels: k_pm_half_group<7> with 8 lanes per pixel "
+                      "on the 200x150 and 400x300 levels, k_pm_half<7> on the others; "
+                      "random-plane launches read the u8 quads, the others the FP16 "
+                      "coefficients)",
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
