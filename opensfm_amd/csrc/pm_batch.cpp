@@ -136,12 +136,21 @@ int validate(pm_batch* b) {
   if (!pm::precision_supported(b->precision, b->patch_size))
     return fail(PM_ERR_INVALID, "FP16/BF16 samples need patch size 5 or 7");
   const int hpz = (b->patch_size - 1) / 2;
+  // with the pyramid, the coarsest level's views (box-averaged by
+  // 2^(levels - 1)): references at least one patch, the others 2 x 2
+  const int sh = b->pyramid_iters > 0 ? b->pyramid_levels - 1 : 0;
   for (size_t k = 0; k < b->jobs.size(); ++k) {
     const PoolView& r = *b->views[b->jobs[k].ref];
-    if (r.width < 2 * hpz + 1 || r.height < 2 * hpz + 1)
+    if ((r.width >> sh) < 2 * hpz + 1 || (r.height >> sh) < 2 * hpz + 1)
       return fail(PM_ERR_INVALID, "job " + std::to_string(k) +
-                                      ": reference view smaller than one patch");
+                                      (sh ? ": reference view smaller than one patch at the "
+                                            "pyramid's coarsest level"
+                                          : ": reference view smaller than one patch"));
   }
+  if (sh)
+    for (const auto& V : b->views)
+      if ((V->width >> sh) < 2 || (V->height >> sh) < 2)
+        return fail(PM_ERR_INVALID, "a view is too small for the pyramid's coarsest level");
   return PM_OK;
 }
 

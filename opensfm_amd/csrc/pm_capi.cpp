@@ -136,7 +136,8 @@ void homography_baked(const double* K1inv, const double* Q2, const double* a2,
   for (int k = 0; k < 9; ++k) H[k] = (float)T[k];
 }
 
-int validate_for_compute(pm_estimator* e) {
+// with_pyramid: the compute runs the pyramid when one is set
+int validate_for_compute(pm_estimator* e, bool with_pyramid = false) {
   if (e->views.size() < 2)
     return fail(PM_ERR_INVALID, "need a reference view and at least one neighbour view (got " +
                                     std::to_string(e->views.size()) + ")");
@@ -149,6 +150,17 @@ int validate_for_compute(pm_estimator* e) {
   const int hpz = (e->patch_size - 1) / 2;
   if (r.width < 2 * hpz + 1 || r.height < 2 * hpz + 1)
     return fail(PM_ERR_INVALID, "reference view smaller than one patch");
+  if (with_pyramid && e->pyramid_iters > 0 && !e->init_map) {
+    // the coarsest level: every view box-averaged by 2^(levels - 1), the
+    // reference at least one patch
+    const int sh = e->pyramid_levels - 1;
+    bool ok = (r.width >> sh) >= 2 * hpz + 1 && (r.height >> sh) >= 2 * hpz + 1;
+    for (const auto& V : e->views) ok = ok && (V->width >> sh) >= 2 && (V->height >> sh) >= 2;
+    if (!ok)
+      return fail(PM_ERR_INVALID, "views too small for a pyramid of " +
+                                      std::to_string(e->pyramid_levels) +
+                                      " levels (the coarsest reference must hold one patch)");
+  }
   return PM_OK;
 }
 
@@ -353,7 +365,7 @@ int run_half(pm_estimator* e, int mode) {
 }
 
 int run(pm_estimator* e, int mode) {
-  int rc = validate_for_compute(e);
+  int rc = validate_for_compute(e, mode != PM_MODE_PLANE_SWEEP);
   if (rc) return rc;
   if (mode != PM_MODE_PLANE_SWEEP && mode != PM_MODE_PATCHMATCH &&
       mode != PM_MODE_PATCHMATCH_SAMPLE)

@@ -195,6 +195,13 @@ def test_batch_argument_errors(street):
         b.run()  # no jobs
     with pytest.raises(ValueError):
         b.set_tile(7)
+    # a pyramid deeper than the views allow (160x120 by 2^7: 1x0 pixels)
+    p = _batch(street)
+    p.set_pyramid(2, levels=8)
+    with pytest.raises(ValueError):
+        p.run()
+    with pytest.raises(ValueError):
+        p.set_pyramid(2, levels=9)
 
 
 def test_config4_full_size_batch():

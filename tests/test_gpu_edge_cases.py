@@ -97,6 +97,32 @@ def test_ragged_sizes(ragged):
     assert (depth > 0).mean() > 0.3
 
 
+def test_pyramid_on_ragged_sizes(ragged):
+    """The pyramid on a 67x45 reference with a 50x33 neighbour: 3 levels
+    (16x11 and 12x8 at the coarsest) run, deterministic and valid, also in
+    sample mode; 4 levels (an 8x5 coarsest reference, smaller than one
+    patch) raise ValueError, while the plane sweep ignores the pyramid."""
+    print("pyramid on a 67x45 reference with a 50x33 neighbour", flush=True)
+    views = [ragged.views[0], _cropped(ragged.views[1], 9, 5, 50, 33), ragged.views[2]]
+    est = _setup(DepthmapEstimator(), views, iters=2, seed=4)
+    est.set_pyramid(2, levels=3)
+    a = est.compute_patch_match()
+    for x, y in zip(a, est.compute_patch_match()):
+        assert np.array_equal(x, y)
+    depth, plane, score, nghbr = a
+    assert np.all(np.isfinite(depth)) and np.all(depth >= 0)
+    assert np.all(score <= 1.0 + 1e-5) and np.all(score >= -1.0)
+    assert (depth > 0).mean() > 0.3
+    s = est.compute_patch_match_sample()
+    assert np.all(np.isfinite(s[0])) and (s[0] > 0).mean() > 0.2
+    est.set_pyramid(2, levels=4)
+    with pytest.raises(ValueError):
+        est.compute_patch_match()
+    with pytest.raises(ValueError):
+        est.compute_patch_match_sample()
+    assert est.compute_plane_sweep()[0].shape == depth.shape
+
+
 def test_one_patch_reference_and_size_errors(ragged):
     print("7x7 reference (one patch), 6x7 and no-neighbour errors", flush=True)
     v0 = ragged.views[0]
