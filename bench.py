@@ -8,8 +8,8 @@ FP32 NCC.  Inputs are resident in HBM when the timed region starts (views
 uploaded by add_view); each step runs the schedule that reaches the
 reference's 5-iteration accuracy and mean score (checked over seeds 1-5
 against the oracle's recorded figures): a 4-level pyramid (the views
-box-averaged by 8, 4 and 2), 2 iterations per coarser level, each level's
-planes seeding the next, then 2 full-resolution iterations and the median
+box-averaged by 8, 4 and 2 with 2, 2 and 1 iterations), each level's planes
+seeding the next, then 2 full-resolution iterations and the median
 post-process, into device-resident output maps.
 Plain PatchMatch at the reference's 5 iterations is timed beside it.
 
@@ -43,7 +43,7 @@ FLOP_PER_SCORING = 160
 WORKLOAD = dict(width=1600, height=1200, neighbours=8, boxes=20,
                 terrain=(1.5, 6.0), min_depth=1.0, max_depth=40.0, patch=7,
                 min_patch_sd=1.0, iterations=5, seed=2000,
-                pyramid_levels=4, half_iterations=2, full_iterations=2)
+                pyramid=(2, 2, 1), full_iterations=2)
 
 
 GOLDEN_CFG2 = os.path.join(HERE, "tests", "golden", "config2_oracle_fullview.json")
@@ -164,7 +164,7 @@ def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, 
                         seed_sets=2):
     """Accuracy reference for the street batch: the oracle's ComputePatchMatch
     at `iterations` and the GPU engine with gpu_schedule = (iterations per
-    coarser level, full iterations, pyramid levels) on the same crops (side x side from the
+    coarser level, coarsest first; full iterations) on the same crops (side x side from the
     centre of the first `jobs` reference views, K shifted, all of the job's
     neighbours), for `seed_sets` sets of seeds, one oracle run per host
     thread."""
@@ -219,7 +219,7 @@ def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, 
     cpu_s = time.perf_counter() - t0
     def gpu_est(views, s):
         e = setup(DepthmapEstimator(), views, s)
-        e.set_pyramid(gpu_schedule[0], levels=gpu_schedule[2])
+        e.set_pyramid(gpu_schedule[0])
         e.set_patchmatch_iterations(gpu_schedule[1])
         return e
 
@@ -238,15 +238,15 @@ def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, 
             "oracle_s": round(cpu_s, 1)}
 
 
-# street batch: iterations per coarser level, full iterations, pyramid levels
-BATCH_SCHEDULE = (2, 2, 4)
+# street batch: iterations per coarser level (1/8, 1/4, 1/2), full iterations
+BATCH_SCHEDULE = ((2, 2, 1), 2)
 
 
 def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
     """Batched street workload: n_views reference views of sizes drawn from
     {640x480, 1024x768, 1600x1200, 2048x1536}, 4-12 neighbours each
-    (rendered on the GPU), a 4-level pyramid with 2 iterations per level
-    (full resolution included) (the reference's count is 3; accuracy checked against the
+    (rendered on the GPU), a 4-level pyramid with 2, 2, 1 and 2 iterations
+    (coarsest first; the reference's count is 3; accuracy checked against the
     oracle's 3 iterations on crops) in one captured hipGraph, tile
     autotuned by speed and patch size {5, 7} gated on accuracy (the fastest
     within 0.2 points of the most accurate), then the cleaner with 2
@@ -265,7 +265,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         b.add_view(v.K, v.R, v.t, v.image, v.mask)
     for ref, nb in sc.jobs:
         b.add_job(ref, nb, sc.min_depth, sc.max_depth)
-    b.set_pyramid(BATCH_SCHEDULE[0], levels=BATCH_SCHEDULE[2])
+    b.set_pyramid(BATCH_SCHEDULE[0])
     b.set_patchmatch_iterations(BATCH_SCHEDULE[1])
     b.set_min_patch_sd(1.0)
     b.set_seed(seed)
@@ -300,7 +300,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
     plain3 = {"iterations": 3, "mpix_s": round(npx / plain_s / 1e6, 3),
               "accuracy_within_1pct": round(accuracy(b), 4)}
     log(f"street batch, plain 3 iterations: {plain3}")
-    b.set_pyramid(BATCH_SCHEDULE[0], levels=BATCH_SCHEDULE[2])
+    b.set_pyramid(BATCH_SCHEDULE[0])
     b.set_patchmatch_iterations(BATCH_SCHEDULE[1])
     b.run()
     b.clean(0.01, 2)
@@ -343,7 +343,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
             V = sc.views[v]
             e.add_view(V.K, V.R, V.t, V.image, V.mask)
         e.set_depth_range(sc.min_depth, sc.max_depth, 50)
-        e.set_pyramid(BATCH_SCHEDULE[0], levels=BATCH_SCHEDULE[2])
+        e.set_pyramid(BATCH_SCHEDULE[0])
         e.set_patchmatch_iterations(BATCH_SCHEDULE[1])
         e.set_patch_size(patch)
         e.set_min_patch_sd(1.0)
@@ -373,8 +373,8 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
                                  for r, (ref, _) in zip(cres, sc.jobs)]))
     return {
         "views": n_views, "mpix": round(npx / 1e6, 2),
-        "pyramid_levels": BATCH_SCHEDULE[2], "coarse_level_iterations": BATCH_SCHEDULE[0],
-        "iterations": BATCH_SCHEDULE[1],
+        "pyramid_levels": len(BATCH_SCHEDULE[0]) + 1,
+        "coarse_level_iterations": list(BATCH_SCHEDULE[0]), "iterations": BATCH_SCHEDULE[1],
         "plain_reference_iterations": plain3,
         "patch": patch, "tile": TILE_NAMES[tile], "autotune_s": round(tune_s, 2),
         "autotune": b.tune_report,
@@ -734,7 +734,7 @@ def main():
         """half: the pyramid (coarser levels + full iterations) that reaches
         the reference's 5-iteration accuracy; else plain PatchMatch at the
         reference's 5 iterations."""
-        est.set_pyramid(w["half_iterations"] if half else 0, levels=w["pyramid_levels"])
+        est.set_pyramid(w["pyramid"] if half else 0)
         est.set_patchmatch_iterations(w["full_iterations"] if half else w["iterations"])
 
     def timed(steps, warmup):
@@ -845,8 +845,8 @@ def main():
     schedule(False)
     sample_mode = {"iterations": w["iterations"], **sample_run()}
     schedule(True)
-    sample_mode["pyramid"] = {"pyramid_levels": w["pyramid_levels"],
-                              "coarse_level_iterations": w["half_iterations"],
+    sample_mode["pyramid"] = {"pyramid_levels": len(w["pyramid"]) + 1,
+                              "coarse_level_iterations": list(w["pyramid"]),
                               "iterations": w["full_iterations"], **sample_run()}
     log(f"sample mode: {sample_mode}")
 
@@ -876,11 +876,12 @@ def main():
         "config": {"workload": "single-view plain PatchMatch 1600x1200 + 8 neighbours, "
                                "patch 7, depth [1, 40], FP32, at the accuracy and mean score of the "
                                "reference's 5 iterations: a 4-level pyramid (1/8, 1/4, 1/2, full "
-                               "resolution), 2 iterations per level, each level seeding the next; "
+                               "resolution) with 2, 2, 1 and 2 iterations, each level seeding the "
+                               "next; "
                                "value = device-resident views to device-resident maps",
                    "width": w["width"], "height": w["height"], "neighbours": w["neighbours"],
-                   "patch": w["patch"], "pyramid_levels": w["pyramid_levels"],
-                   "coarse_level_iterations": w["half_iterations"],
+                   "patch": w["patch"], "pyramid_levels": len(w["pyramid"]) + 1,
+                   "coarse_level_iterations": list(w["pyramid"]),
                    "iterations": w["full_iterations"],
                    "reference_iterations": w["iterations"]},
         "accuracy_within_1pct": round(acc, 4),
@@ -930,7 +931,7 @@ def main():
             acc, inner = [], []   # inner: at least 20 px from the crop's edges
             for ce, (x0, y0, cw, ch, _) in zip(crop_est, crops):
                 gt = scene.ref.depth[y0:y0 + ch, x0:x0 + cw]
-                ce.set_pyramid(w["half_iterations"], levels=w["pyramid_levels"])
+                ce.set_pyramid(w["pyramid"])
                 ce.set_patchmatch_iterations(w["full_iterations"])
                 gd = ce.compute_patch_match()[0]
                 acc.append(scenes.depth_accuracy(gd, gt, w["min_depth"], w["max_depth"]))

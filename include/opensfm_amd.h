@@ -139,6 +139,12 @@ int pm_estimator_set_pyramid(pm_estimator* est, int half_iterations);
  * level is itself seeded by a quarter level, and so on; every level coarser
  * than the full one runs half_iterations iterations. */
 int pm_estimator_set_pyramid_levels(pm_estimator* est, int levels);
+/* The pyramid with its own iteration count per coarser level: n coarser
+ * levels (1 to 7; levels = n + 1), iterations[0] for the coarsest (views
+ * box-averaged by 2^n) up to iterations[n - 1] for the half level, each at
+ * least 1; set_patchmatch_iterations still counts the full level's.
+ * set_pyramid and set_pyramid_levels replace the schedule by a uniform one. */
+int pm_estimator_set_pyramid_schedule(pm_estimator* est, const int32_t* iterations, int n);
 /* Lanes per pixel of the red/black PatchMatch launches (no counterpart in
  * the reference, whose CPU loop scores the views of a candidate one after
  * another, depthmap.cc:408-419): 0 (default) scores each pixel on 8 lanes,
@@ -530,6 +536,8 @@ int pm_batch_set_exploration(pm_batch* b, int n);
 int pm_batch_set_pyramid(pm_batch* b, int half_iterations);
 /* pm_estimator_set_pyramid_levels for every job (default 2). */
 int pm_batch_set_pyramid_levels(pm_batch* b, int levels);
+/* pm_estimator_set_pyramid_schedule for every job. */
+int pm_batch_set_pyramid_schedule(pm_batch* b, const int32_t* iterations, int n);
 /* pm_estimator_set_consistency for every job (default 1). */
 int pm_batch_set_consistency(pm_batch* b, int enable);
 int pm_batch_set_post_process(pm_batch* b, int enable);

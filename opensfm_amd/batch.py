@@ -89,12 +89,18 @@ class DepthmapBatch:
     def set_seed(self, seed: int) -> None:
         _capi.check(self._lib.pm_batch_set_seed(self._h, int(seed) & (2 ** 64 - 1)))
 
-    def set_pyramid(self, half_iterations: int, levels: int = 2) -> None:
+    def set_pyramid(self, half_iterations, levels: int = 2) -> None:
         """Plain PatchMatch of every job seeded by half_iterations iterations
-        at half resolution (0: off), with levels - 1 coarser levels, as
+        at half resolution (0: off), with levels - 1 coarser levels, or a
+        sequence of iterations per coarser level (coarsest first), as
         DepthmapEstimator.set_pyramid."""
-        _capi.check(self._lib.pm_batch_set_pyramid_levels(self._h, int(levels)))
-        _capi.check(self._lib.pm_batch_set_pyramid(self._h, int(half_iterations)))
+        if isinstance(half_iterations, (int, np.integer)):
+            _capi.check(self._lib.pm_batch_set_pyramid_levels(self._h, int(levels)))
+            _capi.check(self._lib.pm_batch_set_pyramid(self._h, int(half_iterations)))
+            return
+        it = np.ascontiguousarray(half_iterations, np.int32).ravel()
+        _capi.check(self._lib.pm_batch_set_pyramid_schedule(
+            self._h, it.ctypes.data_as(C.POINTER(C.c_int32)), int(it.size)))
 
     def set_exploration(self, n: int) -> None:
         _capi.check(self._lib.pm_batch_set_exploration(self._h, int(n)))

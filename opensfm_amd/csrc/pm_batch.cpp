@@ -80,6 +80,7 @@ struct pm_batch {
   // on, every coarser level running pyramid_iters iterations
   int pyramid_iters = 0;
   int pyramid_levels = 2;
+  std::vector<int> level_iters;  // explicit schedule, coarsest first
   std::unique_ptr<pm_batch> half;
   DevBuf half_map;  // the half level's state as initialisation maps
   DevBuf jobtab_pyr;  // the job table with those maps (plain mode)
@@ -131,6 +132,13 @@ struct pm_batch {
 
 namespace {
 
+// Iterations of the coarser levels, coarsest first (empty: no pyramid).
+std::vector<int> pyramid_schedule(const pm_batch* b) {
+  if (!b->level_iters.empty()) return b->level_iters;
+  if (b->pyramid_iters <= 0) return {};
+  return std::vector<int>(b->pyramid_levels - 1, b->pyramid_iters);
+}
+
 int validate(pm_batch* b) {
   if (b->jobs.empty()) return fail(PM_ERR_INVALID, "batch has no jobs");
   if (!pm::precision_supported(b->precision, b->patch_size))
@@ -138,7 +146,7 @@ int validate(pm_batch* b) {
   const int hpz = (b->patch_size - 1) / 2;
   // with the pyramid, the coarsest level's views (box-averaged by
   // 2^(levels - 1)): references at least one patch, the others 2 x 2
-  const int sh = b->pyramid_iters > 0 ? b->pyramid_levels - 1 : 0;
+  const int sh = (int)pyramid_schedule(b).size();
   for (size_t k = 0; k < b->jobs.size(); ++k) {
     const PoolView& r = *b->views[b->jobs[k].ref];
     if ((r.width >> sh) < 2 * hpz + 1 || (r.height >> sh) < 2 * hpz + 1)
@@ -213,9 +221,12 @@ void ensure_half(pm_batch* b, int count) {
   }
   pm_batch* h = b->half.get();
   h->patch_size = b->patch_size;
-  h->iterations = b->pyramid_iters;
-  h->pyramid_iters = b->pyramid_levels > 2 ? b->pyramid_iters : 0;
-  h->pyramid_levels = b->pyramid_levels - 1;
+  std::vector<int> sched = pyramid_schedule(b);
+  h->iterations = sched.back();
+  sched.pop_back();
+  h->level_iters = sched;  // empty: the coarsest level
+  h->pyramid_iters = 0;
+  h->pyramid_levels = (int)sched.size() + 1;
   h->min_patch_variance = b->min_patch_variance;
   h->seed = b->seed + pm::kHalfLevelSeed;
   h->n_far = b->n_far;
@@ -378,7 +389,7 @@ void prepare(pm_batch* b, int count) {
   b->jobtab.ensure(jt.size() * sizeof(pm::PmJob));
   HIP_CHECK(hipMemcpy(b->jobtab.p, jt.data(), jt.size() * sizeof(pm::PmJob),
                       hipMemcpyHostToDevice));
-  if (b->pyramid_iters > 0) {
+  if (!pyramid_schedule(b).empty()) {
     // jobs without a caller's map start from the half level's, sampled
     // bilinearly with no spread (estimator: run_half)
     ensure_half(b, count);
@@ -411,7 +422,7 @@ void prepare(pm_batch* b, int count) {
 void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   const bool sample = mode == PM_MODE_PATCHMATCH_SAMPLE;
   const int nj = b->prepared_jobs;
-  const bool pyramid = b->pyramid_iters > 0;
+  const bool pyramid = !pyramid_schedule(b).empty();
   const pm::PmJob* jt = (pyramid ? b->jobtab_pyr : b->jobtab).as<pm::PmJob>();
   if (pyramid) {
     // the coarser level in the same mode (sample mode's levels sample too)
@@ -524,7 +535,7 @@ int run(pm_batch* b, int mode) {
   b->stats.launches = 2 * b->iterations;
   // the coarser levels ran in the same sequence: their launches of the same
   // kernels count too (the device time above already holds them)
-  for (const pm_batch* c = b; c->pyramid_iters > 0; c = c->half.get()) {
+  for (const pm_batch* c = b; !pyramid_schedule(c).empty(); c = c->half.get()) {
     const pm_batch* h = c->half.get();
     unsigned long long hc[2] = {0, 0};
     HIP_CHECK(hipMemcpy(hc, h->counter.p, sizeof(hc), hipMemcpyDeviceToHost));
@@ -551,8 +562,10 @@ float probe(pm_batch* b, int mode) {
   }
   const int iters = b->iterations;
   const int pyr = b->pyramid_iters;  // the tile is timed on plain runs
+  const std::vector<int> lv = b->level_iters;
   b->iterations = 2;
   b->pyramid_iters = 0;
+  b->level_iters.clear();
   std::vector<hipEvent_t> ev(8, nullptr);
   float ms = 0;
   try {
@@ -570,12 +583,14 @@ float probe(pm_batch* b, int mode) {
       if (e) hipEventDestroy(e);
     b->iterations = iters;
     b->pyramid_iters = pyr;
+    b->level_iters = lv;
     b->prepared_jobs = -1;
     throw;
   }
   for (auto& e : ev) hipEventDestroy(e);
   b->iterations = iters;
   b->pyramid_iters = pyr;
+  b->level_iters = lv;
   b->prepared_jobs = -1;
   return ms / (float)(px / 1e6);
 }
@@ -678,6 +693,20 @@ int pm_batch_set_pyramid_levels(pm_batch* b, int levels) {
   if (!b) return fail(PM_ERR_INVALID, "null batch");
   if (levels < 2 || levels > 8) return fail(PM_ERR_INVALID, "pyramid levels must be in [2, 8]");
   b->pyramid_levels = levels;
+  b->level_iters.clear();
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_pyramid_schedule(pm_batch* b, const int32_t* iterations, int n) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  if (n < 1 || n > 7) return fail(PM_ERR_INVALID, "a schedule has 1 to 7 coarser levels");
+  if (!iterations) return fail(PM_ERR_INVALID, "null schedule");
+  for (int k = 0; k < n; ++k)
+    if (iterations[k] < 1 || iterations[k] > 1000)
+      return fail(PM_ERR_INVALID, "level iterations must be in [1, 1000]");
+  b->level_iters.assign(iterations, iterations + n);
+  b->pyramid_levels = n + 1;
   b->invalidate();
   return PM_OK;
 }
@@ -687,6 +716,7 @@ int pm_batch_set_pyramid(pm_batch* b, int half_iterations) {
   if (half_iterations < 0 || half_iterations > 1000)
     return fail(PM_ERR_INVALID, "half-level iterations must be in [0, 1000]");
   b->pyramid_iters = half_iterations;
+  b->level_iters.clear();
   b->invalidate();
   return PM_OK;
 }

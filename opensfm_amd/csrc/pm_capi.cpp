@@ -86,6 +86,9 @@ struct pm_estimator {
   // so on: every coarser level runs pyramid_iters iterations.
   int pyramid_iters = 0;
   int pyramid_levels = 2;
+  // explicit iterations per coarser level, coarsest first (set_pyramid_
+  // schedule; empty: pyramid_iters on each of the pyramid_levels - 1 levels)
+  std::vector<int> level_iters;
   // lanes per pixel of the red/black launches (0: chosen by size)
   int lanes_per_pixel = 0;
   std::unique_ptr<pm_estimator> half;
@@ -136,6 +139,13 @@ void homography_baked(const double* K1inv, const double* Q2, const double* a2,
   for (int k = 0; k < 9; ++k) H[k] = (float)T[k];
 }
 
+// Iterations of the coarser levels, coarsest first (empty: no pyramid).
+std::vector<int> pyramid_schedule(const pm_estimator* e) {
+  if (!e->level_iters.empty()) return e->level_iters;
+  if (e->pyramid_iters <= 0) return {};
+  return std::vector<int>(e->pyramid_levels - 1, e->pyramid_iters);
+}
+
 // with_pyramid: the compute runs the pyramid when one is set
 int validate_for_compute(pm_estimator* e, bool with_pyramid = false) {
   if (e->views.size() < 2)
@@ -150,15 +160,15 @@ int validate_for_compute(pm_estimator* e, bool with_pyramid = false) {
   const int hpz = (e->patch_size - 1) / 2;
   if (r.width < 2 * hpz + 1 || r.height < 2 * hpz + 1)
     return fail(PM_ERR_INVALID, "reference view smaller than one patch");
-  if (with_pyramid && e->pyramid_iters > 0 && !e->init_map) {
+  const int sh = (int)pyramid_schedule(e).size();
+  if (with_pyramid && sh > 0 && !e->init_map) {
     // the coarsest level: every view box-averaged by 2^(levels - 1), the
     // reference at least one patch
-    const int sh = e->pyramid_levels - 1;
     bool ok = (r.width >> sh) >= 2 * hpz + 1 && (r.height >> sh) >= 2 * hpz + 1;
     for (const auto& V : e->views) ok = ok && (V->width >> sh) >= 2 && (V->height >> sh) >= 2;
     if (!ok)
       return fail(PM_ERR_INVALID, "views too small for a pyramid of " +
-                                      std::to_string(e->pyramid_levels) +
+                                      std::to_string(sh + 1) +
                                       " levels (the coarsest reference must hold one patch)");
   }
   return PM_OK;
@@ -331,7 +341,8 @@ void ensure_half(pm_estimator* e) {
 
 int run(pm_estimator* e, int mode);
 
-// Runs the half level (same settings, e->pyramid_iters iterations, its own
+// Runs the half level (same settings, the last entry of the schedule as its
+// iterations and the rest as its own pyramid, its own
 // seed) and turns its state into e->half_map: per half-level pixel the
 // normalised log depth, the normal's slopes and a confidence logit (+10 where
 // the depth is live and PostProcess's median test keeps it, else -10).
@@ -342,10 +353,13 @@ int run_half(pm_estimator* e, int mode) {
   h->min_depth = e->min_depth;
   h->max_depth = e->max_depth;
   h->num_depth_planes = e->num_depth_planes;
-  h->iterations = e->pyramid_iters;
+  std::vector<int> sched = pyramid_schedule(e);
+  h->iterations = sched.back();
+  sched.pop_back();
+  h->level_iters = sched;  // empty: the coarsest level
+  h->pyramid_iters = 0;
+  h->pyramid_levels = (int)sched.size() + 1;
   h->lanes_per_pixel = e->lanes_per_pixel;
-  h->pyramid_iters = e->pyramid_levels > 2 ? e->pyramid_iters : 0;
-  h->pyramid_levels = e->pyramid_levels - 1;
   h->min_patch_variance = e->min_patch_variance;
   h->seed = e->seed + pm::kHalfLevelSeed;
   h->n_far = e->n_far;
@@ -375,7 +389,8 @@ int run(pm_estimator* e, int mode) {
   DeviceGuard dg(e->device);
   e->have_result = false;
   // the half level seeds plain PatchMatch when no initialisation map is set
-  const bool pyramid = mode != PM_MODE_PLANE_SWEEP && e->pyramid_iters > 0 && !e->init_map;
+  const bool pyramid =
+      mode != PM_MODE_PLANE_SWEEP && !pyramid_schedule(e).empty() && !e->init_map;
   if (pyramid) {
     rc = run_half(e, mode);
     if (rc) return rc;
@@ -893,8 +908,22 @@ int pm_estimator_set_precision(pm_estimator* e, int precision) {
 
 int pm_estimator_set_pyramid(pm_estimator* e, int half_iterations) {
   if (!e) return fail(PM_ERR_INVALID, "null estimator");
-  if (half_iterations < 0) return fail(PM_ERR_INVALID, "half-level iterations must be >= 0");
+  if (half_iterations < 0 || half_iterations > 1000)
+    return fail(PM_ERR_INVALID, "half-level iterations must be in [0, 1000]");
   e->pyramid_iters = half_iterations;
+  e->level_iters.clear();
+  return PM_OK;
+}
+
+int pm_estimator_set_pyramid_schedule(pm_estimator* e, const int32_t* iterations, int n) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  if (n < 1 || n > 7) return fail(PM_ERR_INVALID, "a schedule has 1 to 7 coarser levels");
+  if (!iterations) return fail(PM_ERR_INVALID, "null schedule");
+  for (int k = 0; k < n; ++k)
+    if (iterations[k] < 1 || iterations[k] > 1000)
+      return fail(PM_ERR_INVALID, "level iterations must be in [1, 1000]");
+  e->level_iters.assign(iterations, iterations + n);
+  e->pyramid_levels = n + 1;
   return PM_OK;
 }
 
@@ -911,6 +940,7 @@ int pm_estimator_set_pyramid_levels(pm_estimator* e, int levels) {
   if (!e) return fail(PM_ERR_INVALID, "null estimator");
   if (levels < 2 || levels > 8) return fail(PM_ERR_INVALID, "pyramid levels must be in [2, 8]");
   e->pyramid_levels = levels;
+  e->level_iters.clear();
   return PM_OK;
 }
 

@@ -87,13 +87,19 @@ class DepthmapEstimator:
     def set_seed(self, seed: int) -> None:
         _capi.check(self._lib.pm_estimator_set_seed(self._h, int(seed) & (2 ** 64 - 1)))
 
-    def set_pyramid(self, half_iterations: int, levels: int = 2) -> None:
+    def set_pyramid(self, half_iterations, levels: int = 2) -> None:
         """Plain PatchMatch seeded by half_iterations iterations at half
         resolution (0: off; not in the reference's legacy estimator); with
         levels > 2 each coarser level is seeded by the next, down to
-        1 / 2^(levels - 1) of the full size."""
-        _capi.check(self._lib.pm_estimator_set_pyramid_levels(self._h, int(levels)))
-        _capi.check(self._lib.pm_estimator_set_pyramid(self._h, int(half_iterations)))
+        1 / 2^(levels - 1) of the full size.  A sequence gives each coarser
+        level its own iterations, coarsest first (levels = its length + 1)."""
+        if isinstance(half_iterations, (int, np.integer)):
+            _capi.check(self._lib.pm_estimator_set_pyramid_levels(self._h, int(levels)))
+            _capi.check(self._lib.pm_estimator_set_pyramid(self._h, int(half_iterations)))
+            return
+        it = np.ascontiguousarray(half_iterations, np.int32).ravel()
+        _capi.check(self._lib.pm_estimator_set_pyramid_schedule(
+            self._h, it.ctypes.data_as(C.POINTER(C.c_int32)), int(it.size)))
 
     def set_lanes_per_pixel(self, lanes: int) -> None:
         """Lanes per pixel of the red/black launches: 0 (default) 8 for

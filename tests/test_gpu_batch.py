@@ -69,29 +69,33 @@ def test_batch_equals_single_view_runs(street, tile):
         assert (got[0] > 0).mean() > 0.3
 
 
-@pytest.mark.parametrize("levels", [2, 3])
+@pytest.mark.parametrize("levels", [2, 3, (1, 2)])
 def test_batch_pyramid_equals_single_view_runs(street, levels):
     """With the pyramid (2 iterations on each coarser level -- 1/2, and 1/4
     with 3 levels -- + 2 full iterations), job k still equals the estimator
     with seed + k bit for bit, in a captured graph (replayed twice) and
     eagerly; the statistics count every level's launches; in sample mode
     too (every level samples)."""
-    print(f"batch vs single-view estimator with a {levels}-level pyramid", flush=True)
+    print(f"batch vs single-view estimator with the pyramid {levels}", flush=True)
+    if isinstance(levels, tuple):  # a per-level schedule
+        pyr, lv, launches = levels, 2, 2 * (2 + sum(levels))
+    else:
+        pyr, lv, launches = 2, levels, 2 * 2 * levels
     for graph in (True, False):
         b = _batch(street, graph=graph)
-        b.set_pyramid(2, levels=levels)
+        b.set_pyramid(pyr, levels=lv)
         for rep in range(3 if graph else 1):
             b.run()
-            assert b.stats()["launches"] == 2 * 2 * levels
+            assert b.stats()["launches"] == launches
             for k in range(len(street.jobs)):
                 got = b.result(k)
-                want = _single(street, k, pyramid=2, levels=levels)
+                want = _single(street, k, pyramid=pyr, levels=lv)
                 assert _equal(got, want), f"job {k} graph {graph} run {rep}"
                 assert (got[0] > 0).mean() > 0.3
     b.run(sample=True)
-    assert b.stats()["launches"] == 2 * 2 * levels
+    assert b.stats()["launches"] == launches
     for k in range(len(street.jobs)):
-        want = _single(street, k, sample=True, pyramid=2, levels=levels)
+        want = _single(street, k, sample=True, pyramid=pyr, levels=lv)
         assert _equal(b.result(k), want), f"sample job {k}"
 
 

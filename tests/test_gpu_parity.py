@@ -197,17 +197,18 @@ def test_patchmatch_full_view_accuracy_vs_oracle():
     assert score >= want["mean_score"]
 
 
-@pytest.mark.parametrize("coarse_iters,levels", [(5, 2), (2, 4)])
-def test_pyramid_full_view_accuracy_vs_oracle(coarse_iters, levels):
-    """The pyramid (set_pyramid: coarse_iters iterations on the views
-    box-averaged by 2 -- and by 4, 8 with levels 4, each level seeding the
-    next) and 2 full-resolution iterations on the 1600x1200 + 8-neighbour
-    scene: accuracy and mean score over seeds 1-5 at least the oracle's
-    5-iteration full-view figures."""
-    print(f"PatchMatch with a {levels}-level pyramid vs oracle, 1600x1200 full view", flush=True)
+@pytest.mark.parametrize("pyramid", [(5, 2), (2, 4), ((2, 2, 1),)],
+                         ids=["2 levels, 5", "4 levels, 2 each", "4 levels, 2-2-1"])
+def test_pyramid_full_view_accuracy_vs_oracle(pyramid):
+    """The pyramid (set_pyramid: iterations on the views box-averaged by 2
+    -- and by 4, 8 with 4 levels, each level seeding the next; uniform or
+    per level) and 2 full-resolution iterations on the 1600x1200 +
+    8-neighbour scene: accuracy and mean score over seeds 1-5 at least the
+    oracle's 5-iteration full-view figures."""
+    print(f"PatchMatch with the pyramid {pyramid} vs oracle, 1600x1200 full view", flush=True)
     want = _golden("config2_oracle_fullview.json")
     s = scenes.scene_config2()
-    acc, score = _gpu_runs(s, (1, 2, 3, 4, 5), 2, planes=50, pyramid=(coarse_iters, levels))
+    acc, score = _gpu_runs(s, (1, 2, 3, 4, 5), 2, planes=50, pyramid=pyramid)
     print(f"oracle acc {want['mean_accuracy_within_1pct']:.5f} score {want['mean_score']:.5f} | "
           f"gpu acc {acc:.5f} score {score:.5f}", flush=True)
     assert acc >= want["mean_accuracy_within_1pct"]
@@ -247,6 +248,19 @@ def test_pyramid_runs_and_is_deterministic(cfg1_scene):
         assert np.array_equal(x, y)
     assert (a[0] > 0).mean() > 0.8
     assert scenes.depth_accuracy(a[0], cfg1_scene.ref.depth, 2.0, 20.0) > 0.88
+    # a per-level schedule: uniform equals set_pyramid's, and (1, 2) runs
+    # 1 iteration at 80x60 and 2 at 160x120
+    est.set_pyramid((2, 2))
+    for x, y in zip(est.compute_patch_match(), a):
+        assert np.array_equal(x, y)
+    est.set_pyramid((1, 2))
+    b1 = est.compute_patch_match()
+    assert est.stats()["launches"] == 2 * 3 + 2 * (1 + 2)
+    assert not np.array_equal(b1[0], a[0]) and (b1[0] > 0).mean() > 0.8
+    with pytest.raises(ValueError):
+        est.set_pyramid((2, 0))
+    with pytest.raises(ValueError):
+        est.set_pyramid(tuple([1] * 8))
     est.set_pyramid(0)
     for x, y in zip(est.compute_patch_match(), plain):
         assert np.array_equal(x, y)
