@@ -148,7 +148,7 @@ int pm_estimator_set_pyramid_schedule(pm_estimator* est, const int32_t* iteratio
 /* Lanes per pixel of the red/black PatchMatch launches (no counterpart in
  * the reference, whose CPU loop scores the views of a candidate one after
  * another, depthmap.cc:408-419): 0 (default) scores each pixel on 8 lanes,
- * one share of the views each, when a launch has fewer than 131072 pixels of
+ * one share of the views each, when a launch has fewer than 32768 pixels of
  * one colour (the coarse levels of a pyramid) and FP32 samples are used, else
  * on one lane; 1 always one lane; 8 always 8 lanes (FP32 samples).  The maps
  * are bit-identical for every setting; sample mode always uses one lane. */

@@ -31,10 +31,14 @@ constexpr int kConsistFinalSupport = 10;
 // levels.
 constexpr uint64_t kHalfLevelSeed = 0x9E3779B97F4A7C15ull;
 // Single-view red/black launches with fewer pixels of one colour than this
-// (about the lanes the chip holds at 2 waves per SIMD) score each pixel on
-// kLanesPerPixelSmall lanes, one share of the views each (k_pm_half_group).
+// score each pixel on kLanesPerPixelSmall lanes, one share of the views each
+// (k_pm_half_group).  Measured with 8 neighbours, patch 7: 15 k pixels
+// (200 x 150) 0.15 ms per launch against 0.42 on one lane; 60 k pixels
+// (400 x 300) 0.57 ms against 0.40, since every lane of a group repeats the
+// candidate loop and the reference patch once the chip is full (2 or 4 lanes
+// per pixel were no faster there).
 constexpr int kLanesPerPixelSmall = 8;
-constexpr size_t kGroupBelowPixels = 131072;
+constexpr size_t kGroupBelowPixels = 32768;
 
 // One view as the kernels see it.  For neighbour view v (v >= 1) the plane-
 // induced homography of plane v_pi (in the reference camera frame) is
