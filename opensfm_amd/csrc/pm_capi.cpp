@@ -61,6 +61,9 @@ struct pm_estimator {
   DevBuf mask0;
   // per-compute device buffers
   DevBuf viewtab, wlut, state, score, nghbr, counter, queues;
+  // host copy of the view table: it feeds an asynchronous upload, so it lives
+  // until the compute's final synchronisation
+  std::vector<pm::PmView> tab_host;
   DevBuf cons_count, cons_idx, cons_med;  // median-consistent re-selection list
   // and its window-phase work items (pm::ConsistItems)
   DevBuf item_count, item_entry, item_win, item_score, item_nghbr, item_depth, item_first,
@@ -178,7 +181,8 @@ int validate_for_compute(pm_estimator* e, bool with_pyramid = false) {
 pm::PmArgs prepare(pm_estimator* e) {
   const int n = (int)e->views.size();
   const HostView& r = *e->views[0];
-  std::vector<pm::PmView> tab(n);
+  std::vector<pm::PmView>& tab = e->tab_host;
+  tab.assign(n, pm::PmView{});
   for (int v = 0; v < n; ++v) {
     HostView& V = *e->views[v];
     if (e->precision != PM_PRECISION_FP32 && V.quad16_kind != e->precision) {
@@ -213,7 +217,6 @@ pm::PmArgs prepare(pm_estimator* e) {
   HIP_CHECK(hipMemsetAsync(e->score.p, 0, np * sizeof(float), e->stream));
   HIP_CHECK(hipMemsetAsync(e->nghbr.p, 0, np * sizeof(int), e->stream));
   HIP_CHECK(hipMemsetAsync(e->counter.p, 0, 2 * sizeof(unsigned long long), e->stream));
-  HIP_CHECK(hipStreamSynchronize(e->stream));  // `tab` feeds an async copy
 
   pm::PmArgs a;
   memset(&a, 0, sizeof(a));
@@ -410,7 +413,7 @@ int run(pm_estimator* e, int mode) {
   const HostView& r = *e->views[0];
   const int n = (int)e->views.size();
   float dom_ms = 0.0f;
-  int launches = 0;
+  int launches = 0, nl = 0;  // red/black launches timed by half_ev
   HIP_CHECK(hipEventRecord(e->ev[0], e->stream));
   if (mode == PM_MODE_PLANE_SWEEP) {
     const int D = e->num_depth_planes;
@@ -459,7 +462,7 @@ int run(pm_estimator* e, int mode) {
     ah.counter = a.counter + 1;
     // an event before and after each red/black launch (the consistency
     // passes run between them); no host wait inside the loop
-    const int nl = 2 * e->iterations;
+    nl = 2 * e->iterations;
     while ((int)e->half_ev.size() < 2 * nl) {
       hipEvent_t ev;
       HIP_CHECK(hipEventCreate(&ev));
@@ -483,12 +486,6 @@ int run(pm_estimator* e, int mode) {
       if (e->consistency && (it >= 1 || it + 1 == e->iterations))
         HIP_CHECK(consistency_pass(e, ah, sample, (unsigned)it, it + 1 == e->iterations));
     }
-    if (nl > 0) HIP_CHECK(hipEventSynchronize(e->half_ev[2 * nl - 1]));
-    for (int l = 0; l < nl; ++l) {
-      float ms = 0;
-      HIP_CHECK(hipEventElapsedTime(&ms, e->half_ev[2 * l], e->half_ev[2 * l + 1]));
-      dom_ms += ms;
-    }
   }
   const size_t np = (size_t)r.width * r.height;
   e->out_depth.ensure(np * sizeof(float));
@@ -500,9 +497,14 @@ int run(pm_estimator* e, int mode) {
                             e->out_score.as<float>(), e->out_nghbr.as<int>(),
                             e->stream));
   HIP_CHECK(hipEventRecord(e->ev[1], e->stream));
-  HIP_CHECK(hipEventSynchronize(e->ev[1]));
+  HIP_CHECK(hipEventSynchronize(e->ev[1]));  // the one host wait of a compute
   float total_ms = 0;
   HIP_CHECK(hipEventElapsedTime(&total_ms, e->ev[0], e->ev[1]));
+  for (int l = 0; l < nl; ++l) {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e->half_ev[2 * l], e->half_ev[2 * l + 1]));
+    dom_ms += ms;
+  }
   unsigned long long cnt[2] = {0, 0};
   HIP_CHECK(hipMemcpy(cnt, e->counter.p, sizeof(cnt), hipMemcpyDeviceToHost));
   e->stats.scorings = (int64_t)(cnt[0] + cnt[1]);
