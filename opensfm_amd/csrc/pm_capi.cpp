@@ -96,12 +96,20 @@ struct pm_estimator {
   int lanes_per_pixel = 0;
   std::unique_ptr<pm_estimator> half;
   DevBuf half_map;  // the half level's state as an initialisation map
+  bool own_stream = true;  // a pyramid level runs on its parent's stream
+  // a compute enqueued and not yet waited for (run with wait = false): what
+  // its statistics need once the stream has drained
+  struct {
+    int mode = 0, nl = 0, launches = 0;
+    bool pyramid = false;
+    float sweep_ms = 0.0f;
+  } pending;
 
   ~pm_estimator() {
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
     for (auto& e : half_ev) hipEventDestroy(e);
-    if (stream) hipStreamDestroy(stream);
+    if (stream && own_stream) hipStreamDestroy(stream);
   }
 };
 
@@ -304,7 +312,8 @@ void ensure_half(pm_estimator* e) {
   if (e->half && e->half->views.size() == e->views.size()) return;
   std::unique_ptr<pm_estimator> h(new pm_estimator());
   h->device = e->device;
-  HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  h->stream = e->stream;  // in stream order with this level: no host wait between levels
+  h->own_stream = false;
   for (auto& ev : h->ev) HIP_CHECK(hipEventCreate(&ev));
   for (size_t k = 0; k < e->views.size(); ++k) {
     const HostView& V = *e->views[k];
@@ -342,7 +351,7 @@ void ensure_half(pm_estimator* e) {
   e->half = std::move(h);
 }
 
-int run(pm_estimator* e, int mode);
+int run(pm_estimator* e, int mode, bool wait = true);
 
 // Runs the half level (same settings, the last entry of the schedule as its
 // iterations and the rest as its own pyramid, its own
@@ -371,7 +380,7 @@ int run_half(pm_estimator* e, int mode) {
   h->precision = e->precision;
   h->post_process = e->post_process;
   h->consistency = e->consistency;
-  int rc = run(h, mode);
+  int rc = run(h, mode, false);  // enqueued on this level's stream
   if (rc) return rc;
   const HostView& r = *h->views[0];
   e->half_map.ensure((size_t)r.width * r.height * 4 * sizeof(float));
@@ -381,7 +390,44 @@ int run_half(pm_estimator* e, int mode) {
   return PM_OK;
 }
 
-int run(pm_estimator* e, int mode) {
+// Statistics of a compute whose stream has drained (its levels' too).
+void collect_stats(pm_estimator* e) {
+  const auto& pd = e->pending;
+  if (pd.pyramid) collect_stats(e->half.get());
+  const HostView& r = *e->views[0];
+  float total_ms = 0, dom_ms = pd.sweep_ms;
+  HIP_CHECK(hipEventElapsedTime(&total_ms, e->ev[0], e->ev[1]));
+  for (int l = 0; l < pd.nl; ++l) {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e->half_ev[2 * l], e->half_ev[2 * l + 1]));
+    dom_ms += ms;
+  }
+  unsigned long long cnt[2] = {0, 0};
+  HIP_CHECK(hipMemcpy(cnt, e->counter.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  e->stats.scorings = (int64_t)(cnt[0] + cnt[1]);
+  e->stats.dominant_scorings =
+      (int64_t)(pd.mode == PM_MODE_PLANE_SWEEP ? cnt[0] : cnt[1]);
+  e->stats.patch_size = e->patch_size;
+  e->stats.width = r.width;
+  e->stats.height = r.height;
+  e->stats.launches = pd.launches;
+  e->stats.dominant_ms = dom_ms;
+  e->stats.total_ms = total_ms;
+  if (pd.pyramid) {
+    // the whole compute: the coarser levels' launches of the same kernels
+    // and their device time count too
+    const pm_stats& hs = e->half->stats;
+    e->stats.scorings += hs.scorings;
+    e->stats.dominant_scorings += hs.dominant_scorings;
+    e->stats.launches += hs.launches;
+    e->stats.dominant_ms += hs.dominant_ms;
+    e->stats.total_ms += hs.total_ms;
+  }
+}
+
+// One compute, and with a pyramid its coarser levels first, all on e's
+// stream; wait = false leaves it enqueued (collect_stats after a wait).
+int run(pm_estimator* e, int mode, bool wait) {
   int rc = validate_for_compute(e, mode != PM_MODE_PLANE_SWEEP);
   if (rc) return rc;
   if (mode != PM_MODE_PLANE_SWEEP && mode != PM_MODE_PATCHMATCH &&
@@ -497,35 +543,14 @@ int run(pm_estimator* e, int mode) {
                             e->out_score.as<float>(), e->out_nghbr.as<int>(),
                             e->stream));
   HIP_CHECK(hipEventRecord(e->ev[1], e->stream));
+  e->pending.mode = mode;
+  e->pending.nl = nl;
+  e->pending.launches = launches;
+  e->pending.pyramid = pyramid;
+  e->pending.sweep_ms = dom_ms;
+  if (!wait) return PM_OK;
   HIP_CHECK(hipEventSynchronize(e->ev[1]));  // the one host wait of a compute
-  float total_ms = 0;
-  HIP_CHECK(hipEventElapsedTime(&total_ms, e->ev[0], e->ev[1]));
-  for (int l = 0; l < nl; ++l) {
-    float ms = 0;
-    HIP_CHECK(hipEventElapsedTime(&ms, e->half_ev[2 * l], e->half_ev[2 * l + 1]));
-    dom_ms += ms;
-  }
-  unsigned long long cnt[2] = {0, 0};
-  HIP_CHECK(hipMemcpy(cnt, e->counter.p, sizeof(cnt), hipMemcpyDeviceToHost));
-  e->stats.scorings = (int64_t)(cnt[0] + cnt[1]);
-  e->stats.dominant_scorings =
-      (int64_t)(mode == PM_MODE_PLANE_SWEEP ? cnt[0] : cnt[1]);
-  e->stats.patch_size = e->patch_size;
-  e->stats.width = r.width;
-  e->stats.height = r.height;
-  e->stats.launches = launches;
-  e->stats.dominant_ms = dom_ms;
-  e->stats.total_ms = total_ms;
-  if (pyramid) {
-    // the whole compute: the half level's launches of the same kernels and
-    // its device time count too
-    const pm_stats& hs = e->half->stats;
-    e->stats.scorings += hs.scorings;
-    e->stats.dominant_scorings += hs.dominant_scorings;
-    e->stats.launches += hs.launches;
-    e->stats.dominant_ms += hs.dominant_ms;
-    e->stats.total_ms += hs.total_ms;
-  }
+  collect_stats(e);
   e->have_result = true;
   return PM_OK;
 }
