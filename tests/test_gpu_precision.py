@@ -108,6 +108,40 @@ def test_patchmatch_reduced_precision_accuracy(cfg1_scene, prec):
     assert res[prec][1] >= res["fp32"][1] - 0.005
 
 
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_reduced_precision_with_pyramid(wide_scene, prec):
+    """The pyramid at FP16/BF16 (every level builds its 16-bit quads; lane
+    groups are FP32-only, so every level runs one lane per pixel): the
+    batch equals the single-view estimator bit for bit, and the estimator's
+    accuracy and mean score stay within 0.005 of its FP32 pyramid run."""
+    print(f"pyramid at {prec}", flush=True)
+    gt = wide_scene.ref.depth
+    res = {}
+    for p in ("fp32", prec):
+        e = _add(DepthmapEstimator(), wide_scene, iters=2)
+        e.set_precision(p)
+        e.set_seed(4)
+        e.set_pyramid((2, 1))
+        d, _, s, _ = e.compute_patch_match()
+        res[p] = (scenes.depth_accuracy(d, gt, 1.0, 40.0), float(s[d > 0].mean()), (d, s))
+    print(f"fp32 acc {res['fp32'][0]:.4f} score {res['fp32'][1]:.4f} | "
+          f"{prec} acc {res[prec][0]:.4f} score {res[prec][1]:.4f}", flush=True)
+    assert res[prec][0] >= res["fp32"][0] - 0.005
+    assert res[prec][1] >= res["fp32"][1] - 0.005
+    b = DepthmapBatch()
+    for v in wide_scene.views:
+        b.add_view(v.K, v.R, v.t, v.image, v.mask)
+    b.add_job(0, list(range(1, 11)), wide_scene.min_depth, wide_scene.max_depth)
+    b.set_patchmatch_iterations(2)
+    b.set_min_patch_sd(1.0)
+    b.set_seed(4)
+    b.set_precision(prec)
+    b.set_pyramid((2, 1))
+    b.run()
+    d, _, s, _ = b.result(0)
+    assert np.array_equal(d, res[prec][2][0]) and np.array_equal(s, res[prec][2][1])
+
+
 def test_precision_needs_patch_5_or_7(cfg1_scene):
     e = _add(DepthmapEstimator(), cfg1_scene, patch=9)
     e.set_precision("fp16")
