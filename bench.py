@@ -380,6 +380,15 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         "autotune": b.tune_report,
         "mpix_s": round(npx / batch_s / 1e6, 3),
         "ms_per_run": round(batch_s * 1e3, 2), "device_ms": round(st["total_ms"], 2),
+        # the batched red/black kernels' algorithmic rate over the whole run's
+        # device time (every launch of the graph counted as red/black time,
+        # so a lower bound on the kernels' own rate)
+        "achieved_tflops_lower_bound": round(
+            st["dominant_scorings"] * (FLOP_PER_SAMPLE * patch ** 2 + FLOP_PER_SCORING)
+            / (st["total_ms"] / 1e3) / 1e12, 3),
+        "frac_lower_bound": round(
+            st["dominant_scorings"] * (FLOP_PER_SAMPLE * patch ** 2 + FLOP_PER_SCORING)
+            / (st["total_ms"] / 1e3) / 1e12 / FP32_PEAK_TFLOPS, 4),
         "per_view_loop_mpix_s": round(npx / per_view_s / 1e6, 3),
         "accuracy_within_1pct": round(float(np.mean(accs)), 4),
         "mean_score": round(float(np.mean(scores)), 4),
