@@ -597,11 +597,13 @@ def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
     """Learned initialiser configuration: 64 nadir views of 4000x3000 over
     fractal terrain (an 8 x 8 camera grid, rendered on the GPU), 10
     neighbours each, all in one DepthmapBatch.  Plain PatchMatch from random
-    planes at 5, 2 and 1 iterations, and through a 4-level pyramid with 2
-    iterations per level, against the learned initialiser (network
-    input from the batch's 1/4-resolution views and 32-plane cost volume,
-    the U-Net, then seeded PatchMatch) at 1 and 2 iterations, each timed once
-    from device-resident views to device-resident maps."""
+    planes at 5, 2 and 1 iterations, and through a 4-level pyramid (2, 2, 2
+    + 2 and 2, 2, 1 + 1 iterations), against the learned initialiser
+    (network input from the batch's 1/4-resolution views and 32-plane cost
+    volume, the U-Net, then seeded PatchMatch) at 1 and 2 iterations, and
+    with its map seeding the 1/4 level of a 3-level pyramid (1, 1 + 1
+    iterations), each timed once from device-resident views to
+    device-resident maps, the network inside the timed region."""
     import torch
     from opensfm_amd import DepthmapBatch, initializer as I, nn, scenes
     log(f"learned initialiser: rendering {rows * cols} nadir views of {width}x{height} on the GPU")
@@ -630,13 +632,17 @@ def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
                        f"batch", "mpix": round(npx / 1e6, 1), "params": init.net.n_params}
     # warm the network and the coarse views outside the timed region
     init.predict_batch(b, 0, width, height)
-    for mode, learned, iters in (("plain_5", False, 5), ("plain_2", False, 2),
-                                 ("plain_1", False, 1), ("plain_pyramid4_2", False, 2),
-                                 ("plain_pyramid4_1", False, 1),
-                                 ("learned_1", True, 1), ("learned_2", True, 2)):
-        # plain_pyramid4_*: levels at 1/8, 1/4 and 1/2 resolution with 2
-        # iterations each, every level seeding the next, then 2 or 1 at full
-        b.set_pyramid(2 if mode.startswith("plain_pyramid") else 0, levels=4)
+    # coarser levels' iterations, coarsest first (None: one level); the
+    # learned map seeds the coarsest level when there is a pyramid
+    schedules = {"plain_5": (False, 5, None), "plain_2": (False, 2, None),
+                 "plain_1": (False, 1, None), "plain_pyramid_2": (False, 2, (2, 2, 2)),
+                 "plain_pyramid_1": (False, 1, (2, 2, 1)), "learned_1": (True, 1, None),
+                 "learned_2": (True, 2, None), "learned_pyramid_1": (True, 1, (1, 1))}
+    out["schedules"] = {m: {"coarse_level_iterations": list(p) if p else None,
+                            "full_iterations": it, "learned_map": lrn}
+                        for m, (lrn, it, p) in schedules.items()}
+    for mode, (learned, iters, pyr) in schedules.items():
+        b.set_pyramid(pyr if pyr else 0)
         b.set_patchmatch_iterations(iters)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -656,16 +662,17 @@ def learned_bench(rows=8, cols=8, width=4000, height=3000, seed=5000):
                 b.set_init_map(k, None)
         log(f"learned initialiser {mode}: {out[mode]}")
     base = out["plain_5"]["mpix_s"]
-    for mode in ("learned_1", "learned_2", "plain_2", "plain_1", "plain_pyramid4_2",
-                 "plain_pyramid4_1"):
-        out[f"{mode}_over_plain_5"] = round(out[mode]["mpix_s"] / base, 2)
+    for mode in schedules:
+        if mode != "plain_5":
+            out[f"{mode}_over_plain_5"] = round(out[mode]["mpix_s"] / base, 2)
     # the fastest plain schedule reaching plain-5's accuracy, and the
     # learned runs at that accuracy
     target = out["plain_5"]["acc"]
-    plain_modes = ("plain_1", "plain_2", "plain_pyramid4_2", "plain_pyramid4_1", "plain_5")
-    eq_plain = max((m for m in plain_modes if out[m]["acc"] >= target),
+    eq_plain = max((m for m in schedules if m.startswith("plain") and out[m]["acc"] >= target),
                    key=lambda m: out[m]["mpix_s"])
-    eq_learned = [m for m in ("learned_1", "learned_2") if out[m]["acc"] >= target]
+    eq_learned = sorted((m for m in schedules
+                         if m.startswith("learned") and out[m]["acc"] >= target),
+                        key=lambda m: -out[m]["mpix_s"])
     out["equal_accuracy"] = {
         "plain_fastest_at_plain_5_accuracy": eq_plain,
         "plain_fastest_over_plain_5": round(out[eq_plain]["mpix_s"] / base, 2),

@@ -130,7 +130,9 @@ int pm_estimator_set_far_probes(pm_estimator* est, int n);
  * the median test keeps it -- seed this level's initialisation (bilinear in
  * depth and slopes, as an initialisation map would); the rest start from
  * random planes.  Then set_patchmatch_iterations iterations run here.
- * Default 0 (off); ignored when an initialisation map is set.  In sample
+ * Default 0 (off).  With an initialisation map set, the map seeds the
+ * coarsest level (its scale must be divisible by 2 per coarser level) and
+ * each level seeds the next as above.  In sample
  * mode every level runs in sample mode. */
 int pm_estimator_set_pyramid(pm_estimator* est, int half_iterations);
 /* Levels of that pyramid, the full resolution included (default 2; in
@@ -241,7 +243,8 @@ int pm_estimator_score_hypotheses_fast(pm_estimator* est, int n,
  * sample from the first iteration on).  The map is read by the
  * initialisation and the first red/black iteration, so it must stay valid
  * through every compute made while it is set; NULL restores random
- * initialisation. */
+ * initialisation.  With set_pyramid the map seeds the coarsest level, at
+ * scale / 2^(levels - 1) of that level's pixels. */
 int pm_estimator_initializer_input(pm_estimator* est, int scale, int nplanes,
                                    void* out, int out_w, int out_h);
 int pm_estimator_set_init_map(pm_estimator* est, const float* map, int w,

@@ -155,10 +155,16 @@ int validate(pm_batch* b) {
                                             "pyramid's coarsest level"
                                           : ": reference view smaller than one patch"));
   }
-  if (sh)
+  if (sh) {
     for (const auto& V : b->views)
       if ((V->width >> sh) < 2 || (V->height >> sh) < 2)
         return fail(PM_ERR_INVALID, "a view is too small for the pyramid's coarsest level");
+    for (size_t k = 0; k < b->jobs.size(); ++k)
+      if (b->jobs[k].init_map && b->jobs[k].init_scale % (1 << sh) != 0)
+        return fail(PM_ERR_INVALID, "job " + std::to_string(k) +
+                                        ": an initialisation map seeding the pyramid needs a "
+                                        "scale divisible by " + std::to_string(1 << sh));
+  }
   return PM_OK;
 }
 
@@ -221,6 +227,20 @@ void ensure_half(pm_batch* b, int count) {
   }
   pm_batch* h = b->half.get();
   h->patch_size = b->patch_size;
+  // a job's own initialisation map seeds the coarsest level (its cells cover
+  // half as many pixels of each coarser level)
+  for (size_t k = 0; k < b->jobs.size(); ++k) {
+    const BatchJob& J = b->jobs[k];
+    BatchJob& c = h->jobs[k];
+    c.init_map = J.init_map;
+    c.init_w = J.init_w;
+    c.init_h = J.init_h;
+    c.init_stride = J.init_stride;
+    c.init_scale = J.init_map ? J.init_scale / 2 : 1;
+  }
+  h->init_search = b->init_search;
+  h->init_sigma = b->init_sigma;
+  h->init_sigma_slope = b->init_sigma_slope;
   std::vector<int> sched = pyramid_schedule(b);
   h->iterations = sched.back();
   sched.pop_back();
@@ -390,13 +410,13 @@ void prepare(pm_batch* b, int count) {
   HIP_CHECK(hipMemcpy(b->jobtab.p, jt.data(), jt.size() * sizeof(pm::PmJob),
                       hipMemcpyHostToDevice));
   if (!pyramid_schedule(b).empty()) {
-    // jobs without a caller's map start from the half level's, sampled
-    // bilinearly with no spread (estimator: run_half)
+    // every job starts from the half level's map, sampled bilinearly with
+    // no spread (estimator: run_half); a caller's map seeds the coarsest
+    // level instead
     ensure_half(b, count);
     const pm_batch& h = *b->half;
     b->half_map.ensure(h.total_px * 4 * sizeof(float));
     for (int k = 0; k < count; ++k) {
-      if (b->jobs[k].init_map) continue;
       const PoolView& HR = *h.views[h.jobs[k].ref];
       pm::PmArgs& a = jt[k].a;
       a.init_map = b->half_map.as<float>() + 4 * h.jobs[k].off;

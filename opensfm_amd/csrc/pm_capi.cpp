@@ -172,7 +172,11 @@ int validate_for_compute(pm_estimator* e, bool with_pyramid = false) {
   if (r.width < 2 * hpz + 1 || r.height < 2 * hpz + 1)
     return fail(PM_ERR_INVALID, "reference view smaller than one patch");
   const int sh = (int)pyramid_schedule(e).size();
-  if (with_pyramid && sh > 0 && !e->init_map) {
+  if (with_pyramid && sh > 0 && e->init_map && e->init_scale % (1 << sh) != 0)
+    return fail(PM_ERR_INVALID, "an initialisation map seeding a pyramid of " +
+                                    std::to_string(sh + 1) + " levels needs a scale divisible by " +
+                                    std::to_string(1 << sh));
+  if (with_pyramid && sh > 0) {
     // the coarsest level: every view box-averaged by 2^(levels - 1), the
     // reference at least one patch
     bool ok = (r.width >> sh) >= 2 * hpz + 1 && (r.height >> sh) >= 2 * hpz + 1;
@@ -380,6 +384,16 @@ int run_half(pm_estimator* e, int mode) {
   h->precision = e->precision;
   h->post_process = e->post_process;
   h->consistency = e->consistency;
+  // a caller's initialisation map seeds the coarsest level (its cells cover
+  // half as many pixels of each coarser level)
+  h->init_map = e->init_map;
+  h->init_w = e->init_map ? e->init_w : 0;
+  h->init_h = e->init_map ? e->init_h : 0;
+  h->init_stride = e->init_map ? e->init_stride : 0;
+  h->init_scale = e->init_map ? e->init_scale / 2 : 1;
+  h->init_search = e->init_search;
+  h->init_sigma = e->init_sigma;
+  h->init_sigma_slope = e->init_sigma_slope;
   int rc = run(h, mode, false);  // enqueued on this level's stream
   if (rc) return rc;
   const HostView& r = *h->views[0];
@@ -438,8 +452,7 @@ int run(pm_estimator* e, int mode, bool wait) {
   DeviceGuard dg(e->device);
   e->have_result = false;
   // the half level seeds plain PatchMatch when no initialisation map is set
-  const bool pyramid =
-      mode != PM_MODE_PLANE_SWEEP && !pyramid_schedule(e).empty() && !e->init_map;
+  const bool pyramid = mode != PM_MODE_PLANE_SWEEP && !pyramid_schedule(e).empty();
   if (pyramid) {
     rc = run_half(e, mode);
     if (rc) return rc;

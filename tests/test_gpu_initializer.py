@@ -15,7 +15,8 @@ trained network gains.
 * the batch's network input equals the estimator's bit for bit, and the
   init map's arguments are checked;
 * a batched job seeded with a map equals the single-view estimator seeded
-  with the same map (and an unseeded job equals plain PatchMatch).
+  with the same map (and an unseeded job equals plain PatchMatch), also when
+  the map seeds the coarsest level of a pyramid.
 """
 import numpy as np
 import pytest
@@ -236,3 +237,28 @@ def test_batched_learned_init_equals_estimator(nadir):
     assert min(acc[0], acc[2]) > acc[1]
     with pytest.raises(IndexError):
         b.set_init_map(7, None)
+    # with a pyramid the map seeds the coarsest level (scale 4 over two
+    # coarser levels: one cell per pixel of the 1/4 level), in the batch and
+    # in the estimator alike, bit for bit; 3 coarser levels need a scale
+    # divisible by 8
+    b.set_pyramid((1, 1))
+    b.run()
+    for k, (ref, nb) in enumerate(jobs):
+        e = _est(nadir, ref, nb, seed=seed + k)
+        e.set_patchmatch_iterations(1)
+        e.set_init_search(3)
+        e.set_pyramid((1, 1))
+        if k in maps:
+            out, ow, oh = maps[k]
+            e.set_init_map(out, ow, oh, nn.OUT_CHANNELS, I.SCALE)
+        want = e.compute_patch_match()
+        got = b.result(k)
+        assert all(np.array_equal(x, y) for x, y in zip(got, want)), f"pyramid job {k}"
+        assert (got[0] > 0).mean() > 0.5
+        if k in maps:
+            e.set_pyramid((1, 1, 1))
+            with pytest.raises(ValueError):
+                e.compute_patch_match()
+    b.set_pyramid((1, 1, 1))
+    with pytest.raises(ValueError):
+        b.run()
