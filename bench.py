@@ -408,7 +408,8 @@ def precision_bench(steps=2, seed=3000):
     """Mixed-precision configuration: 16 reference views of 2048x1536 with 10
     neighbours each (the 1600x1200 scene's family on an arc of 26 cameras,
     rendered on the GPU), all in one batch, 5 iterations, with FP32, FP16 and
-    BF16 samples (FP32 NCC sums, FP32 state, FP64 host homographies)."""
+    BF16 samples (FP32 NCC sums, FP32 state, FP64 host homographies); then
+    the same with the headline's pyramid (2, 2, 1 + 2 iterations)."""
     from opensfm_amd import DepthmapBatch, scenes
     log("mixed precision: rendering 26 views of 2048x1536 on the GPU")
     t0 = time.perf_counter()
@@ -426,8 +427,7 @@ def precision_bench(steps=2, seed=3000):
     b.set_seed(seed)
     out = {"workload": "16 reference views 2048x1536, 10 neighbours each, patch 7, "
                        "5 iterations, one batch", "mpix": round(npx / 1e6, 2)}
-    for prec in ("fp32", "fp16", "bf16"):
-        b.set_precision(prec)
+    def timed_run():
         b.run()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -438,10 +438,22 @@ def precision_bench(steps=2, seed=3000):
             d, _, s, _ = b.result(k)
             accs.append(scenes.depth_accuracy(d, sc.views[r].depth, sc.min_depth, sc.max_depth))
             scores.append(float(s[d > 0].mean()))
-        out[prec] = {"mpix_s": round(npx / dt / 1e6, 3), "ms_per_run": round(dt * 1e3, 1),
-                     "accuracy_within_1pct": round(float(np.mean(accs)), 4),
-                     "mean_score": round(float(np.mean(scores)), 4)}
+        return {"mpix_s": round(npx / dt / 1e6, 3), "ms_per_run": round(dt * 1e3, 1),
+                "accuracy_within_1pct": round(float(np.mean(accs)), 4),
+                "mean_score": round(float(np.mean(scores)), 4)}
+
+    for prec in ("fp32", "fp16", "bf16"):
+        b.set_precision(prec)
+        out[prec] = timed_run()
         log(f"precision {prec}: {out[prec]}")
+    b.set_pyramid(WORKLOAD["pyramid"])
+    b.set_patchmatch_iterations(WORKLOAD["full_iterations"])
+    out["pyramid"] = {"coarse_level_iterations": list(WORKLOAD["pyramid"]),
+                      "iterations": WORKLOAD["full_iterations"]}
+    for prec in ("fp32", "fp16", "bf16"):
+        b.set_precision(prec)
+        out["pyramid"][prec] = timed_run()
+        log(f"precision {prec} with the pyramid: {out['pyramid'][prec]}")
     return out
 
 
