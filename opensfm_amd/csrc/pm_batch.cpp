@@ -464,8 +464,13 @@ void enqueue(pm_batch* b, int mode, hipEvent_t* half_ev) {
   HIP_CHECK(pm::launch_fill_u32(b->score.as<unsigned>(), b->total_px, 0u, b->stream));
   HIP_CHECK(pm::launch_fill_u32(b->nghbr.as<unsigned>(), b->total_px, 0u, b->stream));
   // FP32: the random planes of the initialisation and the first iteration
-  // are scored from the u8 quads (pm_types.h, kPrecFp32Quads)
-  const int first_prec = b->precision == PM_PRECISION_FP32 ? pm::kPrecFp32Quads : b->precision;
+  // are scored from the u8 quads (pm_types.h, kPrecFp32Quads); seeded runs
+  // (a pyramid level's map, or any job's own map) start from mostly right
+  // planes, which the coefficients score faster, as the estimator does
+  bool seeded = pyramid;
+  for (int k = 0; k < nj && !seeded; ++k) seeded = b->jobs[k].init_map != nullptr;
+  const int first_prec =
+      b->precision == PM_PRECISION_FP32 && !seeded ? pm::kPrecFp32Quads : b->precision;
   HIP_CHECK(pm::launch_pm_init_batch(jt, nj, b->pix_total, b->patch_size,
                                      first_prec, sample, q, cnt, b->stream));
   int l = 0;
