@@ -9,8 +9,9 @@ uploaded by add_view); each step runs the schedule that reaches the
 reference's 5-iteration accuracy and mean score (checked over seeds 1-5
 against the oracle's recorded figures): a 4-level pyramid (the views
 box-averaged by 8, 4 and 2 with 2, 2 and 1 iterations), each level's planes
-seeding the next, then 2 full-resolution iterations and the median
-post-process, into device-resident output maps.
+seeding the next, then 2 full-resolution iterations with the
+median-consistent re-selection and the median post-process, into
+device-resident output maps.
 Plain PatchMatch at the reference's 5 iterations is timed beside it.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the red/black
@@ -43,7 +44,7 @@ FLOP_PER_SCORING = 160
 WORKLOAD = dict(width=1600, height=1200, neighbours=8, boxes=20,
                 terrain=(1.5, 6.0), min_depth=1.0, max_depth=40.0, patch=7,
                 min_patch_sd=1.0, iterations=5, seed=2000,
-                pyramid=(2, 2, 1), full_iterations=2)
+                pyramid=(2, 2, 1), full_iterations=2, coarse_reselection=False)
 
 
 GOLDEN_CFG2 = os.path.join(HERE, "tests", "golden", "config2_oracle_fullview.json")
@@ -164,7 +165,8 @@ def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, 
                         seed_sets=2):
     """Accuracy reference for the street batch: the oracle's ComputePatchMatch
     at `iterations` and the GPU engine with gpu_schedule = (iterations per
-    coarser level, coarsest first; full iterations) on the same crops (side x side from the
+    coarser level, coarsest first; full iterations; re-selection at the
+    coarser levels) on the same crops (side x side from the
     centre of the first `jobs` reference views, K shifted, all of the job's
     neighbours), for `seed_sets` sets of seeds, one oracle run per host
     thread."""
@@ -221,6 +223,7 @@ def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, 
         e = setup(DepthmapEstimator(), views, s)
         e.set_pyramid(gpu_schedule[0])
         e.set_patchmatch_iterations(gpu_schedule[1])
+        e.set_pyramid_consistency(gpu_schedule[2])
         return e
 
     gpu = [acc(gpu_est(crops[k][0], s).compute_patch_match()[0], k) for k, s in runs]
@@ -238,8 +241,9 @@ def street_crops_oracle(sc, patch, iterations, gpu_schedule, jobs=16, side=160, 
             "oracle_s": round(cpu_s, 1)}
 
 
-# street batch: iterations per coarser level (1/8, 1/4, 1/2), full iterations
-BATCH_SCHEDULE = ((2, 2, 1), 2)
+# street batch: iterations per coarser level (1/8, 1/4, 1/2), full iterations,
+# the median-consistent re-selection at the coarser levels
+BATCH_SCHEDULE = ((2, 2, 1), 2, False)
 
 
 def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
@@ -267,6 +271,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         b.add_job(ref, nb, sc.min_depth, sc.max_depth)
     b.set_pyramid(BATCH_SCHEDULE[0])
     b.set_patchmatch_iterations(BATCH_SCHEDULE[1])
+    b.set_pyramid_consistency(BATCH_SCHEDULE[2])
     b.set_min_patch_sd(1.0)
     b.set_seed(seed)
 
@@ -345,6 +350,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         e.set_depth_range(sc.min_depth, sc.max_depth, 50)
         e.set_pyramid(BATCH_SCHEDULE[0])
         e.set_patchmatch_iterations(BATCH_SCHEDULE[1])
+        e.set_pyramid_consistency(BATCH_SCHEDULE[2])
         e.set_patch_size(patch)
         e.set_min_patch_sd(1.0)
         e.set_seed(seed + k)
@@ -375,6 +381,7 @@ def batch_bench(n_views, steps=3, seed=4000, cpu_baseline=True):
         "views": n_views, "mpix": round(npx / 1e6, 2),
         "pyramid_levels": len(BATCH_SCHEDULE[0]) + 1,
         "coarse_level_iterations": list(BATCH_SCHEDULE[0]), "iterations": BATCH_SCHEDULE[1],
+        "coarse_reselection": BATCH_SCHEDULE[2],
         "plain_reference_iterations": plain3,
         "patch": patch, "tile": TILE_NAMES[tile], "autotune_s": round(tune_s, 2),
         "autotune": b.tune_report,
@@ -448,6 +455,7 @@ def precision_bench(steps=2, seed=3000):
         log(f"precision {prec}: {out[prec]}")
     b.set_pyramid(WORKLOAD["pyramid"])
     b.set_patchmatch_iterations(WORKLOAD["full_iterations"])
+    b.set_pyramid_consistency(WORKLOAD["coarse_reselection"])
     out["pyramid"] = {"coarse_level_iterations": list(WORKLOAD["pyramid"]),
                       "iterations": WORKLOAD["full_iterations"]}
     for prec in ("fp32", "fp16", "bf16"):
@@ -763,6 +771,7 @@ def main():
         the reference's 5-iteration accuracy; else plain PatchMatch at the
         reference's 5 iterations."""
         est.set_pyramid(w["pyramid"] if half else 0)
+        est.set_pyramid_consistency(w["coarse_reselection"])
         est.set_patchmatch_iterations(w["full_iterations"] if half else w["iterations"])
 
     def timed(steps, warmup):
@@ -905,11 +914,12 @@ def main():
                                "patch 7, depth [1, 40], FP32, at the accuracy and mean score of the "
                                "reference's 5 iterations: a 4-level pyramid (1/8, 1/4, 1/2, full "
                                "resolution) with 2, 2, 1 and 2 iterations, each level seeding the "
-                               "next; "
+                               "next, the re-selection at the full level only; "
                                "value = device-resident views to device-resident maps",
                    "width": w["width"], "height": w["height"], "neighbours": w["neighbours"],
                    "patch": w["patch"], "pyramid_levels": len(w["pyramid"]) + 1,
                    "coarse_level_iterations": list(w["pyramid"]),
+                   "coarse_reselection": w["coarse_reselection"],
                    "iterations": w["full_iterations"],
                    "reference_iterations": w["iterations"]},
         "accuracy_within_1pct": round(acc, 4),
@@ -960,6 +970,7 @@ def main():
             for ce, (x0, y0, cw, ch, _) in zip(crop_est, crops):
                 gt = scene.ref.depth[y0:y0 + ch, x0:x0 + cw]
                 ce.set_pyramid(w["pyramid"])
+                ce.set_pyramid_consistency(w["coarse_reselection"])
                 ce.set_patchmatch_iterations(w["full_iterations"])
                 gd = ce.compute_patch_match()[0]
                 acc.append(scenes.depth_accuracy(gd, gt, w["min_depth"], w["max_depth"]))

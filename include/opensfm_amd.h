@@ -147,6 +147,11 @@ int pm_estimator_set_pyramid_levels(pm_estimator* est, int levels);
  * least 1; set_patchmatch_iterations still counts the full level's.
  * set_pyramid and set_pyramid_levels replace the schedule by a uniform one. */
 int pm_estimator_set_pyramid_schedule(pm_estimator* est, const int32_t* iterations, int n);
+/* Median-consistent re-selection at the pyramid's coarser levels (default
+ * 1: as set_consistency; 0: at the full level only).  Measured: on the
+ * 1600x1200 view with 2, 2, 1 + 2 iterations off is 3 % faster at the same
+ * accuracy, but a learned map seeding the coarsest level needs it on. */
+int pm_estimator_set_pyramid_consistency(pm_estimator* est, int enable);
 /* Lanes per pixel of the red/black PatchMatch launches (no counterpart in
  * the reference, whose CPU loop scores the views of a candidate one after
  * another, depthmap.cc:408-419): 0 (default) scores each pixel on 8 lanes,
@@ -541,6 +546,8 @@ int pm_batch_set_pyramid(pm_batch* b, int half_iterations);
 int pm_batch_set_pyramid_levels(pm_batch* b, int levels);
 /* pm_estimator_set_pyramid_schedule for every job. */
 int pm_batch_set_pyramid_schedule(pm_batch* b, const int32_t* iterations, int n);
+/* pm_estimator_set_pyramid_consistency for every job (default 1). */
+int pm_batch_set_pyramid_consistency(pm_batch* b, int enable);
 /* pm_estimator_set_consistency for every job (default 1). */
 int pm_batch_set_consistency(pm_batch* b, int enable);
 int pm_batch_set_post_process(pm_batch* b, int enable);

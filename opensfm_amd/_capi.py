@@ -76,6 +76,7 @@ SIGNATURES = {
     "pm_estimator_set_pyramid": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_pyramid_levels": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_pyramid_schedule": (C.c_int, [_vp, C.POINTER(C.c_int32), C.c_int]),
+    "pm_estimator_set_pyramid_consistency": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_lanes_per_pixel": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_refinement_steps": (C.c_int, [_vp, C.c_int]),
     "pm_estimator_set_exploration": (C.c_int, [_vp, C.c_int]),
@@ -168,6 +169,7 @@ SIGNATURES = {
     "pm_batch_set_pyramid": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_pyramid_levels": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_pyramid_schedule": (C.c_int, [_vp, C.POINTER(C.c_int32), C.c_int]),
+    "pm_batch_set_pyramid_consistency": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_consistency": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_post_process": (C.c_int, [_vp, C.c_int]),
     "pm_batch_set_tile": (C.c_int, [_vp, C.c_int]),

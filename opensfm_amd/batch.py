@@ -89,6 +89,12 @@ class DepthmapBatch:
     def set_seed(self, seed: int) -> None:
         _capi.check(self._lib.pm_batch_set_seed(self._h, int(seed) & (2 ** 64 - 1)))
 
+    def set_pyramid_consistency(self, enable: bool) -> None:
+        """The median-consistent re-selection at the pyramid's coarser
+        levels (default on, as set_consistency; off: at the full level
+        only)."""
+        _capi.check(self._lib.pm_batch_set_pyramid_consistency(self._h, int(bool(enable))))
+
     def set_pyramid(self, half_iterations, levels: int = 2) -> None:
         """Plain PatchMatch of every job seeded by half_iterations iterations
         at half resolution (0: off), with levels - 1 coarser levels, or a

@@ -85,6 +85,8 @@ struct pm_batch {
   DevBuf half_map;  // the half level's state as initialisation maps
   DevBuf jobtab_pyr;  // the job table with those maps (plain mode)
   bool own_stream = true;
+  int top_consistency = -1;  // a coarser level: the full level's setting
+  int pyramid_consistency = 1;  // the re-selection at the coarser levels too
   std::vector<std::unique_ptr<PoolView>> views;
   std::vector<BatchJob> jobs;
   // device state of the prepared configuration
@@ -248,12 +250,15 @@ void ensure_half(pm_batch* b, int count) {
   h->pyramid_iters = 0;
   h->pyramid_levels = (int)sched.size() + 1;
   h->min_patch_variance = b->min_patch_variance;
+  const int top = b->top_consistency >= 0 ? b->top_consistency : b->consistency;
+  h->top_consistency = top;
+  h->pyramid_consistency = b->pyramid_consistency;
   h->seed = b->seed + pm::kHalfLevelSeed;
   h->n_far = b->n_far;
   h->n_refine = b->n_refine;
   h->n_explore = b->n_explore;
   h->post_process = b->post_process;
-  h->consistency = b->consistency;
+  h->consistency = top && b->pyramid_consistency;  // estimator: run_half
   h->tile = b->tile;
   h->precision = b->precision;
   h->use_graph = 0;
@@ -710,6 +715,13 @@ int pm_batch_add_job(pm_batch* b, int ref_view, const int* neighbours,
   J.max_depth = max_depth;
   if (job_id) *job_id = (int)b->jobs.size();
   b->jobs.push_back(std::move(J));
+  b->invalidate();
+  return PM_OK;
+}
+
+int pm_batch_set_pyramid_consistency(pm_batch* b, int enable) {
+  if (!b) return fail(PM_ERR_INVALID, "null batch");
+  b->pyramid_consistency = enable ? 1 : 0;
   b->invalidate();
   return PM_OK;
 }

@@ -97,6 +97,8 @@ struct pm_estimator {
   std::unique_ptr<pm_estimator> half;
   DevBuf half_map;  // the half level's state as an initialisation map
   bool own_stream = true;  // a pyramid level runs on its parent's stream
+  int top_consistency = -1;  // a coarser level: the full level's setting
+  int pyramid_consistency = 1;  // the re-selection at the coarser levels too
   // a compute enqueued and not yet waited for (run with wait = false): what
   // its statistics need once the stream has drained
   struct {
@@ -383,7 +385,12 @@ int run_half(pm_estimator* e, int mode) {
   h->n_explore = e->n_explore;
   h->precision = e->precision;
   h->post_process = e->post_process;
-  h->consistency = e->consistency;
+  // the re-selection at a coarser level: the full level's setting unless
+  // set_pyramid_consistency(0)
+  const int top = e->top_consistency >= 0 ? e->top_consistency : e->consistency;
+  h->top_consistency = top;
+  h->pyramid_consistency = e->pyramid_consistency;
+  h->consistency = top && e->pyramid_consistency;
   // a caller's initialisation map seeds the coarsest level (its cells cover
   // half as many pixels of each coarser level)
   h->init_map = e->init_map;
@@ -973,6 +980,12 @@ int pm_estimator_set_lanes_per_pixel(pm_estimator* e, int lanes) {
     return fail(PM_ERR_INVALID, "lanes per pixel must be 0, 1 or " +
                                     std::to_string(pm::kLanesPerPixelSmall));
   e->lanes_per_pixel = lanes;
+  return PM_OK;
+}
+
+int pm_estimator_set_pyramid_consistency(pm_estimator* e, int enable) {
+  if (!e) return fail(PM_ERR_INVALID, "null estimator");
+  e->pyramid_consistency = enable ? 1 : 0;
   return PM_OK;
 }
 

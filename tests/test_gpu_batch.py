@@ -99,6 +99,36 @@ def test_batch_pyramid_equals_single_view_runs(street, levels):
         assert _equal(b.result(k), want), f"sample job {k}"
 
 
+def test_batch_pyramid_without_coarse_reselection(street):
+    """set_pyramid_consistency(False): the coarser levels skip the
+    median-consistent re-selection; job k still equals the estimator with the
+    same setting bit for bit, and the maps differ from the default's."""
+    print("batch vs estimator, pyramid without the coarse levels' re-selection", flush=True)
+    b = _batch(street)
+    b.set_pyramid((2, 1))
+    b.run()
+    with_coarse = [b.result(k) for k in range(len(street.jobs))]
+    b.set_pyramid_consistency(False)
+    b.run()
+    differs = False
+    for k, (ref, nb) in enumerate(street.jobs):
+        e = DepthmapEstimator()
+        for v in [ref] + list(nb):
+            V = street.views[v]
+            e.add_view(V.K, V.R, V.t, V.image, V.mask)
+        e.set_depth_range(street.min_depth, street.max_depth, 50)
+        e.set_patchmatch_iterations(2)
+        e.set_patch_size(7)
+        e.set_min_patch_sd(1.0)
+        e.set_seed(SEED + k)
+        e.set_pyramid((2, 1))
+        e.set_pyramid_consistency(False)
+        got = b.result(k)
+        assert _equal(got, e.compute_patch_match()), f"job {k}"
+        differs |= not _equal(got, with_coarse[k])
+    assert differs
+
+
 def test_batch_graph_replays_and_eager_agree(street):
     print("batch graph replay vs eager", flush=True)
     e = _batch(street, graph=False)
